@@ -1,0 +1,325 @@
+/*
+ * montepath_hip.h -- C-ABI of the MI355X (gfx950) Monte Carlo path engine,
+ * the `rocm_hip` backend of montepath.
+ *
+ * Plain C: integers, doubles and caller-owned pointers only.  Every entry
+ * point returns an mp_status; no exception or abort crosses the boundary.
+ * The library never keeps a caller pointer past return, except that
+ * mp_hip_submit copies the request (and its exercise schedule) it is given.
+ *
+ * One mp_ctx owns one HIP stream and the device arenas resident on it (RNG
+ * descriptors, LSM path/cash-flow buffers, reduction scratch).  A context
+ * is single-thread-affine, like the reference's thread-local Metal runtime
+ * cache (crates/mc-core/src/backend/metal.rs:52-55): use one context per
+ * host thread.
+ *
+ * Reference interfaces this ABI replaces are cited per entry point
+ * (paths relative to the montepath repository).
+ */
+#ifndef MONTEPATH_HIP_H
+#define MONTEPATH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MP_ABI_VERSION 1
+
+/* Status codes.  Mapping onto the reference's BackendError
+ * (crates/mc-core/src/backend/mod.rs:186-194):
+ *   MP_E_UNKNOWN_DEVICE      -> BackendError::UnknownDevice
+ *   MP_E_INCOMPATIBLE_INPUT  -> BackendError::IncompatibleExecutionInput
+ *   MP_E_UNSUPPORTED, MP_E_OOM, MP_E_HIP -> BackendError::UnsupportedFeature(msg)
+ *   MP_E_INVALID_CONFIG      -> the assert! panics of the reference's config
+ *                               validators (crates/mc-core/src/runtime/cpu.rs:5010-5153)
+ *   MP_E_NOT_FOUND           -> unknown serving ticket
+ *   MP_E_CANCELLED           -> a serving request cancelled before it ran */
+typedef enum mp_status {
+    MP_OK = 0,
+    MP_E_UNKNOWN_DEVICE = 1,
+    MP_E_INCOMPATIBLE_INPUT = 2,
+    MP_E_UNSUPPORTED = 3,
+    MP_E_OOM = 4,
+    MP_E_HIP = 5,
+    MP_E_INVALID_CONFIG = 6,
+    MP_E_NOT_FOUND = 7,
+    MP_E_CANCELLED = 8
+} mp_status;
+
+/* Workload families (reference: PricingWorkloadFamily and the
+ * BackendExecutionInput variants, backend/mod.rs:179-184, extended with the
+ * families the reference prices only on CPU). */
+typedef enum mp_workload {
+    MP_EUROPEAN_CALL = 1,
+    MP_ARITHMETIC_ASIAN_CALL = 2,
+    MP_DOWN_AND_OUT_CALL = 3,
+    MP_LOOKBACK_CALL = 4,
+    MP_HESTON_EUROPEAN_CALL = 5,
+    MP_MERTON_JUMP_DIFFUSION_CALL = 6,
+    MP_BASKET_CALL = 7,
+    MP_AMERICAN_PUT = 8,
+    MP_BERMUDAN_PUT = 9
+} mp_workload;
+
+/* Sampling (reference SamplingMethod, runtime/cpu.rs).  Halton and Latin
+ * hypercube are not offered by this engine (MP_E_UNSUPPORTED). */
+typedef enum mp_sampling {
+    MP_PSEUDORANDOM = 0,
+    MP_SCRAMBLED_SOBOL = 3,
+    MP_SCRAMBLED_SOBOL_BRIDGE = 4
+} mp_sampling;
+
+/* European method (reference EuropeanCallMethod): the reference's Auto and
+ * TerminalDistribution sample S_T directly; StepwisePaths walks n_steps. */
+typedef enum mp_method {
+    MP_METHOD_STEPWISE = 0,
+    MP_METHOD_TERMINAL = 1
+} mp_method;
+
+typedef enum mp_heston_scheme {
+    MP_HESTON_REFERENCE_FLOOR = 0,  /* variance floored after each update, cpu.rs:8682 */
+    MP_HESTON_FULL_TRUNCATION = 1   /* Lord et al. (2010): only v+ enters drift/diffusion */
+} mp_heston_scheme;
+
+#define MP_MAX_STRIKES 4
+#define MP_MAX_ASSETS 8
+
+/* One pricing request.  Zero-initialise, then fill what the workload uses.
+ * Field meanings follow the reference configs (EuropeanCallConfig,
+ * HestonEuropeanCallConfig, MertonJumpDiffusionCallConfig, BasketCallConfig,
+ * AmericanPutConfig, BermudanPutConfig; runtime/cpu.rs:1296-1860). */
+typedef struct mp_request {
+    int32_t workload;        /* mp_workload */
+    int32_t precision;       /* 32: FP32 paths + FP64 accumulation; 64: FP64 throughout */
+    int32_t sampling;        /* mp_sampling */
+    int32_t technique;       /* 0 = standard (the only technique offered) */
+    int32_t method;          /* mp_method (European only) */
+    int32_t heston_scheme;   /* mp_heston_scheme */
+    int32_t n_strikes;       /* 0 or 1: price strike k; 2..4: price strikes[] from one simulation */
+    int32_t basis_degree;    /* LSM Laguerre degree, clamped to [1,3] like the reference */
+    int32_t fp64_check;      /* nonzero: also run FP64 on the same draws, report fp64_ref_error */
+    int32_t allow_intrinsic_now; /* American put floor at max(K - S0, 0) (cpu.rs:3911-3917) */
+    int32_t n_assets;        /* basket */
+    int32_t n_exercise;      /* Bermudan: entries in exercise_steps */
+    uint64_t n_paths;
+    uint64_t n_steps;
+    uint64_t seed;
+    uint64_t contract_id;    /* Philox key word: (seed, contract_id) names the stream */
+    double s0, k, r, sigma, t;
+    double barrier;                     /* down-and-out */
+    double strikes[MP_MAX_STRIKES];     /* multi-strike European/Heston */
+    double v0, kappa, theta, xi, rho;   /* Heston (xi = vol_of_vol) */
+    double jump_intensity, jump_mean, jump_volatility; /* Merton */
+    double asset_s0[MP_MAX_ASSETS];
+    double asset_sigma[MP_MAX_ASSETS];
+    double asset_weight[MP_MAX_ASSETS];
+    double correlation[MP_MAX_ASSETS * MP_MAX_ASSETS]; /* row-major n_assets x n_assets */
+    const uint32_t* exercise_steps;     /* Bermudan schedule, 1..n_steps */
+} mp_request;
+
+/* Result of one request (reference RunOutput {price, stderr, runtime_ms},
+ * backend/mod.rs:172-177, plus the per-strike and LSM details the
+ * reference returns in its typed results). */
+typedef struct mp_result {
+    int32_t status;          /* mp_status of this request */
+    int32_t n_strikes;
+    double price;
+    double std_error;        /* population-variance standard error (cpu.rs:7641-7652) */
+    double runtime_ms;
+    double fp64_ref_error;   /* |price - price of an FP64 run on the same draws|; NaN if not run */
+    double strike_price[MP_MAX_STRIKES];
+    double strike_std_error[MP_MAX_STRIKES];
+    uint64_t n_paths;
+    uint64_t n_steps;
+    uint64_t early_exercise_count;    /* LSM (AmericanPutResult fields) */
+    uint64_t maturity_exercise_count;
+    uint64_t regression_steps;
+} mp_result;
+
+/* Device description (reference DeviceInfo, backend/mod.rs:32-43). */
+typedef struct mp_device_info {
+    char device_id[32];      /* "rocm:<ordinal>" */
+    char name[96];
+    char arch[32];           /* e.g. "gfx950" */
+    uint64_t memory_total_mb;
+    uint64_t memory_free_mb;
+    int32_t compute_units;
+    int32_t supports_float64;
+    int32_t max_threads_hint;
+    int32_t wavefront_size;
+} mp_device_info;
+
+typedef struct mp_ctx mp_ctx;
+
+/* Library version string. */
+const char* mp_hip_version(void);
+
+/* Number of visible HIP devices (reference RuntimeBackend::discover_devices,
+ * backend/mod.rs:200; CUDA's nvidia-smi probe, backend/cuda.rs:205-241). */
+int mp_hip_device_count(int* count);
+
+/* Open a context on one device: stream, arenas, Sobol tables (reference
+ * RuntimeBackend::compile, backend/mod.rs:203-207, and the Metal runtime
+ * cache set-up, backend/metal.rs:686-751). */
+int mp_hip_open(int device_ordinal, mp_ctx** out);
+
+/* Release the context and every device buffer it owns. */
+void mp_hip_close(mp_ctx* ctx);
+
+/* Fill mp_device_info (reference DeviceInfo, backend/mod.rs:32-43). */
+int mp_hip_device_info(mp_ctx* ctx, mp_device_info* out);
+
+/* Price one request on the GPU, blocking (reference RuntimeBackend::execute,
+ * backend/mod.rs:208-212, and the strict native entry
+ * execute_apple_metal_native, backend/metal.rs:142-174: never falls back to
+ * the CPU). */
+int mp_hip_price(mp_ctx* ctx, const mp_request* req, mp_result* out);
+
+/* Price n independent requests in one persistent launch (work queue of
+ * (request, path-tile) items); results are bit-identical to pricing each
+ * request alone.  No reference counterpart: the reference executes one
+ * input per call (backend/mod.rs:208-212). */
+int mp_hip_price_batch(mp_ctx* ctx, const mp_request* reqs, size_t n, mp_result* out);
+
+/* Prepared batch: requests validated, packed and uploaded once (descriptors,
+ * tile map and queue order resident in HBM), optionally captured as a
+ * hipGraph (queue reset + persistent kernel + finalize) and replayed on its
+ * own stream.  launch only enqueues; sync waits; results copies back.  GBM,
+ * Heston and Merton pseudorandom requests of one precision.  No reference
+ * counterpart (see mp_hip_price_batch). */
+typedef struct mp_batch mp_batch;
+int mp_hip_batch_prepare(mp_ctx* ctx, const mp_request* reqs, size_t n, int use_graph,
+                         mp_batch** out);
+int mp_hip_batch_launch(mp_batch* batch);
+int mp_hip_batch_sync(mp_batch* batch);
+int mp_hip_batch_results(mp_batch* batch, mp_result* out);
+int mp_hip_batch_info(mp_batch* batch, uint64_t* n_tiles, uint64_t* path_steps, int* grid,
+                      int* block);
+void mp_hip_batch_free(mp_batch* batch);
+
+/* Plain FP64 comparator: one launch per request, one lane per path, FP64
+ * normals and arithmetic, per-path payoffs written to HBM and reduced by a
+ * second kernel -- the shape of the reference's CUDA kernel contract
+ * (backend/cuda.rs:247-310, kernels/european_call_stepwise_v1.cu).
+ * GBM European/Asian requests only. */
+int mp_hip_price_plain_fp64(mp_ctx* ctx, const mp_request* req, mp_result* out);
+
+/* Parity hook: run the request's path arithmetic on caller-supplied normal
+ * draws instead of the engine's generator ("identical normal buffers").
+ * normals are step-major: normals[(step * F + f) * n_paths + path] with F
+ * factors (1 for GBM, 2 for Heston as (z_var, z_ind), n_assets for a
+ * basket).  For Merton, uniforms and jump_normals (same layout, F = 1) carry
+ * the Poisson uniforms and jump-size normals.  Writes terminal values (basket
+ * value for a basket) and undiscounted payoffs per path for strike 0.
+ * Reference counterpart: the same-normals test hooks
+ * *_from_f32_normals (runtime/cpu.rs:5434-5652). */
+int mp_hip_paths_from_normals(mp_ctx* ctx, const mp_request* req, const double* normals,
+                              const double* uniforms, const double* jump_normals,
+                              double* terminal_out, double* payoff_out);
+
+/* Price one request and also return, per path, the terminal value (basket
+ * value for a basket) and the undiscounted payoff for strike 0, from the
+ * engine's own draws (Philox or Sobol).  Same kernels as mp_hip_price. */
+int mp_hip_price_with_paths(mp_ctx* ctx, const mp_request* req, mp_result* out,
+                            double* terminal_out, double* payoff_out);
+
+/* Parity hook for LSM: price an American/Bermudan put from caller-supplied
+ * normals (step-major, n_steps x n_paths); coefficients_out (n_steps x 4,
+ * may be NULL) receives the regression coefficients per date (zero where a
+ * date was skipped) and exercise_out (n_paths, may be NULL) the exercise
+ * step per path.  Reference: price_put_lsm_cpu, runtime/cpu.rs:3813-3945. */
+int mp_hip_lsm_from_normals(mp_ctx* ctx, const mp_request* req, const double* normals,
+                            mp_result* out, double* coefficients_out, uint32_t* exercise_out);
+
+/* Debug/parity hook: the engine's own draws for paths [path0, path0+n_paths)
+ * of a request, written path-major ([path][n_draws]).  kind 0: Philox
+ * normals of stream `stream`; kind 1: Philox uniforms; kind 2: scrambled
+ * Sobol standard normals of the point (n_draws dims, reference block/seed
+ * rules); kind 3: raw scrambled Sobol 32-bit integers as doubles. */
+int mp_hip_draws(mp_ctx* ctx, const mp_request* req, int kind, uint32_t stream, uint64_t path0,
+                 uint64_t n_paths, uint64_t n_draws, double* out);
+
+/* HBM bytes a request needs while in flight (reference
+ * estimate_gpu_bytes_per_path / plan_gpu_chunking, backend/mod.rs:372-405). */
+int mp_hip_request_footprint(const mp_request* req, uint64_t* bytes);
+
+/* ---- serving: continuous batching on one device -------------------------
+ * Requests are admitted by HBM footprint as slots free up; admitted work is
+ * packed into persistent launches on up to `max_inflight` concurrent
+ * streams while later requests keep arriving.  No reference counterpart
+ * (the reference is synchronous; production.py:440-481). */
+typedef struct mp_serve_config {
+    uint64_t hbm_budget_bytes;   /* admission budget (0: 75% of free memory) */
+    int32_t max_inflight;        /* concurrent launches in flight (streams), 1..8 */
+    int32_t max_batch;           /* requests per launch, upper bound */
+} mp_serve_config;
+
+typedef struct mp_server mp_server;
+
+int mp_hip_server_open(mp_ctx* ctx, const mp_serve_config* cfg, mp_server** out);
+void mp_hip_server_close(mp_server* srv);
+/* Queue a request (copied); returns a ticket. */
+int mp_hip_submit(mp_server* srv, const mp_request* req, uint64_t* ticket);
+/* Non-blocking: *done = 1 and *out filled once the ticket has finished. */
+int mp_hip_poll(mp_server* srv, uint64_t ticket, mp_result* out, int* done);
+/* Blocking wait for a ticket. */
+int mp_hip_wait(mp_server* srv, uint64_t ticket, mp_result* out);
+/* Cancel a ticket that has not been launched yet. */
+int mp_hip_cancel(mp_server* srv, uint64_t ticket);
+
+/* Replay an arrival trace in real time through a server and report latency
+ * (arrival -> result) per request.  arrival_s[i] is seconds from start. */
+typedef struct mp_serve_stats {
+    uint64_t completed;
+    double wall_s;
+    double requests_per_s;
+    double path_steps_per_s;
+    double p50_ms;
+    double p99_ms;
+    double max_ms;
+    uint64_t launches;
+    uint64_t max_admitted_bytes;
+} mp_serve_stats;
+
+int mp_hip_serve_trace(mp_ctx* ctx, const mp_serve_config* cfg, const mp_request* reqs,
+                       const double* arrival_s, size_t n, mp_result* results,
+                       double* latency_ms, mp_serve_stats* stats);
+
+/* ---- tuning --------------------------------------------------------------
+ * Kernel launch parameters: block size, paths interleaved per lane, paths
+ * per work-queue tile.  mp_hip_autotune times the candidates on a sample of
+ * the request and keeps the fastest per (model, payoff, steps bucket). */
+typedef struct mp_tuning {
+    int32_t block_size;
+    int32_t paths_per_lane;
+    int32_t tile_paths;
+    int32_t reserved;
+} mp_tuning;
+
+int mp_hip_autotune(mp_ctx* ctx, const mp_request* req, mp_tuning* best, double* best_ms);
+int mp_hip_get_tuning(mp_ctx* ctx, const mp_request* req, mp_tuning* out);
+int mp_hip_set_tuning(mp_ctx* ctx, const mp_request* req, const mp_tuning* t);
+
+/* Time of the last mp_hip_price / mp_hip_price_batch kernel region measured
+ * with HIP events on the context stream, per kernel role. */
+typedef struct mp_timing {
+    double total_ms;        /* whole request on the stream */
+    double main_kernel_ms;  /* the path-simulation kernel(s) */
+    double finalize_ms;
+    uint64_t main_launches;
+} mp_timing;
+
+int mp_hip_last_timing(mp_ctx* ctx, mp_timing* out);
+
+/* Thread-local message for the last failing call on this thread; valid until
+ * the next call. */
+const char* mp_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MONTEPATH_HIP_H */

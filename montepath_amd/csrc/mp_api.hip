@@ -1,0 +1,1721 @@
+// mp_api.hip -- C-ABI host layer of the montepath HIP engine.
+//
+// Validates requests the way the reference's config validators do
+// (crates/mc-core/src/runtime/cpu.rs:5010-5153) but returns status codes
+// instead of panicking, builds per-request device descriptors, packs
+// requests into work-queue launches, drives the LSM date loop, and runs the
+// continuous-batching server.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "montepath_hip.h"
+#include "mp_device.h"
+#include "mp_engine.h"
+#include "sobol_joe_kuo_table.h"
+
+#define MP_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(MP_E_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+// compiler-rt __powidf2: the lowering of Rust's f64::powi used by the reference
+double powi_ref(double a, int b) {
+    const bool recip = b < 0;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+uint64_t pow2_floor(uint64_t x) {
+    uint64_t p = 1;
+    while (p * 2 <= x) p *= 2;
+    return p;
+}
+
+// Device buffer that grows on demand and is reused across calls.
+struct dbuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~dbuf() {
+        if (p) (void)hipFree(p);
+    }
+    int ensure(size_t bytes) {
+        if (bytes == 0) bytes = 8;
+        if (cap >= bytes) return MP_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = bytes + bytes / 4;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return fail(MP_E_OOM, "device allocation of " + std::to_string(want) + " bytes failed");
+        }
+        cap = want;
+        return MP_OK;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+#define ENSURE(buf, bytes)                      \
+    do {                                        \
+        int rc_ = (buf).ensure(bytes);          \
+        if (rc_ != MP_OK) return rc_;           \
+    } while (0)
+
+enum req_kind { KIND_TILE = 0, KIND_QMC = 1, KIND_BASKET = 2, KIND_LSM = 3 };
+
+struct prepared {
+    mp_dreq d;
+    int kind;
+    int precision;
+    int factors;       // QMC
+    int bridge;        // QMC
+    int n_assets;      // basket
+    std::vector<uint32_t> schedule;  // LSM dates
+    int allow_intrinsic;
+    int degree;
+    uint64_t n_steps_cfg;  // steps as configured (terminal method walks 1)
+};
+
+}  // namespace
+
+struct mp_batch;
+
+struct mp_ctx {
+    int device = 0;
+    mp_batch* bat[2] = {nullptr, nullptr};  // reusable FP32 / FP64 batches on the context stream
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    size_t max_lds = 0;
+    std::string name, arch;
+    uint32_t dirs[256][32];
+    uint32_t* d_nib = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    mp_timing last{};
+    std::map<uint64_t, mp_tuning> tuning;
+    // arenas
+    dbuf reqs, tile_req, order, partial, out, counter;
+    dbuf normals, uniforms, jumps, term, pay;
+    dbuf plan, basket_par;
+    dbuf lsm_paths, lsm_cash, lsm_ex, lsm_coeff, lsm_valid, lsm_gram, lsm_disc, lsm_grow,
+        lsm_part, lsm_out, plain_pay;
+    dbuf draws;
+};
+
+namespace {
+
+// 32-bit direction numbers from the Joe-Kuo primitive polynomials and
+// initial values (Bratley-Fox recurrence).
+void sobol_directions(uint32_t dirs[256][32]) {
+    for (int d = 0; d < 256; ++d) {
+        const uint32_t poly = mp_sobol_poly[d];
+        int s = 0;
+        while ((poly >> (s + 1)) != 0) ++s;
+        uint64_t m[33];
+        if (d == 0) {
+            for (int k = 1; k <= 32; ++k) m[k] = 1;
+        } else {
+            for (int k = 1; k <= s; ++k) m[k] = mp_sobol_vinit[d][k - 1];
+            for (int k = s + 1; k <= 32; ++k) {
+                uint64_t v = m[k - s] ^ (m[k - s] << s);
+                for (int i = 1; i < s; ++i)
+                    if ((poly >> (s - i)) & 1u) v ^= m[k - i] << i;
+                m[k] = v;
+            }
+        }
+        for (int k = 1; k <= 32; ++k) dirs[d][k - 1] = (uint32_t)(m[k] << (32 - k));
+    }
+}
+
+// Nibble tables: nib[d][j][n] = XOR of direction numbers 4j..4j+3 selected
+// by the bits of n, so a 16-bit index needs four lookups.
+void sobol_nibbles(const uint32_t dirs[256][32], std::vector<uint32_t>& nib) {
+    nib.assign(256 * 64, 0u);
+    for (int d = 0; d < 256; ++d)
+        for (int j = 0; j < 4; ++j)
+            for (int n = 0; n < 16; ++n) {
+                uint32_t x = 0;
+                for (int b = 0; b < 4; ++b)
+                    if ((n >> b) & 1) x ^= dirs[d][4 * j + b];
+                nib[d * 64 + j * 16 + n] = x;
+            }
+}
+
+// Streaming Brownian-bridge plan (preorder of the reference's
+// BrownianBridgePlan::new, cpu.rs:5923-5961) with stack slots.
+int bridge_plan(uint64_t n_steps, double maturity, std::vector<mp_bbop>& ops, int& n_slots) {
+    ops.clear();
+    n_slots = 1;
+    const double dt = maturity / (double)n_steps;
+    struct item {
+        uint64_t l, r;
+        int slot;
+    };
+    std::vector<item> st;
+    st.push_back({0, n_steps, 0});
+    while (!st.empty()) {
+        item it = st.back();
+        st.pop_back();
+        if (it.r <= it.l + 1) continue;
+        const uint64_t mid = (it.l + it.r) / 2;
+        const double tl = (double)it.l * dt, tm = (double)mid * dt, tr = (double)it.r * dt;
+        const double denom = tr - tl;
+        mp_bbop op;
+        op.lw = (tr - tm) / denom;
+        op.rw = (tm - tl) / denom;
+        op.sd = std::sqrt((tm - tl) * (tr - tm) / denom);
+        op.slot = it.slot;
+        const bool left_leaf = mid <= it.l + 1;
+        const bool right_leaf = it.r <= mid + 1;
+        op.flags = (left_leaf ? 1 : 0) | (right_leaf ? 2 : 0);
+        ops.push_back(op);
+        n_slots = std::max(n_slots, it.slot + (left_leaf ? 1 : 2));
+        st.push_back({mid, it.r, it.slot});
+        st.push_back({it.l, mid, it.slot + 1});
+    }
+    return MP_OK;
+}
+
+bool finite(double x) { return std::isfinite(x); }
+
+// Paths per work-queue tile: a pure function of the request, so a request's
+// tiles -- and hence its FP64 partial sums -- are the same alone and in a mix.
+uint32_t tile_paths_for(uint64_t n_paths, uint64_t n_steps) {
+    const uint64_t by_work = (1ull << 17) / std::max<uint64_t>(n_steps, 1);
+    const uint64_t by_count = n_paths / 256;
+    uint64_t t = pow2_floor(std::max<uint64_t>(1, std::min(by_work, by_count)));
+    t = std::min<uint64_t>(std::max<uint64_t>(t, 64), 1024);
+    return (uint32_t)t;
+}
+
+int prepare_request(const mp_request& r, prepared& P) {
+    std::memset(&P.d, 0, sizeof(P.d));
+    P.schedule.clear();
+    P.precision = r.precision == 64 ? 64 : 32;
+    if (r.precision != 0 && r.precision != 32 && r.precision != 64)
+        return fail(MP_E_INVALID_CONFIG, "precision must be 32 or 64");
+    if (r.technique != 0)
+        return fail(MP_E_UNSUPPORTED, "only the standard Monte Carlo technique is offered on rocm_hip");
+    if (r.sampling != MP_PSEUDORANDOM && r.sampling != MP_SCRAMBLED_SOBOL &&
+        r.sampling != MP_SCRAMBLED_SOBOL_BRIDGE)
+        return fail(MP_E_UNSUPPORTED, "sampling method not offered on rocm_hip");
+    if (r.n_paths == 0) return fail(MP_E_INVALID_CONFIG, "n_paths must be > 0");
+    if (r.n_paths > 0xFFFFFFFFull)
+        return fail(MP_E_UNSUPPORTED, "n_paths above 2^32 per request is not supported");
+    const double s0 = r.s0, k = r.k, rr = r.r, sig = r.sigma, t = r.t;
+    if (!finite(s0) || !finite(k) || !finite(rr) || !finite(sig) || !finite(t))
+        return fail(MP_E_INVALID_CONFIG, "non-finite market parameter");
+    mp_dreq& d = P.d;
+    d.n_paths = r.n_paths;
+    d.key0 = (uint32_t)r.seed;
+    d.key1 = (uint32_t)r.contract_id;
+    d.seed_hi = (uint32_t)(r.seed >> 32);
+    d.s0 = s0;
+    d.log_s0 = std::log(s0);
+    d.rate = rr;
+    d.discount = std::exp(-rr * t);
+    d.log_barrier_rel = -1e300;
+    // strikes
+    if (r.n_strikes < 0 || r.n_strikes > MP_MAX_STRIKES)
+        return fail(MP_E_INVALID_CONFIG, "n_strikes must be in 0..4");
+    if (r.n_strikes <= 1) {
+        d.n_strikes = 1;
+        d.strikes[0] = k;
+    } else {
+        d.n_strikes = r.n_strikes;
+        for (int i = 0; i < r.n_strikes; ++i) {
+            if (!(r.strikes[i] >= 0.0)) return fail(MP_E_INVALID_CONFIG, "strikes must be >= 0");
+            d.strikes[i] = r.strikes[i];
+        }
+    }
+    P.kind = KIND_TILE;
+    P.factors = 1;
+    P.bridge = r.sampling == MP_SCRAMBLED_SOBOL_BRIDGE;
+    P.n_steps_cfg = r.n_steps;
+    const bool qmc = r.sampling != MP_PSEUDORANDOM;
+
+    switch (r.workload) {
+        case MP_EUROPEAN_CALL:
+        case MP_ARITHMETIC_ASIAN_CALL:
+        case MP_DOWN_AND_OUT_CALL:
+        case MP_LOOKBACK_CALL: {
+            if (r.n_steps == 0) return fail(MP_E_INVALID_CONFIG, "n_steps must be > 0");
+            if (r.workload == MP_EUROPEAN_CALL) {
+                // validate_european_call_config cpu.rs:5010-5017
+                if (!(s0 > 0.0)) return fail(MP_E_INVALID_CONFIG, "s0 must be > 0");
+                if (!(k > 0.0)) return fail(MP_E_INVALID_CONFIG, "k must be > 0");
+                if (!(sig > 0.0)) return fail(MP_E_INVALID_CONFIG, "sigma must be > 0");
+                if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
+            } else if (r.workload == MP_LOOKBACK_CALL) {
+                // validate_lookback_call_config
+                if (!(s0 > 0.0)) return fail(MP_E_INVALID_CONFIG, "s0 must be > 0");
+                if (!(k >= 0.0)) return fail(MP_E_INVALID_CONFIG, "k must be >= 0");
+                if (!(sig >= 0.0)) return fail(MP_E_INVALID_CONFIG, "sigma must be >= 0");
+                if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
+            } else {
+                if (!(s0 > 0.0)) return fail(MP_E_INVALID_CONFIG, "s0 must be > 0");
+                if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
+                if (!(sig >= 0.0)) return fail(MP_E_INVALID_CONFIG, "sigma must be >= 0");
+            }
+            if (r.workload == MP_DOWN_AND_OUT_CALL) {
+                if (!(r.barrier > 0.0)) return fail(MP_E_INVALID_CONFIG, "barrier must be > 0");
+                d.log_barrier_rel = std::log(r.barrier) - d.log_s0;
+                d.knocked_at_start = s0 <= r.barrier;
+            }
+            d.model = MP_MODEL_GBM;
+            d.payoff = r.workload == MP_EUROPEAN_CALL ? MP_PAY_EUROPEAN
+                       : r.workload == MP_ARITHMETIC_ASIAN_CALL ? MP_PAY_ASIAN
+                       : r.workload == MP_DOWN_AND_OUT_CALL ? MP_PAY_DOWN_AND_OUT
+                                                              : MP_PAY_LOOKBACK;
+            uint64_t m = r.n_steps;
+            if (r.workload == MP_EUROPEAN_CALL && r.method == MP_METHOD_TERMINAL) m = 1;
+            if (m > 0x7FFFFFFFull) return fail(MP_E_UNSUPPORTED, "n_steps too large");
+            d.n_steps = (int32_t)m;
+            d.dt = t / (double)m;
+            d.sqrt_dt = std::sqrt(d.dt);
+            d.drift_dt = (rr - 0.5 * sig * sig) * d.dt;
+            d.vol_dt = sig * std::sqrt(d.dt);
+            if (qmc) {
+                P.kind = KIND_QMC;
+                P.factors = 1;
+            }
+            break;
+        }
+        case MP_HESTON_EUROPEAN_CALL: {
+            // validate_heston_european_call_config cpu.rs (Heston block)
+            if (r.n_steps == 0) return fail(MP_E_INVALID_CONFIG, "n_steps must be > 0");
+            if (!(s0 > 0.0)) return fail(MP_E_INVALID_CONFIG, "s0 must be > 0");
+            if (!(k >= 0.0)) return fail(MP_E_INVALID_CONFIG, "k must be >= 0");
+            if (!(r.v0 >= 0.0)) return fail(MP_E_INVALID_CONFIG, "v0 must be >= 0");
+            if (!(r.kappa >= 0.0)) return fail(MP_E_INVALID_CONFIG, "kappa must be >= 0");
+            if (!(r.theta >= 0.0)) return fail(MP_E_INVALID_CONFIG, "theta must be >= 0");
+            if (!(r.xi >= 0.0)) return fail(MP_E_INVALID_CONFIG, "vol_of_vol must be >= 0");
+            if (!(r.rho >= -1.0 && r.rho <= 1.0))
+                return fail(MP_E_INVALID_CONFIG, "rho must be in [-1, 1]");
+            if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
+            if (r.heston_scheme != 0 && r.heston_scheme != 1)
+                return fail(MP_E_INVALID_CONFIG, "heston_scheme must be 0 or 1");
+            if (r.n_steps > 0x7FFFFFFFull) return fail(MP_E_UNSUPPORTED, "n_steps too large");
+            d.model = MP_MODEL_HESTON;
+            d.payoff = MP_PAY_EUROPEAN;
+            d.scheme = r.heston_scheme;
+            d.n_steps = (int32_t)r.n_steps;
+            d.v0 = r.v0;
+            d.kappa = r.kappa;
+            d.theta = r.theta;
+            d.xi = r.xi;
+            d.rho = r.rho;
+            const double rp = 1.0 - r.rho * r.rho;
+            d.rho_perp = std::sqrt(rp > 0.0 ? rp : 0.0);
+            d.dt = t / (double)r.n_steps;
+            d.sqrt_dt = std::sqrt(t / (double)r.n_steps);
+            if (qmc) {
+                P.kind = KIND_QMC;
+                P.factors = 2;
+            }
+            break;
+        }
+        case MP_MERTON_JUMP_DIFFUSION_CALL: {
+            // validate_merton_jump_diffusion_call_config
+            if (!(s0 > 0.0)) return fail(MP_E_INVALID_CONFIG, "s0 must be > 0");
+            if (!(k > 0.0)) return fail(MP_E_INVALID_CONFIG, "k must be > 0");
+            if (!(sig >= 0.0)) return fail(MP_E_INVALID_CONFIG, "sigma must be >= 0");
+            if (!(r.jump_intensity >= 0.0))
+                return fail(MP_E_INVALID_CONFIG, "jump_intensity must be >= 0");
+            if (!(r.jump_volatility >= 0.0))
+                return fail(MP_E_INVALID_CONFIG, "jump_volatility must be >= 0");
+            if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
+            if (qmc) return fail(MP_E_UNSUPPORTED, "Merton is priced with pseudorandom draws only");
+            const uint64_t m = r.n_steps == 0 ? 1 : r.n_steps;  // reference Merton is terminal
+            if (m > 0x7FFFFFFFull) return fail(MP_E_UNSUPPORTED, "n_steps too large");
+            d.model = MP_MODEL_MERTON;
+            d.payoff = MP_PAY_EUROPEAN;
+            d.n_steps = (int32_t)m;
+            d.dt = t / (double)m;
+            d.sqrt_dt = std::sqrt(d.dt);
+            const double comp = r.jump_intensity *
+                                (std::exp(r.jump_mean + 0.5 * (r.jump_volatility * r.jump_volatility)) - 1.0);
+            d.drift_dt = (rr - comp - 0.5 * sig * sig) * d.dt;
+            d.vol_dt = sig * std::sqrt(d.dt);
+            d.mu_j = r.jump_mean;
+            d.sigma_j = r.jump_volatility;
+            d.jump_mean_dt = r.jump_intensity * d.dt;
+            // cumulative Poisson probabilities by the reference recursion
+            double prob = std::exp(-d.jump_mean_dt), cum = prob;
+            d.n_cum = MP_POISSON_TERMS;
+            for (int i = 0; i < MP_POISSON_TERMS; ++i) {
+                d.cum[i] = cum;
+                prob *= d.jump_mean_dt / (double)(i + 1);
+                cum += prob;
+            }
+            P.n_steps_cfg = m;
+            break;
+        }
+        case MP_BASKET_CALL: {
+            if (r.n_assets < 1 || r.n_assets > MP_MAX_ASSETS)
+                return fail(MP_E_INVALID_CONFIG, "n_assets must be in 1..8");
+            if (!(k >= 0.0)) return fail(MP_E_INVALID_CONFIG, "k must be >= 0");
+            if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
+            if (qmc) return fail(MP_E_UNSUPPORTED, "basket is priced with pseudorandom draws only");
+            for (int a = 0; a < r.n_assets; ++a) {
+                if (!(r.asset_s0[a] > 0.0)) return fail(MP_E_INVALID_CONFIG, "asset s0 must be > 0");
+                if (!(r.asset_sigma[a] >= 0.0))
+                    return fail(MP_E_INVALID_CONFIG, "asset sigma must be >= 0");
+            }
+            const uint64_t m = r.n_steps == 0 ? 1 : r.n_steps;
+            d.model = -1;
+            d.n_steps = (int32_t)m;
+            d.dt = t / (double)m;
+            d.n_strikes = 1;
+            d.strikes[0] = k;
+            P.kind = KIND_BASKET;
+            P.n_assets = r.n_assets;
+            P.n_steps_cfg = m;
+            break;
+        }
+        case MP_AMERICAN_PUT:
+        case MP_BERMUDAN_PUT: {
+            // validate_american_put_config / validate_bermudan_put_config
+            if (r.n_steps < 2) return fail(MP_E_INVALID_CONFIG, "n_steps must be >= 2 for LSM regression");
+            if (r.n_steps > 65535) return fail(MP_E_UNSUPPORTED, "LSM supports at most 65535 steps");
+            if (!(s0 > 0.0)) return fail(MP_E_INVALID_CONFIG, "s0 must be > 0");
+            if (!(k > 0.0)) return fail(MP_E_INVALID_CONFIG, "k must be > 0");
+            if (!(sig >= 0.0)) return fail(MP_E_INVALID_CONFIG, "sigma must be >= 0");
+            if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
+            if (r.basis_degree <= 0) return fail(MP_E_INVALID_CONFIG, "basis_degree must be > 0");
+            if (qmc) return fail(MP_E_UNSUPPORTED, "LSM is priced with pseudorandom draws only");
+            P.kind = KIND_LSM;
+            P.degree = std::min(std::max(r.basis_degree, 1), 3);
+            if (r.workload == MP_AMERICAN_PUT) {
+                for (uint32_t s = 1; s <= r.n_steps; ++s) P.schedule.push_back(s);
+                P.allow_intrinsic = 1;
+            } else {
+                if (r.n_exercise <= 0 || !r.exercise_steps)
+                    return fail(MP_E_INVALID_CONFIG, "exercise_steps must include at least one exercise date");
+                for (int i = 0; i < r.n_exercise; ++i) {
+                    const uint32_t s = r.exercise_steps[i];
+                    if (s == 0 || s > r.n_steps)
+                        return fail(MP_E_INVALID_CONFIG, "exercise_steps must be in 1..=n_steps");
+                    P.schedule.push_back(s);
+                }
+                // normalize_bermudan_exercise_schedule: sort and deduplicate
+                std::sort(P.schedule.begin(), P.schedule.end());
+                P.schedule.erase(std::unique(P.schedule.begin(), P.schedule.end()), P.schedule.end());
+                P.allow_intrinsic = 0;
+            }
+            if (r.workload == MP_AMERICAN_PUT && r.allow_intrinsic_now == 0) P.allow_intrinsic = 1;
+            d.n_steps = (int32_t)r.n_steps;
+            break;
+        }
+        default:
+            return fail(MP_E_UNSUPPORTED, "unknown workload " + std::to_string(r.workload));
+    }
+    if (P.kind == KIND_TILE || P.kind == KIND_BASKET) {
+        d.tile_paths = tile_paths_for(d.n_paths, (uint64_t)d.n_steps);
+        d.n_tiles = (uint32_t)((d.n_paths + d.tile_paths - 1) / d.tile_paths);
+    }
+    return MP_OK;
+}
+
+double cost_of(const mp_dreq& d) {
+    const double w = d.model == MP_MODEL_HESTON ? 2.5 : d.model == MP_MODEL_MERTON ? 1.4
+                     : d.payoff == MP_PAY_ASIAN ? 1.2 : 1.0;
+    return w * (double)d.n_steps;
+}
+
+void fill_result(mp_result* out, const double* fin, const prepared& P) {
+    out->status = MP_OK;
+    out->n_strikes = P.d.n_strikes;
+    for (int j = 0; j < MP_MAX_STRIKES; ++j) {
+        out->strike_price[j] = j < P.d.n_strikes ? fin[2 * j] : 0.0;
+        out->strike_std_error[j] = j < P.d.n_strikes ? fin[2 * j + 1] : 0.0;
+    }
+    out->price = fin[0];
+    out->std_error = fin[1];
+    out->n_paths = P.d.n_paths;
+    out->n_steps = P.n_steps_cfg;
+    out->fp64_ref_error = NAN;
+}
+
+int grid_for_tiles(const mp_ctx* c, uint32_t n_tiles, int block) {
+    const int wpb = block / 64;
+    const int want = (int)((n_tiles + wpb - 1) / wpb);
+    return std::max(1, std::min(want, c->n_cu * 8));
+}
+
+uint64_t tuning_key(const mp_dreq& d, int precision) {
+    const uint64_t bucket = d.n_steps <= 32 ? 0 : d.n_steps <= 128 ? 1 : d.n_steps <= 512 ? 2 : 3;
+    return ((uint64_t)(d.model + 1) << 24) | ((uint64_t)d.payoff << 16) | (bucket << 8) |
+           (uint64_t)(precision == 64);
+}
+
+mp_tuning tuning_for(const mp_ctx* c, const mp_dreq& d, int precision) {
+    auto it = c->tuning.find(tuning_key(d, precision));
+    if (it != c->tuning.end()) return it->second;
+    mp_tuning t{};
+    t.block_size = 256;
+    t.paths_per_lane = 1;
+    t.tile_paths = (int32_t)d.tile_paths;
+    return t;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Batches: a set of tile-kernel requests packed into one persistent launch.
+// ===========================================================================
+struct mp_batch {
+    mp_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int precision = 32;
+    std::vector<prepared> reqs;     // tile-kernel requests of this batch
+    std::vector<size_t> req_index;  // position in the caller's array
+    uint32_t n_tiles = 0;
+    int grid = 1, block = 256, ppl = 1;
+    dbuf d_reqs, d_tile_req, d_order, d_partial, d_out, d_counter;
+    std::vector<double> h_out;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = false;
+    ~mp_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (e2) (void)hipEventDestroy(e2);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int batch_build(mp_batch* b) {
+    const size_t n = b->reqs.size();
+    std::vector<mp_dreq> dreqs(n);
+    uint32_t tiles = 0;
+    for (size_t i = 0; i < n; ++i) {
+        dreqs[i] = b->reqs[i].d;
+        dreqs[i].tile0 = tiles;
+        tiles += dreqs[i].n_tiles;
+    }
+    b->n_tiles = tiles;
+    std::vector<uint32_t> tile_req(tiles), order(tiles);
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t t = 0; t < dreqs[i].n_tiles; ++t) tile_req[dreqs[i].tile0 + t] = (uint32_t)i;
+    // Longest-processing-time-first queue order: costly tiles are pulled first.
+    for (uint32_t t = 0; t < tiles; ++t) order[t] = t;
+    std::vector<double> cost(n);
+    for (size_t i = 0; i < n; ++i) cost[i] = cost_of(dreqs[i]) * dreqs[i].tile_paths;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+        return cost[tile_req[a]] > cost[tile_req[c]];
+    });
+    ENSURE(b->d_reqs, n * sizeof(mp_dreq));
+    ENSURE(b->d_tile_req, (size_t)tiles * 4);
+    ENSURE(b->d_order, (size_t)tiles * 4);
+    ENSURE(b->d_partial, (size_t)tiles * MP_PARTIAL_STRIDE * 8);
+    ENSURE(b->d_out, n * MP_PARTIAL_STRIDE * 8);
+    ENSURE(b->d_counter, 64);
+    HIPCHK(hipMemcpyAsync(b->d_reqs.p, dreqs.data(), n * sizeof(mp_dreq), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_tile_req.p, tile_req.data(), (size_t)tiles * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_order.p, order.data(), (size_t)tiles * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    // launch shape: tuned for the dominant request class
+    mp_tuning tu = tuning_for(b->ctx, dreqs.empty() ? mp_dreq{} : dreqs[0], b->precision);
+    b->block = tu.block_size > 0 ? tu.block_size : 256;
+    b->ppl = tu.paths_per_lane > 0 ? tu.paths_per_lane : 1;
+    b->grid = grid_for_tiles(b->ctx, tiles, b->block);
+    b->h_out.assign(n * MP_PARTIAL_STRIDE, 0.0);
+    return MP_OK;
+}
+
+int batch_enqueue(mp_batch* b) {
+    if (b->reqs.empty()) return MP_OK;
+    if (b->use_graph && b->exec) {
+        HIPCHK(hipGraphLaunch(b->exec, b->stream));
+        return MP_OK;
+    }
+    mp::tile_launch L{};
+    L.reqs = b->d_reqs.as<mp_dreq>();
+    L.tile_req = b->d_tile_req.as<uint32_t>();
+    L.order = b->d_order.as<uint32_t>();
+    L.n_tiles = b->n_tiles;
+    L.counter = b->d_counter.as<unsigned int>();
+    L.partial = b->d_partial.as<double>();
+    L.grid = b->grid;
+    L.block = b->block;
+    L.ppl = b->ppl;
+    HIPCHK(mp::launch_reset(L.counter, 1, b->stream));
+    HIPCHK(hipEventRecord(b->e0, b->stream));
+    HIPCHK(mp::launch_tiles(L, b->precision, b->stream));
+    HIPCHK(hipEventRecord(b->e1, b->stream));
+    HIPCHK(mp::launch_finalize(L.reqs, (int)b->reqs.size(), L.partial, b->d_out.as<double>(), b->stream));
+    HIPCHK(hipEventRecord(b->e2, b->stream));
+    return MP_OK;
+}
+
+// Capture reset + tiles + finalize into a graph (no memset nodes: the queue
+// counter is cleared by a kernel).
+int batch_capture(mp_batch* b) {
+    if (b->reqs.empty()) return MP_OK;
+    mp::tile_launch L{};
+    L.reqs = b->d_reqs.as<mp_dreq>();
+    L.tile_req = b->d_tile_req.as<uint32_t>();
+    L.order = b->d_order.as<uint32_t>();
+    L.n_tiles = b->n_tiles;
+    L.counter = b->d_counter.as<unsigned int>();
+    L.partial = b->d_partial.as<double>();
+    L.grid = b->grid;
+    L.block = b->block;
+    L.ppl = b->ppl;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = mp::launch_reset(L.counter, 1, b->stream);
+    hipError_t e2 = mp::launch_tiles(L, b->precision, b->stream);
+    hipError_t e3 = mp::launch_finalize(L.reqs, (int)b->reqs.size(), L.partial, b->d_out.as<double>(), b->stream);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(b->stream, &g));
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(MP_E_HIP, "kernel launch failed during graph capture");
+    }
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->use_graph = true;
+    return MP_OK;
+}
+
+int batch_collect(mp_batch* b, mp_result* results, bool timed) {
+    if (b->reqs.empty()) return MP_OK;
+    HIPCHK(hipMemcpyAsync(b->h_out.data(), b->d_out.p, b->h_out.size() * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    float main_ms = 0.f, fin_ms = 0.f;
+    if (timed && !b->use_graph) {
+        (void)hipEventElapsedTime(&main_ms, b->e0, b->e1);
+        (void)hipEventElapsedTime(&fin_ms, b->e1, b->e2);
+        b->ctx->last.main_kernel_ms = main_ms;
+        b->ctx->last.finalize_ms = fin_ms;
+        b->ctx->last.main_launches = 1;
+    }
+    for (size_t i = 0; i < b->reqs.size(); ++i) {
+        fill_result(&results[b->req_index[i]], &b->h_out[i * MP_PARTIAL_STRIDE], b->reqs[i]);
+        results[b->req_index[i]].runtime_ms = main_ms + fin_ms;
+    }
+    return MP_OK;
+}
+
+int batch_init(mp_ctx* ctx, mp_batch* b, hipStream_t stream, int precision) {
+    b->ctx = ctx;
+    b->precision = precision;
+    if (stream) {
+        b->stream = stream;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->own_stream = true;
+    }
+    HIPCHK(hipEventCreate(&b->e0));
+    HIPCHK(hipEventCreate(&b->e1));
+    HIPCHK(hipEventCreate(&b->e2));
+    return MP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Single-request paths that are not work-queue tiles
+// ---------------------------------------------------------------------------
+int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_result* out,
+            double* term_host, double* pay_host) {
+    mp_dreq d = P.d;
+    std::vector<mp_bbop> ops;
+    int n_slots = 1;
+    mp_qmc_params qp{};
+    qp.base_seed = (uint32_t)r.seed;
+    qp.factors = P.factors;
+    qp.bridge = P.bridge;
+    qp.sqrt_t = std::sqrt(r.t);
+    qp.inv_sqrt_dt = 1.0 / std::sqrt(r.t / (double)d.n_steps);
+    if (P.bridge) bridge_plan((uint64_t)d.n_steps, r.t, ops, n_slots);
+    qp.n_ops = (int)ops.size();
+    qp.n_slots = n_slots;
+    // block size: largest power of two <= 1024 whose LDS fits
+    // block size: largest power of two <= 1024 whose LDS (plus 2 KB of static
+    // reduction space) fits; the Sobol tables move to global memory if the
+    // per-block LDS cannot hold them
+    const size_t reserve = 2048;
+    bool nib_lds = true;
+    int block = 1024;
+    while (block > 64 && mp::qmc_lds_bytes(precision, P.factors, n_slots, block, true) + reserve > c->max_lds)
+        block /= 2;
+    if (mp::qmc_lds_bytes(precision, P.factors, n_slots, block, true) + reserve > c->max_lds) {
+        nib_lds = false;
+        block = 1024;
+        while (block > 64 && mp::qmc_lds_bytes(precision, P.factors, n_slots, block, false) + reserve > c->max_lds)
+            block /= 2;
+        if (mp::qmc_lds_bytes(precision, P.factors, n_slots, block, false) + reserve > c->max_lds)
+            return fail(MP_E_UNSUPPORTED, "Brownian-bridge stack does not fit in LDS for this step count");
+    }
+    const size_t lds = mp::qmc_lds_bytes(precision, P.factors, n_slots, block, nib_lds) + reserve;
+    const int per_cu = std::max<int>(1, (int)(c->max_lds / std::max<size_t>(lds, 1)));
+    d.tile_paths = (uint32_t)block;
+    d.n_tiles = (uint32_t)((d.n_paths + block - 1) / block);
+    d.tile0 = 0;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(d.n_tiles, (uint64_t)c->n_cu * per_cu));
+    ENSURE(c->reqs, sizeof(mp_dreq));
+    ENSURE(c->plan, std::max<size_t>(ops.size(), 1) * sizeof(mp_bbop));
+    ENSURE(c->partial, (size_t)d.n_tiles * MP_PARTIAL_STRIDE * 8);
+    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
+    if (term_host) ENSURE(c->term, d.n_paths * 8);
+    if (pay_host) ENSURE(c->pay, d.n_paths * 8);
+    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    if (!ops.empty())
+        HIPCHK(hipMemcpyAsync(c->plan.p, ops.data(), ops.size() * sizeof(mp_bbop), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(mp::launch_qmc(c->reqs.as<mp_dreq>(), qp, c->d_nib, c->plan.as<mp_bbop>(),
+                          c->partial.as<double>(), term_host ? c->term.as<double>() : nullptr,
+                          pay_host ? c->pay.as<double>() : nullptr, precision, grid, block, nib_lds,
+                          c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->out.as<double>(), c->stream));
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    double fin[MP_PARTIAL_STRIDE];
+    HIPCHK(hipMemcpyAsync(fin, c->out.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
+    if (term_host) HIPCHK(hipMemcpyAsync(term_host, c->term.p, d.n_paths * 8, hipMemcpyDeviceToHost, c->stream));
+    if (pay_host) HIPCHK(hipMemcpyAsync(pay_host, c->pay.p, d.n_paths * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float main_ms = 0.f, fin_ms = 0.f;
+    (void)hipEventElapsedTime(&main_ms, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&fin_ms, c->ev[1], c->ev[2]);
+    c->last.main_kernel_ms = main_ms;
+    c->last.finalize_ms = fin_ms;
+    c->last.main_launches = 1;
+    fill_result(out, fin, P);
+    out->runtime_ms = main_ms + fin_ms;
+    return MP_OK;
+}
+
+int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const double* normals_host,
+               mp_result* out, double* term_host, double* pay_host) {
+    const int na = P.n_assets;
+    // Cholesky of the correlation matrix
+    std::vector<double> corr(na * na), L(na * na, 0.0);
+    for (int i = 0; i < na; ++i)
+        for (int j = 0; j < na; ++j) {
+            double v = r.correlation[i * na + j];
+            if (i == j && v == 0.0) v = 1.0;  // zero-initialised requests mean identity diagonal
+            corr[i * na + j] = v;
+        }
+    for (int i = 0; i < na; ++i)
+        for (int j = 0; j < na; ++j)
+            if (std::fabs(corr[i * na + j] - corr[j * na + i]) > 1e-12)
+                return fail(MP_E_INVALID_CONFIG, "correlation matrix must be symmetric");
+    for (int i = 0; i < na; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = corr[i * na + j];
+            for (int q = 0; q < j; ++q) s -= L[i * na + q] * L[j * na + q];
+            if (i == j) {
+                if (!(s > 0.0)) return fail(MP_E_INVALID_CONFIG, "correlation matrix is not positive definite");
+                L[i * na + i] = std::sqrt(s);
+            } else {
+                L[i * na + j] = s / L[j * na + j];
+            }
+        }
+    const double dt = r.t / (double)P.d.n_steps;
+    std::vector<double> par(na * na + 4 * na);
+    for (int i = 0; i < na * na; ++i) par[i] = L[i];
+    for (int a = 0; a < na; ++a) {
+        par[na * na + a] = (r.r - 0.5 * r.asset_sigma[a] * r.asset_sigma[a]) * dt;
+        par[na * na + na + a] = r.asset_sigma[a] * std::sqrt(dt);
+        par[na * na + 2 * na + a] = std::log(r.asset_s0[a]);
+        par[na * na + 3 * na + a] = r.asset_weight[a];
+    }
+    mp_dreq d = P.d;
+    const int block = 256;
+    d.tile_paths = block;
+    d.n_tiles = (uint32_t)((d.n_paths + block - 1) / block);
+    d.tile0 = 0;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(d.n_tiles, (uint64_t)c->n_cu * 8));
+    ENSURE(c->reqs, sizeof(mp_dreq));
+    ENSURE(c->basket_par, par.size() * 8);
+    ENSURE(c->partial, (size_t)d.n_tiles * MP_PARTIAL_STRIDE * 8);
+    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
+    const size_t nn = (size_t)d.n_paths * d.n_steps * na;
+    if (normals_host) ENSURE(c->normals, nn * 8);
+    if (term_host) ENSURE(c->term, d.n_paths * 8);
+    if (pay_host) ENSURE(c->pay, d.n_paths * 8);
+    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->basket_par.p, par.data(), par.size() * 8, hipMemcpyHostToDevice, c->stream));
+    if (normals_host) HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, nn * 8, hipMemcpyHostToDevice, c->stream));
+    const double* bp = c->basket_par.as<double>();
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(mp::launch_basket(c->reqs.as<mp_dreq>(), na, bp, bp + na * na, bp + na * na + na,
+                             bp + na * na + 2 * na, bp + na * na + 3 * na,
+                             normals_host ? c->normals.as<double>() : nullptr, c->partial.as<double>(),
+                             term_host ? c->term.as<double>() : nullptr,
+                             pay_host ? c->pay.as<double>() : nullptr, precision, grid, block, c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->out.as<double>(), c->stream));
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    double fin[MP_PARTIAL_STRIDE];
+    HIPCHK(hipMemcpyAsync(fin, c->out.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
+    if (term_host) HIPCHK(hipMemcpyAsync(term_host, c->term.p, d.n_paths * 8, hipMemcpyDeviceToHost, c->stream));
+    if (pay_host) HIPCHK(hipMemcpyAsync(pay_host, c->pay.p, d.n_paths * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float main_ms = 0.f, fin_ms = 0.f;
+    (void)hipEventElapsedTime(&main_ms, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&fin_ms, c->ev[1], c->ev[2]);
+    c->last.main_kernel_ms = main_ms;
+    c->last.finalize_ms = fin_ms;
+    c->last.main_launches = 1;
+    fill_result(out, fin, P);
+    out->runtime_ms = main_ms + fin_ms;
+    return MP_OK;
+}
+
+int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const double* normals_host,
+            mp_result* out, double* coeff_host, uint32_t* ex_host) {
+    const uint64_t n = r.n_paths;
+    const int m = (int)r.n_steps;
+    mp_lsm_params p{};
+    p.n_steps = m;
+    p.degree = P.degree;
+    p.coeffs = P.degree + 1;
+    p.n_paths = n;
+    p.s0 = r.s0;
+    p.log_s0 = std::log(r.s0);
+    p.k = r.k;
+    // the reference divides by k.max(f64::MIN_POSITIVE); k > 0 is validated
+    p.inv_k = 1.0 / r.k;
+    const double dt = r.t / (double)m;
+    p.drift_dt = (r.r - 0.5 * r.sigma * r.sigma) * dt;
+    p.vol_dt = r.sigma * std::sqrt(dt);
+    p.disc_step = std::exp(-r.r * dt);
+    p.disc_m = powi_ref(p.disc_step, m);
+    p.key0 = (uint32_t)r.seed;
+    p.key1 = (uint32_t)r.contract_id;
+    p.seed_hi = (uint32_t)(r.seed >> 32);
+    std::vector<double> disc_pow(m + 1), growth(m + 1);
+    for (int s = 0; s <= m; ++s) {
+        disc_pow[s] = powi_ref(p.disc_step, s);
+        growth[s] = powi_ref(p.disc_step, -s);
+    }
+    const size_t rb = precision == 64 ? 8 : 4;
+    const int block = 256;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + block - 1) / block, (uint64_t)c->n_cu * 8));
+    ENSURE(c->lsm_paths, (size_t)(m - 1) * n * rb);
+    ENSURE(c->lsm_cash, n * 8);
+    ENSURE(c->lsm_ex, n * 2);
+    ENSURE(c->lsm_coeff, (size_t)(m + 1) * 4 * 8);
+    ENSURE(c->lsm_valid, (size_t)(m + 1) * 4);
+    ENSURE(c->lsm_gram, (size_t)grid * 16 * 8);
+    ENSURE(c->lsm_disc, (size_t)(m + 1) * 8);
+    ENSURE(c->lsm_grow, (size_t)(m + 1) * 8);
+    ENSURE(c->lsm_part, (size_t)grid * 4 * 8);
+    ENSURE(c->lsm_out, 8 * 8);
+    if (normals_host) ENSURE(c->normals, (size_t)m * n * 8);
+    HIPCHK(hipMemcpyAsync(c->lsm_disc.p, disc_pow.data(), (m + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->lsm_grow.p, growth.data(), (m + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->lsm_valid.p, 0, (size_t)(m + 1) * 4, c->stream));
+    HIPCHK(hipMemsetAsync(c->lsm_coeff.p, 0, (size_t)(m + 1) * 4 * 8, c->stream));
+    if (normals_host)
+        HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, (size_t)m * n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(mp::launch_lsm_paths(p, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
+                                c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), precision, grid,
+                                block, c->stream));
+    // regression dates: schedule without its last element, latest first
+    const int ns = (int)P.schedule.size();
+    int prev = 0;
+    for (int i = ns - 2; i >= 0; --i) {
+        const int date = (int)P.schedule[i];
+        HIPCHK(mp::launch_lsm_step(p, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
+                                   c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, date,
+                                   c->lsm_disc.as<double>(), c->lsm_grow.as<double>(),
+                                   c->lsm_gram.as<double>(), precision, grid, block, c->stream));
+        HIPCHK(mp::launch_lsm_solve(p, c->lsm_gram.as<double>(), grid, date, c->lsm_coeff.as<double>(),
+                                    c->lsm_valid.as<int>(), c->stream));
+        prev = date;
+    }
+    if (prev >= 1)
+        HIPCHK(mp::launch_lsm_step(p, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
+                                   c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, 0,
+                                   c->lsm_disc.as<double>(), c->lsm_grow.as<double>(),
+                                   c->lsm_gram.as<double>(), precision, grid, block, c->stream));
+    HIPCHK(mp::launch_lsm_final(p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
+                                c->lsm_part.as<double>(), grid, block, c->stream));
+    HIPCHK(mp::launch_lsm_reduce(p, c->lsm_part.as<double>(), grid, c->lsm_valid.as<int>(),
+                                 P.allow_intrinsic, c->lsm_out.as<double>(), c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    double res[8];
+    HIPCHK(hipMemcpyAsync(res, c->lsm_out.p, 5 * 8, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> coeff;
+    if (coeff_host) {
+        coeff.resize((size_t)(m + 1) * 4);
+        HIPCHK(hipMemcpyAsync(coeff.data(), c->lsm_coeff.p, coeff.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    std::vector<uint16_t> ex;
+    if (ex_host) {
+        ex.resize(n);
+        HIPCHK(hipMemcpyAsync(ex.data(), c->lsm_ex.p, n * 2, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->last.main_kernel_ms = ms;
+    c->last.finalize_ms = 0.0;
+    c->last.main_launches = (uint64_t)(2 * std::max(ns - 1, 0) + 3);
+    if (coeff_host)
+        for (int s = 0; s < m; ++s)
+            for (int i = 0; i < 4; ++i) coeff_host[s * 4 + i] = coeff[(size_t)s * 4 + i];
+    if (ex_host)
+        for (uint64_t i = 0; i < n; ++i) ex_host[i] = ex[i];
+    std::memset(out, 0, sizeof(*out));
+    out->status = MP_OK;
+    out->n_strikes = 1;
+    out->price = res[0];
+    out->std_error = res[1];
+    out->strike_price[0] = res[0];
+    out->strike_std_error[0] = res[1];
+    out->early_exercise_count = (uint64_t)res[2];
+    out->maturity_exercise_count = (uint64_t)res[3];
+    out->regression_steps = (uint64_t)res[4];
+    out->n_paths = n;
+    out->n_steps = (uint64_t)m;
+    out->runtime_ms = ms;
+    out->fp64_ref_error = NAN;
+    return MP_OK;
+}
+
+int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, const double* normals,
+                     const double* uniforms, const double* jumps, double* term_host,
+                     double* pay_host) {
+    mp_dreq d = P.d;
+    d.tile0 = 0;
+    const bool buffer = normals != nullptr;
+    const uint64_t n = d.n_paths;
+    const int F = d.model == MP_MODEL_HESTON ? 2 : 1;
+    const size_t nn = (size_t)n * d.n_steps * F;
+    ENSURE(c->reqs, sizeof(mp_dreq));
+    ENSURE(c->tile_req, (size_t)d.n_tiles * 4);
+    ENSURE(c->partial, (size_t)d.n_tiles * MP_PARTIAL_STRIDE * 8);
+    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
+    ENSURE(c->counter, 64);
+    const bool want_paths = term_host != nullptr || pay_host != nullptr;
+    if (want_paths) {
+        ENSURE(c->term, n * 8);
+        ENSURE(c->pay, n * 8);
+    }
+    if (buffer) {
+        ENSURE(c->normals, nn * 8);
+        if (d.model == MP_MODEL_MERTON) {
+            ENSURE(c->uniforms, (size_t)n * d.n_steps * 8);
+            ENSURE(c->jumps, (size_t)n * d.n_steps * 8);
+        }
+    }
+    std::vector<uint32_t> tile_req(d.n_tiles, 0u);
+    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->tile_req.p, tile_req.data(), (size_t)d.n_tiles * 4, hipMemcpyHostToDevice, c->stream));
+    if (buffer) {
+        HIPCHK(hipMemcpyAsync(c->normals.p, normals, nn * 8, hipMemcpyHostToDevice, c->stream));
+        if (d.model == MP_MODEL_MERTON) {
+            HIPCHK(hipMemcpyAsync(c->uniforms.p, uniforms, (size_t)n * d.n_steps * 8, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(c->jumps.p, jumps, (size_t)n * d.n_steps * 8, hipMemcpyHostToDevice, c->stream));
+        }
+    }
+    const mp_tuning tu = tuning_for(c, d, precision);
+    mp::tile_launch L{};
+    L.reqs = c->reqs.as<mp_dreq>();
+    L.tile_req = c->tile_req.as<uint32_t>();
+    L.order = nullptr;
+    L.n_tiles = d.n_tiles;
+    L.counter = c->counter.as<unsigned int>();
+    L.partial = c->partial.as<double>();
+    L.normals = buffer ? c->normals.as<double>() : nullptr;
+    L.uniforms = buffer && d.model == MP_MODEL_MERTON ? c->uniforms.as<double>() : nullptr;
+    L.jump_normals = buffer && d.model == MP_MODEL_MERTON ? c->jumps.as<double>() : nullptr;
+    L.terminal_out = want_paths ? c->term.as<double>() : nullptr;
+    L.payoff_out = want_paths ? c->pay.as<double>() : nullptr;
+    L.block = tu.block_size > 0 ? tu.block_size : 256;
+    L.ppl = tu.paths_per_lane > 0 ? tu.paths_per_lane : 1;
+    L.grid = grid_for_tiles(c, d.n_tiles, L.block);
+    HIPCHK(mp::launch_reset(L.counter, 1, c->stream));
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(mp::launch_tiles(L, precision, c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(mp::launch_finalize(L.reqs, 1, L.partial, c->out.as<double>(), c->stream));
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    double fin[MP_PARTIAL_STRIDE];
+    HIPCHK(hipMemcpyAsync(fin, c->out.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
+    if (term_host) HIPCHK(hipMemcpyAsync(term_host, c->term.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (pay_host) HIPCHK(hipMemcpyAsync(pay_host, c->pay.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float main_ms = 0.f, fin_ms = 0.f;
+    (void)hipEventElapsedTime(&main_ms, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&fin_ms, c->ev[1], c->ev[2]);
+    c->last.main_kernel_ms = main_ms;
+    c->last.finalize_ms = fin_ms;
+    c->last.main_launches = 1;
+    fill_result(out, fin, P);
+    out->runtime_ms = main_ms + fin_ms;
+    return MP_OK;
+}
+
+int price_one(mp_ctx* c, const mp_request& r, int precision, mp_result* out) {
+    prepared P;
+    int rc = prepare_request(r, P);
+    if (rc != MP_OK) return rc;
+    switch (P.kind) {
+        case KIND_QMC:
+            return run_qmc(c, r, P, precision, out, nullptr, nullptr);
+        case KIND_BASKET:
+            return run_basket(c, r, P, precision, nullptr, out, nullptr, nullptr);
+        case KIND_LSM:
+            return run_lsm(c, r, P, precision, nullptr, out, nullptr, nullptr);
+        default:
+            return run_tiles_single(c, P, precision, out, nullptr, nullptr, nullptr, nullptr, nullptr);
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+MP_EXPORT const char* mp_hip_version(void) { return "montepath-hip 0.1.0 (gfx950)"; }
+
+MP_EXPORT const char* mp_hip_last_error(void) { return g_err.c_str(); }
+
+MP_EXPORT int mp_hip_device_count(int* count) {
+    if (!count) return fail(MP_E_INVALID_CONFIG, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        return fail(MP_E_UNKNOWN_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    }
+    *count = n;
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
+    if (!out) return fail(MP_E_INVALID_CONFIG, "null context pointer");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device_ordinal < 0 || device_ordinal >= n) {
+        (void)hipGetLastError();
+        return fail(MP_E_UNKNOWN_DEVICE, "rocm:" + std::to_string(device_ordinal) + " is not available");
+    }
+    std::unique_ptr<mp_ctx> c(new mp_ctx());
+    c->device = device_ordinal;
+    HIPCHK(hipSetDevice(device_ordinal));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device_ordinal));
+    c->n_cu = prop.multiProcessorCount;
+    c->max_lds = prop.sharedMemPerBlock;
+    c->name = prop.name;
+    c->arch = prop.gcnArchName;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+    sobol_directions(c->dirs);
+    std::vector<uint32_t> nib;
+    sobol_nibbles(c->dirs, nib);
+    HIPCHK(hipMalloc(&c->d_nib, nib.size() * 4));
+    HIPCHK(hipMemcpy(c->d_nib, nib.data(), nib.size() * 4, hipMemcpyHostToDevice));
+    *out = c.release();
+    return MP_OK;
+}
+
+MP_EXPORT void mp_hip_close(mp_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& b : c->bat) delete b;
+    if (c->d_nib) (void)hipFree(c->d_nib);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+MP_EXPORT int mp_hip_device_info(mp_ctx* c, mp_device_info* out) {
+    if (!c || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    std::snprintf(out->device_id, sizeof(out->device_id), "rocm:%d", c->device);
+    std::snprintf(out->name, sizeof(out->name), "%s", c->name.c_str());
+    std::snprintf(out->arch, sizeof(out->arch), "%s", c->arch.c_str());
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    out->memory_total_mb = tot >> 20;
+    out->memory_free_mb = fr >> 20;
+    out->compute_units = c->n_cu;
+    out->supports_float64 = 1;
+    out->max_threads_hint = 1024;
+    out->wavefront_size = 64;
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_price(mp_ctx* c, const mp_request* req, mp_result* out) {
+    if (!c || !req || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const int prec = req->precision == 64 ? 64 : 32;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = price_one(c, *req, prec, out);
+    if (rc != MP_OK) {
+        out->status = rc;
+        return rc;
+    }
+    if (req->fp64_check && prec == 32) {
+        mp_result ref;
+        const mp_timing keep = c->last;
+        rc = price_one(c, *req, 64, &ref);
+        c->last = keep;
+        if (rc != MP_OK) return rc;
+        out->fp64_ref_error = std::fabs(out->price - ref.price);
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    c->last.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_price_batch(mp_ctx* c, const mp_request* reqs, size_t n, mp_result* out) {
+    if (!c || (!reqs && n) || (!out && n)) return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = MP_OK;
+    for (int i = 0; i < 2; ++i) {
+        if (!c->bat[i]) {
+            c->bat[i] = new mp_batch();
+            rc = batch_init(c, c->bat[i], c->stream, i ? 64 : 32);
+            if (rc != MP_OK) return rc;
+        }
+        c->bat[i]->reqs.clear();
+        c->bat[i]->req_index.clear();
+    }
+    mp_batch& b32 = *c->bat[0];
+    mp_batch& b64 = *c->bat[1];
+    std::vector<size_t> others;
+    for (size_t i = 0; i < n; ++i) {
+        prepared P;
+        rc = prepare_request(reqs[i], P);
+        if (rc != MP_OK) {
+            out[i].status = rc;
+            return fail(rc, "request " + std::to_string(i) + ": " + g_err);
+        }
+        if (P.kind == KIND_TILE && !reqs[i].fp64_check) {
+            mp_batch& b = P.precision == 64 ? b64 : b32;
+            b.reqs.push_back(P);
+            b.req_index.push_back(i);
+        } else {
+            others.push_back(i);
+        }
+    }
+    for (mp_batch* b : {&b32, &b64}) {
+        if (b->reqs.empty()) continue;
+        rc = batch_build(b);
+        if (rc != MP_OK) return rc;
+        rc = batch_enqueue(b);
+        if (rc != MP_OK) return rc;
+        rc = batch_collect(b, out, true);
+        if (rc != MP_OK) return rc;
+    }
+    for (size_t i : others) {
+        rc = mp_hip_price(c, &reqs[i], &out[i]);
+        if (rc != MP_OK) return rc;
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    c->last.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return MP_OK;
+}
+
+// ---- prepared batches (resident inputs, graph replay) ----------------------
+MP_EXPORT int mp_hip_batch_prepare(mp_ctx* c, const mp_request* reqs, size_t n, int use_graph,
+                                   mp_batch** out) {
+    if (!c || !reqs || !out || n == 0) return fail(MP_E_INVALID_CONFIG, "null argument or empty batch");
+    HIPCHK(hipSetDevice(c->device));
+    std::unique_ptr<mp_batch> b(new mp_batch());
+    int prec = reqs[0].precision == 64 ? 64 : 32;
+    int rc = batch_init(c, b.get(), nullptr, prec);
+    if (rc != MP_OK) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        prepared P;
+        rc = prepare_request(reqs[i], P);
+        if (rc != MP_OK) return fail(rc, "request " + std::to_string(i) + ": " + g_err);
+        if (P.kind != KIND_TILE)
+            return fail(MP_E_UNSUPPORTED, "prepared batches carry GBM/Heston/Merton pseudorandom requests only");
+        if (P.precision != prec) return fail(MP_E_INVALID_CONFIG, "a prepared batch has one precision");
+        b->reqs.push_back(P);
+        b->req_index.push_back(i);
+    }
+    rc = batch_build(b.get());
+    if (rc != MP_OK) return rc;
+    if (use_graph) {
+        rc = batch_capture(b.get());
+        if (rc != MP_OK) return rc;
+    }
+    *out = b.release();
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_batch_launch(mp_batch* b) {
+    if (!b) return fail(MP_E_INVALID_CONFIG, "null batch");
+    return batch_enqueue(b);
+}
+
+MP_EXPORT int mp_hip_batch_sync(mp_batch* b) {
+    if (!b) return fail(MP_E_INVALID_CONFIG, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_batch_results(mp_batch* b, mp_result* out) {
+    if (!b || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    return batch_collect(b, out, true);
+}
+
+MP_EXPORT int mp_hip_batch_info(mp_batch* b, uint64_t* n_tiles, uint64_t* path_steps, int* grid,
+                                int* block) {
+    if (!b) return fail(MP_E_INVALID_CONFIG, "null batch");
+    uint64_t ps = 0;
+    for (auto& p : b->reqs) ps += p.d.n_paths * (uint64_t)p.d.n_steps;
+    if (n_tiles) *n_tiles = b->n_tiles;
+    if (path_steps) *path_steps = ps;
+    if (grid) *grid = b->grid;
+    if (block) *block = b->block;
+    return MP_OK;
+}
+
+MP_EXPORT void mp_hip_batch_free(mp_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    delete b;
+}
+
+MP_EXPORT int mp_hip_price_plain_fp64(mp_ctx* c, const mp_request* req, mp_result* out) {
+    if (!c || !req || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    if (P.kind != KIND_TILE || P.d.model != MP_MODEL_GBM ||
+        (P.d.payoff != MP_PAY_EUROPEAN && P.d.payoff != MP_PAY_ASIAN) || P.d.n_strikes != 1)
+        return fail(MP_E_UNSUPPORTED, "the plain FP64 path prices single-strike GBM European/Asian calls");
+    mp_dreq d = P.d;
+    d.discount = std::exp(-req->r * req->t);
+    const uint64_t n = d.n_paths;
+    const int block = 256;
+    const int grid = (int)((n + block - 1) / block);
+    const int rblocks = std::max(1, std::min(grid, c->n_cu * 4));
+    ENSURE(c->reqs, sizeof(mp_dreq));
+    ENSURE(c->plain_pay, n * 8);
+    ENSURE(c->partial, (size_t)rblocks * MP_PARTIAL_STRIDE * 8);
+    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
+    // finalize sums payoffs that are already discounted
+    mp_dreq fin_d = d;
+    fin_d.tile0 = 0;
+    fin_d.n_tiles = (uint32_t)rblocks;
+    fin_d.n_strikes = 1;
+    fin_d.discount = 1.0;
+    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(mp::launch_plain_fp64(c->reqs.as<mp_dreq>(), c->plain_pay.as<double>(), grid, block, c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(mp::launch_plain_reduce(c->plain_pay.as<double>(), n, c->partial.as<double>(), rblocks, c->stream));
+    HIPCHK(hipMemcpyAsync(c->reqs.p, &fin_d, sizeof(fin_d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->out.as<double>(), c->stream));
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    double fin[MP_PARTIAL_STRIDE];
+    HIPCHK(hipMemcpyAsync(fin, c->out.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float main_ms = 0.f, fin_ms = 0.f;
+    (void)hipEventElapsedTime(&main_ms, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&fin_ms, c->ev[1], c->ev[2]);
+    c->last.main_kernel_ms = main_ms;
+    c->last.finalize_ms = fin_ms;
+    c->last.main_launches = 1;
+    fill_result(out, fin, P);
+    out->runtime_ms = main_ms + fin_ms;
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_paths_from_normals(mp_ctx* c, const mp_request* req, const double* normals,
+                                        const double* uniforms, const double* jump_normals,
+                                        double* terminal_out, double* payoff_out) {
+    if (!c || !req || !normals || !terminal_out || !payoff_out)
+        return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    if (req->sampling != MP_PSEUDORANDOM)
+        return fail(MP_E_INCOMPATIBLE_INPUT, "normal buffers replace pseudorandom draws only");
+    const int prec = P.precision;
+    mp_result res;
+    if (P.kind == KIND_BASKET)
+        return run_basket(c, *req, P, prec, normals, &res, terminal_out, payoff_out);
+    if (P.kind != KIND_TILE)
+        return fail(MP_E_INCOMPATIBLE_INPUT, "use mp_hip_lsm_from_normals for LSM requests");
+    if (P.d.model == MP_MODEL_MERTON && (!uniforms || !jump_normals))
+        return fail(MP_E_INVALID_CONFIG, "Merton needs uniforms and jump_normals buffers");
+    return run_tiles_single(c, P, prec, &res, normals, uniforms, jump_normals, terminal_out, payoff_out);
+}
+
+MP_EXPORT int mp_hip_price_with_paths(mp_ctx* c, const mp_request* req, mp_result* out,
+                                      double* terminal_out, double* payoff_out) {
+    if (!c || !req || !out || !terminal_out || !payoff_out)
+        return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    switch (P.kind) {
+        case KIND_QMC:
+            return run_qmc(c, *req, P, P.precision, out, terminal_out, payoff_out);
+        case KIND_BASKET:
+            return run_basket(c, *req, P, P.precision, nullptr, out, terminal_out, payoff_out);
+        case KIND_LSM:
+            return fail(MP_E_INCOMPATIBLE_INPUT, "LSM per-path output: use mp_hip_lsm_from_normals");
+        default:
+            return run_tiles_single(c, P, P.precision, out, nullptr, nullptr, nullptr, terminal_out,
+                                    payoff_out);
+    }
+}
+
+MP_EXPORT int mp_hip_lsm_from_normals(mp_ctx* c, const mp_request* req, const double* normals,
+                                      mp_result* out, double* coefficients_out,
+                                      uint32_t* exercise_out) {
+    if (!c || !req || !normals || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    if (P.kind != KIND_LSM) return fail(MP_E_INCOMPATIBLE_INPUT, "not an American/Bermudan put request");
+    return run_lsm(c, *req, P, P.precision, normals, out, coefficients_out, exercise_out);
+}
+
+MP_EXPORT int mp_hip_draws(mp_ctx* c, const mp_request* req, int kind, uint32_t stream,
+                           uint64_t path0, uint64_t n_paths, uint64_t n_draws, double* out) {
+    if (!c || !req || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    if (kind < 0 || kind > 4) return fail(MP_E_INVALID_CONFIG, "draw kind must be 0..4");
+    if (n_paths == 0 || n_draws == 0) return MP_OK;
+    if (path0 + n_paths > 0xFFFFFFFFull) return fail(MP_E_UNSUPPORTED, "path index beyond 2^32");
+    HIPCHK(hipSetDevice(c->device));
+    mp_dreq d{};
+    d.key0 = (uint32_t)req->seed;
+    d.key1 = (uint32_t)req->contract_id;
+    d.seed_hi = (uint32_t)(req->seed >> 32);
+    mp_qmc_params qp{};
+    qp.base_seed = (uint32_t)req->seed;
+    const size_t bytes = (size_t)n_paths * n_draws * 8;
+    ENSURE(c->reqs, sizeof(mp_dreq));
+    ENSURE(c->draws, bytes);
+    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(mp::launch_draws(c->reqs.as<mp_dreq>(), qp, c->d_nib, kind, stream, path0, n_paths, n_draws,
+                            c->draws.as<double>(), c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->draws.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_request_footprint(const mp_request* req, uint64_t* bytes) {
+    if (!req || !bytes) return fail(MP_E_INVALID_CONFIG, "null argument");
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    uint64_t b = sizeof(mp_dreq) + 64;
+    switch (P.kind) {
+        case KIND_LSM: {
+            const uint64_t rb = P.precision == 64 ? 8 : 4;
+            b += (uint64_t)(req->n_steps - 1) * req->n_paths * rb + req->n_paths * (8 + 2) +
+                 (req->n_steps + 1) * 64 + 4096 * 16 * 8;
+            break;
+        }
+        case KIND_QMC:
+            b += (uint64_t)((P.d.n_paths + 63) / 64) * MP_PARTIAL_STRIDE * 8 + 64 * 1024;
+            break;
+        default:
+            b += (uint64_t)P.d.n_tiles * (MP_PARTIAL_STRIDE * 8 + 8);
+    }
+    *bytes = b;
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_last_timing(mp_ctx* c, mp_timing* out) {
+    if (!c || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    *out = c->last;
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_get_tuning(mp_ctx* c, const mp_request* req, mp_tuning* out) {
+    if (!c || !req || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    *out = tuning_for(c, P.d, P.precision);
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_set_tuning(mp_ctx* c, const mp_request* req, const mp_tuning* t) {
+    if (!c || !req || !t) return fail(MP_E_INVALID_CONFIG, "null argument");
+    if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
+        return fail(MP_E_INVALID_CONFIG, "block_size must be 64, 128 or 256");
+    if (t->paths_per_lane != 1 && t->paths_per_lane != 2 && t->paths_per_lane != 4)
+        return fail(MP_E_INVALID_CONFIG, "paths_per_lane must be 1, 2 or 4");
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    mp_tuning v = *t;
+    v.tile_paths = (int32_t)P.d.tile_paths;
+    c->tuning[tuning_key(P.d, P.precision)] = v;
+    return MP_OK;
+}
+
+// Times each (block size, paths per lane) candidate on the request (or a
+// bounded sample of it) and keeps the fastest for its class.  Results do not
+// depend on the choice: tiles, hence partial sums, are fixed by the request.
+MP_EXPORT int mp_hip_autotune(mp_ctx* c, const mp_request* req, mp_tuning* best, double* best_ms) {
+    if (!c || !req || !best) return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    if (P.kind != KIND_TILE) return fail(MP_E_UNSUPPORTED, "autotuning covers work-queue requests");
+    mp_request sample = *req;
+    const uint64_t budget = 1ull << 28;  // path-steps per trial
+    if (sample.n_paths * std::max<uint64_t>(P.d.n_steps, 1) > budget)
+        sample.n_paths = std::max<uint64_t>(budget / std::max<uint64_t>(P.d.n_steps, 1), 4096);
+    const uint64_t key = tuning_key(P.d, P.precision);
+    const int blocks[] = {128, 256};
+    const int ppls[] = {1, 2, 4};
+    double best_t = 1e300;
+    mp_tuning bt{};
+    for (int bs : blocks)
+        for (int ppl : ppls) {
+            mp_tuning t{bs, ppl, (int32_t)P.d.tile_paths, 0};
+            c->tuning[key] = t;
+            mp_result r;
+            double tmin = 1e300;
+            for (int rep = 0; rep < 3; ++rep) {
+                rc = price_one(c, sample, P.precision, &r);
+                if (rc != MP_OK) return rc;
+                tmin = std::min(tmin, c->last.main_kernel_ms);
+            }
+            if (tmin < best_t) {
+                best_t = tmin;
+                bt = t;
+            }
+        }
+    c->tuning[key] = bt;
+    *best = bt;
+    if (best_ms) *best_ms = best_t;
+    return MP_OK;
+}
+
+// ===========================================================================
+// Continuous-batching server
+// ===========================================================================
+namespace {
+
+struct ticket_state {
+    mp_request req;
+    std::vector<uint32_t> schedule;
+    mp_result res;
+    int done = 0;
+    uint64_t footprint = 0;
+};
+
+}  // namespace
+
+struct mp_server {
+    mp_ctx* ctx = nullptr;
+    mp_serve_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> pending;
+    std::unordered_map<uint64_t, std::unique_ptr<ticket_state>> tickets;
+    uint64_t next_ticket = 1;
+    uint64_t admitted_bytes = 0;
+    uint64_t max_admitted = 0;
+    uint64_t launches = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    mp_ctx* lsm_ctx = nullptr;  // context for requests outside the work queue
+    std::mutex lsm_mu;
+};
+
+namespace {
+
+// A worker owns one stream (one in-flight launch slot).  It takes every
+// pending work-queue request that fits the HBM budget, packs them into one
+// persistent launch, and completes them; requests that arrive meanwhile wait
+// for the next free slot.
+void server_worker(mp_server* s, int slot) {
+    (void)hipSetDevice(s->ctx->device);
+    mp_ctx* own = nullptr;
+    if (mp_hip_open(s->ctx->device, &own) != MP_OK) return;
+    for (;;) {
+        std::vector<uint64_t> take;
+        uint64_t bytes = 0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+            if (s->stop && s->pending.empty()) break;
+            while (!s->pending.empty() && (int)take.size() < s->cfg.max_batch) {
+                const uint64_t id = s->pending.front();
+                auto it = s->tickets.find(id);
+                if (it == s->tickets.end()) {
+                    s->pending.pop_front();
+                    continue;
+                }
+                const uint64_t fp = it->second->footprint;
+                if (!take.empty() && s->admitted_bytes + fp > s->cfg.hbm_budget_bytes) break;
+                s->pending.pop_front();
+                s->admitted_bytes += fp;
+                bytes += fp;
+                take.push_back(id);
+            }
+            s->max_admitted = std::max(s->max_admitted, s->admitted_bytes);
+            s->launches += 1;
+        }
+        // split into work-queue requests (one persistent launch) and the rest
+        std::vector<mp_request> wq;
+        std::vector<uint64_t> wq_ids, rest_ids;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (uint64_t id : take) {
+                ticket_state* t = s->tickets[id].get();
+                prepared P;
+                if (prepare_request(t->req, P) == MP_OK && P.kind == KIND_TILE && !t->req.fp64_check) {
+                    wq.push_back(t->req);
+                    wq_ids.push_back(id);
+                } else {
+                    rest_ids.push_back(id);
+                }
+            }
+        }
+        std::vector<mp_result> res(wq.size());
+        int rc = wq.empty() ? MP_OK : mp_hip_price_batch(own, wq.data(), wq.size(), res.data());
+        std::vector<mp_result> rres(rest_ids.size());
+        std::vector<int> rrc(rest_ids.size(), MP_OK);
+        for (size_t i = 0; i < rest_ids.size(); ++i) {
+            mp_request rq;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                rq = s->tickets[rest_ids[i]]->req;
+            }
+            rrc[i] = mp_hip_price(own, &rq, &rres[i]);
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (size_t i = 0; i < wq_ids.size(); ++i) {
+                ticket_state* t = s->tickets[wq_ids[i]].get();
+                t->res = res[i];
+                if (rc != MP_OK) t->res.status = rc;
+                t->done = 1;
+            }
+            for (size_t i = 0; i < rest_ids.size(); ++i) {
+                ticket_state* t = s->tickets[rest_ids[i]].get();
+                t->res = rres[i];
+                if (rrc[i] != MP_OK) t->res.status = rrc[i];
+                t->done = 1;
+            }
+            s->admitted_bytes -= bytes;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+        (void)slot;
+    }
+    mp_hip_close(own);
+}
+
+}  // namespace
+
+MP_EXPORT int mp_hip_server_open(mp_ctx* c, const mp_serve_config* cfg, mp_server** out) {
+    if (!c || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    std::unique_ptr<mp_server> s(new mp_server());
+    s->ctx = c;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
+    if (s->cfg.max_inflight > 8) s->cfg.max_inflight = 8;
+    if (s->cfg.max_batch <= 0) s->cfg.max_batch = 4096;
+    if (s->cfg.hbm_budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        s->cfg.hbm_budget_bytes = (uint64_t)(fr * 0.75);
+    }
+    for (int i = 0; i < s->cfg.max_inflight; ++i) s->workers.emplace_back(server_worker, s.get(), i);
+    *out = s.release();
+    return MP_OK;
+}
+
+MP_EXPORT void mp_hip_server_close(mp_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    delete s;
+}
+
+MP_EXPORT int mp_hip_submit(mp_server* s, const mp_request* req, uint64_t* ticket) {
+    if (!s || !req || !ticket) return fail(MP_E_INVALID_CONFIG, "null argument");
+    std::unique_ptr<ticket_state> t(new ticket_state());
+    t->req = *req;
+    if (req->exercise_steps && req->n_exercise > 0) {
+        t->schedule.assign(req->exercise_steps, req->exercise_steps + req->n_exercise);
+        t->req.exercise_steps = t->schedule.data();
+    }
+    int rc = mp_hip_request_footprint(&t->req, &t->footprint);
+    if (rc != MP_OK) return rc;
+    if (t->footprint > s->cfg.hbm_budget_bytes)
+        return fail(MP_E_OOM, "request footprint exceeds the server's HBM budget");
+    std::memset(&t->res, 0, sizeof(t->res));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        const uint64_t id = s->next_ticket++;
+        s->tickets[id] = std::move(t);
+        s->pending.push_back(id);
+        *ticket = id;
+    }
+    s->cv_work.notify_one();
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_poll(mp_server* s, uint64_t ticket, mp_result* out, int* done) {
+    if (!s || !out || !done) return fail(MP_E_INVALID_CONFIG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(MP_E_NOT_FOUND, "unknown ticket");
+    *done = it->second->done;
+    if (*done) {
+        *out = it->second->res;
+        s->tickets.erase(it);
+    }
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_wait(mp_server* s, uint64_t ticket, mp_result* out) {
+    if (!s || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(MP_E_NOT_FOUND, "unknown ticket");
+    ticket_state* t = it->second.get();
+    s->cv_done.wait(lk, [&] { return t->done != 0; });
+    *out = t->res;
+    s->tickets.erase(ticket);
+    return out->status == MP_OK ? MP_OK : fail(out->status, "request failed");
+}
+
+MP_EXPORT int mp_hip_cancel(mp_server* s, uint64_t ticket) {
+    if (!s) return fail(MP_E_INVALID_CONFIG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = std::find(s->pending.begin(), s->pending.end(), ticket);
+    if (it == s->pending.end()) return fail(MP_E_NOT_FOUND, "ticket is not pending");
+    s->pending.erase(it);
+    ticket_state* t = s->tickets[ticket].get();
+    t->res.status = MP_E_CANCELLED;
+    t->done = 1;
+    s->cv_done.notify_all();
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_serve_trace(mp_ctx* c, const mp_serve_config* cfg, const mp_request* reqs,
+                                 const double* arrival_s, size_t n, mp_result* results,
+                                 double* latency_ms, mp_serve_stats* stats) {
+    if (!c || !reqs || !arrival_s || !results || !latency_ms || !stats)
+        return fail(MP_E_INVALID_CONFIG, "null argument");
+    mp_server* s = nullptr;
+    int rc = mp_hip_server_open(c, cfg, &s);
+    if (rc != MP_OK) return rc;
+    std::vector<uint64_t> tickets(n);
+    std::vector<std::chrono::steady_clock::time_point> t_arr(n);
+    std::vector<int> got(n, 0);
+    const auto start = std::chrono::steady_clock::now();
+    size_t next = 0, finished = 0;
+    uint64_t path_steps = 0;
+    while (finished < n) {
+        const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        while (next < n && arrival_s[next] <= now) {
+            t_arr[next] = start + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                      std::chrono::duration<double>(arrival_s[next]));
+            rc = mp_hip_submit(s, &reqs[next], &tickets[next]);
+            if (rc != MP_OK) {
+                mp_hip_server_close(s);
+                return rc;
+            }
+            ++next;
+        }
+        for (size_t i = 0; i < next; ++i) {
+            if (got[i]) continue;
+            int done = 0;
+            rc = mp_hip_poll(s, tickets[i], &results[i], &done);
+            if (rc != MP_OK) {
+                mp_hip_server_close(s);
+                return rc;
+            }
+            if (done) {
+                got[i] = 1;
+                ++finished;
+                latency_ms[i] = std::chrono::duration<double, std::milli>(
+                                    std::chrono::steady_clock::now() - t_arr[i]).count();
+                path_steps += reqs[i].n_paths * std::max<uint64_t>(results[i].n_steps, 1);
+            }
+        }
+        if (finished < n) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    std::vector<double> lat(latency_ms, latency_ms + n);
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double q) { return lat.empty() ? 0.0 : lat[std::min(lat.size() - 1, (size_t)(q * (lat.size() - 1) + 0.5))]; };
+    std::memset(stats, 0, sizeof(*stats));
+    stats->completed = n;
+    stats->wall_s = wall;
+    stats->requests_per_s = n / wall;
+    stats->path_steps_per_s = path_steps / wall;
+    stats->p50_ms = pct(0.50);
+    stats->p99_ms = pct(0.99);
+    stats->max_ms = lat.empty() ? 0.0 : lat.back();
+    stats->launches = s->launches;
+    stats->max_admitted_bytes = s->max_admitted;
+    mp_hip_server_close(s);
+    return MP_OK;
+}

@@ -1,0 +1,216 @@
+// mp_device.h -- device helpers of the montepath HIP engine (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MP_WAVE 64
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC 2011).  Counter words:
+// (path, step block, stream, seed high word); key: (seed low word, contract).
+// ---------------------------------------------------------------------------
+struct mp_u4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ mp_u4 mp_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                           uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return mp_u4{c0, c1, c2, c3};
+}
+
+// Uniform in [2^-24, 1 - 2^-24] with 23 significant bits plus a half step:
+// exact in both float and double, so FP32 and FP64 runs see the same draws.
+__device__ __forceinline__ float mp_u01f(uint32_t x) {
+    return ((float)(x >> 9) + 0.5f) * 0x1p-23f;
+}
+__device__ __forceinline__ double mp_u01d(uint32_t x) {
+    return ((double)(x >> 9) + 0.5) * 0x1p-23;
+}
+
+// Box-Muller pair.  FP32 uses the CDNA transcendental units directly:
+// v_log_f32 is log2, v_sin/v_cos take their argument in revolutions, so
+// sin(2*pi*u) is one instruction.
+__device__ __forceinline__ void mp_box_muller(uint32_t a, uint32_t b, float& z0, float& z1) {
+    const float u1 = mp_u01f(a);
+    const float u2 = mp_u01f(b);
+    const float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    z0 = rad * __builtin_amdgcn_cosf(u2);
+    z1 = rad * __builtin_amdgcn_sinf(u2);
+}
+__device__ __forceinline__ void mp_box_muller(uint32_t a, uint32_t b, double& z0, double& z1) {
+    const double u1 = mp_u01d(a);
+    const double u2 = mp_u01d(b);
+    const double rad = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    z0 = rad * c;
+    z1 = rad * s;
+}
+
+template <class R>
+__device__ __forceinline__ void mp_normals4(uint32_t path, uint32_t block, uint32_t stream,
+                                            uint32_t seed_hi, uint32_t k0, uint32_t k1, R z[4]) {
+    const mp_u4 r = mp_philox(path, block, stream, seed_hi, k0, k1);
+    mp_box_muller(r.x, r.y, z[0], z[1]);
+    mp_box_muller(r.z, r.w, z[2], z[3]);
+}
+
+template <class R>
+__device__ __forceinline__ void mp_uniforms4(uint32_t path, uint32_t block, uint32_t stream,
+                                             uint32_t seed_hi, uint32_t k0, uint32_t k1, R u[4]) {
+    const mp_u4 r = mp_philox(path, block, stream, seed_hi, k0, k1);
+    if constexpr (sizeof(R) == 4) {
+        u[0] = mp_u01f(r.x);
+        u[1] = mp_u01f(r.y);
+        u[2] = mp_u01f(r.z);
+        u[3] = mp_u01f(r.w);
+    } else {
+        u[0] = mp_u01d(r.x);
+        u[1] = mp_u01d(r.y);
+        u[2] = mp_u01d(r.z);
+        u[3] = mp_u01d(r.w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Precision helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float mp_exp(float x) { return __expf(x); }
+__device__ __forceinline__ double mp_exp(double x) { return exp(x); }
+__device__ __forceinline__ float mp_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double mp_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float mp_log(float x) {
+    return 0.69314718055994531f * __builtin_amdgcn_logf(x);
+}
+__device__ __forceinline__ double mp_log(double x) { return log(x); }
+__device__ __forceinline__ float mp_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double mp_rcp(double x) { return 1.0 / x; }
+
+// Compensated (Kahan) running sum in the path precision.
+template <class R>
+struct mp_kahan {
+    R s, c;
+    __device__ __forceinline__ void init() { s = R(0); c = R(0); }
+    __device__ __forceinline__ void add(R x) {
+        const R y = x - c;
+        const R t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Wave reductions (DPP-backed shuffles on wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mp_wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, MP_WAVE);
+    return v;
+}
+
+// Neumaier summation of a sequence of doubles, used for every cross-tile and
+// cross-block total so that the global sums are compensated and in a fixed
+// order.
+struct mp_neumaier {
+    double s, c;
+    __device__ __host__ void init() { s = 0.0; c = 0.0; }
+    __device__ __host__ void add(double x) {
+        const double t = s + x;
+        if (fabs(s) >= fabs(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    __device__ __host__ double value() const { return s + c; }
+};
+
+// ---------------------------------------------------------------------------
+// Acklam inverse normal (the reference's inverse_standard_normal,
+// crates/mc-core/src/runtime/cpu.rs:6012-6061, including its clamp of p to
+// [f64::MIN_POSITIVE, 1 - f64::EPSILON]).  For p == 0 the FP32 form uses
+// ln(f64::MIN_POSITIVE) directly so both precisions give the same tail value.
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ R mp_acklam(R p) {
+    const R a0 = R(-3.969683028665376e1), a1 = R(2.209460984245205e2),
+            a2 = R(-2.759285104469687e2), a3 = R(1.383577518672690e2),
+            a4 = R(-3.066479806614716e1), a5 = R(2.506628277459239);
+    const R b0 = R(-5.447609879822406e1), b1 = R(1.615858368580409e2),
+            b2 = R(-1.556989798598866e2), b3 = R(6.680131188771972e1),
+            b4 = R(-1.328068155288572e1);
+    const R c0 = R(-7.784894002430293e-3), c1 = R(-3.223964580411365e-1),
+            c2 = R(-2.400758277161838), c3 = R(-2.549732539343734), c4 = R(4.374664141464968),
+            c5 = R(2.938163982698783);
+    const R d0 = R(7.784695709041462e-3), d1 = R(3.224671290700398e-1),
+            d2 = R(2.445134137142996), d3 = R(3.754408661907416);
+    const R p_low = R(0.02425);
+    const R p_high = R(1) - p_low;
+    if (p >= p_low && p <= p_high) {
+        const R q = p - R(0.5);
+        const R r = q * q;
+        const R num = (((((a0 * r + a1) * r + a2) * r + a3) * r + a4) * r + a5) * q;
+        const R den = ((((b0 * r + b1) * r + b2) * r + b3) * r + b4) * r + R(1);
+        return num * mp_rcp(den);
+    }
+    // tails: q = sqrt(-2 ln(min(p, 1-p))); p == 0 is clamped to f64::MIN_POSITIVE
+    const bool lower = p < p_low;
+    const R tail = lower ? p : R(1) - p;
+    R lnp;
+    if constexpr (sizeof(R) == 4)
+        lnp = tail > R(0) ? mp_log(tail) : R(-708.3964185322641);
+    else
+        lnp = mp_log(tail > R(2.2250738585072014e-308) ? tail : R(2.2250738585072014e-308));
+    const R q = mp_sqrt(R(-2) * lnp);
+    const R num = ((((c0 * q + c1) * q + c2) * q + c3) * q + c4) * q + c5;
+    const R den = (((d0 * q + d1) * q + d2) * q + d3) * q + R(1);
+    const R z = num * mp_rcp(den);
+    return lower ? z : -z;
+}
+
+// ---------------------------------------------------------------------------
+// Owen-scrambled Sobol (hash-based nested uniform scrambling, Burley 2020).
+// lk_scramble lets lower bits influence only higher bits; applied to
+// bit-reversed values it is a nested uniform scramble of the digits.
+// ---------------------------------------------------------------------------
+__device__ __host__ __forceinline__ uint32_t mp_hash32(uint32_t x) {
+    x ^= 0x79c68e4au;
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// scramble with an already-hashed seed
+__device__ __host__ __forceinline__ uint32_t mp_lk(uint32_t n, uint32_t hs) {
+    n ^= n * 0x3d20adeau;
+    n += hs;
+    n *= (hs >> 16) | 1u;
+    n ^= n * 0x05526c56u;
+    n ^= n * 0x53a22864u;
+    return n;
+}
+
+__device__ __forceinline__ uint32_t mp_rev32(uint32_t x) { return __builtin_bitreverse32(x); }
+
+// Shuffled 16-bit point index for a local index (< 2^16) under a seed.
+__device__ __forceinline__ uint32_t mp_sobol_shuffle(uint32_t local, uint32_t hashed_seed) {
+    return mp_rev32(mp_lk(mp_rev32(local & 0xFFFFu), hashed_seed)) & 0xFFFFu;
+}

@@ -1,0 +1,127 @@
+// mp_engine.h -- device-side request descriptors and kernel launchers shared
+// by the kernels (mp_engine.hip) and the C-ABI host layer (mp_api.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MP_NS_MAX 4            // strikes priced from one simulation
+#define MP_POISSON_TERMS 24    // cached cumulative Poisson probabilities per request
+#define MP_PARTIAL_STRIDE (2 * MP_NS_MAX)  // doubles per tile partial: (sum, sumsq) per strike
+
+enum mp_model { MP_MODEL_GBM = 0, MP_MODEL_HESTON = 1, MP_MODEL_MERTON = 2 };
+enum mp_payoff { MP_PAY_EUROPEAN = 0, MP_PAY_ASIAN = 1, MP_PAY_DOWN_AND_OUT = 2, MP_PAY_LOOKBACK = 3 };
+
+// Per-request descriptor resident in HBM.  All fields are uniform across a
+// wave, so the kernels read them with scalar loads.
+struct mp_dreq {
+    int32_t model;
+    int32_t payoff;
+    int32_t n_steps;      // steps actually walked (1 for the terminal method)
+    int32_t n_strikes;
+    int32_t scheme;       // Heston scheme
+    int32_t n_cum;        // Merton: cached cumulative Poisson terms
+    uint32_t key0, key1;  // Philox key (seed low word, contract id)
+    uint32_t seed_hi;     // Philox counter word 3
+    uint32_t tile0;       // first tile id of this request
+    uint32_t n_tiles;
+    uint32_t tile_paths;  // paths per tile (multiple of 64)
+    uint64_t n_paths;
+    double log_s0, s0;
+    double drift_dt, vol_dt;  // GBM/Merton per-step log drift (compensated for Merton) and vol
+    double log_barrier_rel;   // ln(B) - ln(S0): knock-out level in log-return units
+    int32_t knocked_at_start; // S0 <= B
+    int32_t pad0;
+    double strikes[MP_NS_MAX];
+    double discount;
+    double v0, kappa, theta, xi, rho, rho_perp, rate, dt, sqrt_dt;
+    double mu_j, sigma_j, jump_mean_dt;
+    double cum[MP_POISSON_TERMS];
+};
+
+// Brownian-bridge op in streaming (preorder) form: wm = lw*cur + rw*stack[slot] + sd*z.
+// flags bit0: left child is a leaf (emit wm - cur), bit1: right child is a leaf
+// (emit stack[slot] - wm); otherwise wm is pushed to stack[slot + 1].
+struct mp_bbop {
+    double lw, rw, sd;
+    int32_t slot;
+    int32_t flags;
+};
+
+struct mp_qmc_params {
+    uint32_t base_seed;   // low 32 bits of the request seed
+    int32_t factors;      // 1 (GBM) or 2 (Heston)
+    int32_t bridge;       // 0: dims in step order, 1: Brownian bridge
+    int32_t n_ops;
+    int32_t n_slots;
+    double sqrt_t;
+    double inv_sqrt_dt;
+};
+
+struct mp_lsm_params {
+    int32_t n_steps;
+    int32_t degree;
+    int32_t coeffs;
+    int32_t pad;
+    uint64_t n_paths;
+    double log_s0, s0, k, inv_k, drift_dt, vol_dt, disc_step, disc_m;
+    uint32_t key0, key1, seed_hi, pad2;
+};
+
+// Launch helpers (defined in mp_engine.hip).
+namespace mp {
+
+struct tile_launch {
+    const mp_dreq* reqs;
+    const uint32_t* tile_req;   // tile id -> request index
+    const uint32_t* order;      // queue position -> tile id
+    uint32_t n_tiles;
+    unsigned int* counter;
+    double* partial;            // [tile][MP_PARTIAL_STRIDE]
+    const double* normals;      // buffer source (step-major) or null
+    const double* uniforms;
+    const double* jump_normals;
+    double* terminal_out;
+    double* payoff_out;
+    int grid;
+    int block;
+    int ppl;                    // paths interleaved per lane
+};
+
+hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s);
+hipError_t launch_tiles(const tile_launch& L, int precision, hipStream_t s);
+hipError_t launch_finalize(const mp_dreq* reqs, int n_reqs, const double* partial, double* out,
+                           hipStream_t s);
+hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
+                      const mp_bbop* plan, double* partial, double* terminal_out,
+                      double* payoff_out, int precision, int grid, int block, bool nib_lds,
+                      hipStream_t s);
+size_t qmc_lds_bytes(int precision, int factors, int n_slots, int block, bool nib_lds);
+hipError_t launch_basket(const mp_dreq* req, int n_assets, const double* chol,
+                         const double* drift, const double* vol, const double* log_s0,
+                         const double* weight, const double* normals, double* partial,
+                         double* terminal_out, double* payoff_out, int precision, int grid,
+                         int block, hipStream_t s);
+hipError_t launch_lsm_paths(const mp_lsm_params& p, const double* normals, void* paths,
+                            double* cash, uint16_t* ex, int precision, int grid, int block,
+                            hipStream_t s);
+hipError_t launch_lsm_step(const mp_lsm_params& p, const void* paths, double* cash,
+                           uint16_t* ex, const double* coeff_all, const int* valid_all,
+                           int step_update, int step_gram, const double* disc_pow,
+                           const double* growth, double* gram_partial, int precision, int grid,
+                           int block, hipStream_t s);
+hipError_t launch_lsm_solve(const mp_lsm_params& p, const double* gram_partial, int n_blocks,
+                            int step, double* coeff_all, int* valid_all, hipStream_t s);
+hipError_t launch_lsm_final(const mp_lsm_params& p, const double* cash, const uint16_t* ex,
+                            double* partial, int grid, int block, hipStream_t s);
+hipError_t launch_lsm_reduce(const mp_lsm_params& p, const double* partial, int n_blocks,
+                             const int* valid_all, int allow_intrinsic, double* out,
+                             hipStream_t s);
+hipError_t launch_plain_fp64(const mp_dreq* req, double* payoffs, int grid, int block,
+                             hipStream_t s);
+hipError_t launch_plain_reduce(const double* payoffs, uint64_t n, double* partial,
+                               int n_blocks, hipStream_t s);
+hipError_t launch_draws(const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
+                        int kind, uint32_t stream, uint64_t path0, uint64_t n_paths,
+                        uint64_t n_draws, double* out, hipStream_t s);
+
+}  // namespace mp

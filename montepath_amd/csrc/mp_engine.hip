@@ -1,0 +1,1159 @@
+// mp_engine.hip -- HIP kernels of the montepath Monte Carlo path engine for
+// MI355X (gfx950, CDNA4, wave64).
+//
+// Layout and execution model
+//   * one lane per path; a wave owns a tile of tile_paths consecutive paths
+//     of one request and loops over it 64*PPL paths at a time;
+//   * waves pull tiles from a device work queue (one atomic per tile), so a
+//     launch can carry thousands of uneven requests (persistent kernel);
+//   * each tile writes (sum, sum of squares) per strike in FP64 to its own
+//     slot; a finalize kernel sums a request's tiles in tile order with
+//     Neumaier compensation, so results are bit-identical however tiles were
+//     scheduled and whatever else shared the launch;
+//   * path arithmetic in R = float or double; normals from Philox4x32-10 in
+//     registers, from a caller buffer (parity), or from Owen-scrambled Sobol
+//     with a streaming Brownian bridge (LDS).
+#include "mp_device.h"
+#include "mp_engine.h"
+
+namespace {
+
+constexpr int kNibWords = 256 * 4 * 16;  // Sobol nibble tables: [dim][nibble position][16]
+
+// ---------------------------------------------------------------------------
+// Per-path model state
+// ---------------------------------------------------------------------------
+template <class R>
+struct path_state {
+    R x;             // log-return since t0
+    R v;             // Heston variance
+    mp_kahan<R> acc; // Asian running sum of S/S0
+    R xmax;          // lookback: max log-return (>= 0, S0 included)
+    bool knocked;    // down-and-out
+};
+
+template <class R>
+struct model_params {
+    R drift_dt, vol_dt, xb;
+    R kappa_dt, theta, xi_sqdt, rho, rho_perp, rate_dt, half_dt, sqrt_dt;
+    R mu_j, sigma_j;
+    int payoff, scheme, model;
+};
+
+template <class R>
+__device__ __forceinline__ model_params<R> load_params(const mp_dreq& q) {
+    model_params<R> m;
+    m.drift_dt = R(q.drift_dt);
+    m.vol_dt = R(q.vol_dt);
+    m.xb = R(q.log_barrier_rel);
+    m.kappa_dt = R(q.kappa * q.dt);
+    m.theta = R(q.theta);
+    m.xi_sqdt = R(q.xi * q.sqrt_dt);
+    m.rho = R(q.rho);
+    m.rho_perp = R(q.rho_perp);
+    m.rate_dt = R(q.rate * q.dt);
+    m.half_dt = R(0.5 * q.dt);
+    m.sqrt_dt = R(q.sqrt_dt);
+    m.mu_j = R(q.mu_j);
+    m.sigma_j = R(q.sigma_j);
+    m.payoff = q.payoff;
+    m.scheme = q.scheme;
+    m.model = q.model;
+    return m;
+}
+
+template <class R>
+__device__ __forceinline__ void state_init(path_state<R>& st, const mp_dreq& q) {
+    st.x = R(0);
+    st.v = R(q.v0);
+    st.acc.init();
+    st.xmax = R(0);
+    st.knocked = q.knocked_at_start != 0;
+}
+
+// Path-dependent observation after a step (payoff is wave-uniform).
+template <class R>
+__device__ __forceinline__ void observe(path_state<R>& st, const model_params<R>& m) {
+    if (m.payoff == MP_PAY_ASIAN) {
+        st.acc.add(mp_exp(st.x));
+    } else if (m.payoff == MP_PAY_DOWN_AND_OUT) {
+        st.knocked = st.knocked || (st.x <= m.xb);
+    } else if (m.payoff == MP_PAY_LOOKBACK) {
+        st.xmax = st.x > st.xmax ? st.x : st.xmax;
+    }
+}
+
+// GBM log-Euler step (reference cpu.rs:6464: log_s += drift_dt + vol_dt * z).
+template <class R>
+__device__ __forceinline__ void gbm_step(path_state<R>& st, const model_params<R>& m, R z) {
+    st.x += m.drift_dt + m.vol_dt * z;
+    observe(st, m);
+}
+
+// Heston Euler step (reference heston_path_terminal cpu.rs:8674-8683; scheme 1
+// keeps the unfloored variance -- full truncation).
+template <class R>
+__device__ __forceinline__ void heston_step(path_state<R>& st, const model_params<R>& m, R z_var,
+                                            R z_ind) {
+    const R vp = st.v > R(0) ? st.v : R(0);
+    const R sv = mp_sqrt(vp);
+    const R z_price = m.rho * z_var + m.rho_perp * z_ind;
+    st.x += (m.rate_dt - m.half_dt * vp) + sv * m.sqrt_dt * z_price;
+    R v = st.v + m.kappa_dt * (m.theta - vp) + m.xi_sqdt * sv * z_var;
+    if (m.scheme == 0) v = v > R(0) ? v : R(0);
+    st.v = v;
+    observe(st, m);
+}
+
+// Poisson count by inversion of u against the request's cumulative
+// probabilities (reference sample_poisson cpu.rs:932-953; the same recursion
+// in FP64, so both precisions count identical jumps).
+__device__ __forceinline__ int poisson_count(double u, const mp_dreq& q) {
+    if (q.jump_mean_dt == 0.0) return 0;
+    int n = 0;
+    const int nc = q.n_cum;
+    while (n < nc && u > q.cum[n]) ++n;
+    if (n < nc) return n;
+    // beyond the cached terms: continue the reference recursion
+    double prob = q.cum[nc - 1] - q.cum[nc - 2];
+    double cum = q.cum[nc - 1];
+    int k = nc - 1;
+    while (u > cum && k < 4095) {
+        k += 1;
+        prob *= q.jump_mean_dt / (double)k;
+        cum += prob;
+    }
+    return k;
+}
+
+// Merton stepped jump diffusion: compensated drift, diffusion and a
+// compound-Poisson jump N*mu_j + sigma_j*sqrt(N)*zj.
+template <class R>
+__device__ __forceinline__ void merton_apply(path_state<R>& st, const model_params<R>& m, R z,
+                                             int n_jumps, R zj) {
+    R shock = m.drift_dt + m.vol_dt * z;
+    if (n_jumps != 0) shock += R(n_jumps) * m.mu_j + m.sigma_j * mp_sqrt(R(n_jumps)) * zj;
+    st.x += shock;
+    observe(st, m);
+}
+
+// Undiscounted payoff for strike k (discounting happens once, in FP64, in the
+// finalize kernel).
+template <class R>
+__device__ __forceinline__ R payoff_of(const path_state<R>& st, const model_params<R>& m,
+                                       const mp_dreq& q, int n_steps, R s0, R k) {
+    if (m.payoff == MP_PAY_ASIAN) {
+        const R avg = s0 * (st.acc.s * (R(1) / R(n_steps)));
+        return avg > k ? avg - k : R(0);
+    }
+    if (m.payoff == MP_PAY_LOOKBACK) {
+        const R smax = s0 * mp_exp(st.xmax);
+        return smax > k ? smax - k : R(0);
+    }
+    const R sT = s0 * mp_exp(st.x);
+    if (m.payoff == MP_PAY_DOWN_AND_OUT && st.knocked) return R(0);
+    return sT > k ? sT - k : R(0);
+}
+
+// ---------------------------------------------------------------------------
+// Work-queue tile kernel (GBM / Heston / Merton).  SRC 0: Philox, 1: buffer.
+// ---------------------------------------------------------------------------
+template <class R, int SRC, int PPL>
+__global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    for (;;) {
+        unsigned int qpos = 0;
+        if (lane == 0) qpos = atomicAdd(L.counter, 1u);
+        qpos = __builtin_amdgcn_readfirstlane(qpos);
+        if (qpos >= L.n_tiles) break;
+        const uint32_t tile = L.order ? L.order[qpos] : qpos;
+        const uint32_t ri = L.tile_req[tile];
+        const mp_dreq& q = L.reqs[ri];
+        const model_params<R> m = load_params<R>(q);
+        const int n_steps = q.n_steps;
+        const int ns = q.n_strikes;
+        const R s0 = R(q.s0);
+        R kk[MP_NS_MAX];
+#pragma unroll
+        for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = R(q.strikes[j]);
+        const uint64_t pbeg = (uint64_t)(tile - q.tile0) * q.tile_paths;
+        const uint64_t pend = min(pbeg + (uint64_t)q.tile_paths, q.n_paths);
+
+        double sum[MP_NS_MAX], sq[MP_NS_MAX];
+#pragma unroll
+        for (int j = 0; j < MP_NS_MAX; ++j) sum[j] = sq[j] = 0.0;
+
+        for (uint64_t base = pbeg; base < pend; base += (uint64_t)MP_WAVE * PPL) {
+            path_state<R> st[PPL];
+            uint64_t pid[PPL];
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                pid[j] = base + lane + (uint64_t)MP_WAVE * j;
+                state_init(st[j], q);
+            }
+            for (int s0i = 0; s0i < n_steps; s0i += 4) {
+                const uint32_t blk = (uint32_t)(s0i >> 2);
+                const int nsub = min(4, n_steps - s0i);
+#pragma unroll
+                for (int j = 0; j < PPL; ++j) {
+                    const uint64_t p = pid[j];
+                    const uint64_t pr = p < pend ? p : pend - 1;  // clamp buffer reads
+                    if (m.model == MP_MODEL_GBM) {
+                        R z[4];
+                        if constexpr (SRC == 0) {
+                            mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, z);
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                z[e] = e < nsub ? R(L.normals[(uint64_t)(s0i + e) * q.n_paths + pr]) : R(0);
+                        }
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (e < nsub) gbm_step(st[j], m, z[e]);
+                    } else if (m.model == MP_MODEL_HESTON) {
+                        R zv[4], zi[4];
+                        if constexpr (SRC == 0) {
+                            mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, zv);
+                            mp_normals4<R>((uint32_t)p, blk, 1u, q.seed_hi, q.key0, q.key1, zi);
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                const uint64_t row = (uint64_t)(s0i + e) * 2;
+                                zv[e] = e < nsub ? R(L.normals[row * q.n_paths + pr]) : R(0);
+                                zi[e] = e < nsub ? R(L.normals[(row + 1) * q.n_paths + pr]) : R(0);
+                            }
+                        }
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (e < nsub) heston_step(st[j], m, zv[e], zi[e]);
+                    } else {  // Merton
+                        R z[4];
+                        double u[4];
+                        if constexpr (SRC == 0) {
+                            mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, z);
+                            mp_uniforms4<double>((uint32_t)p, blk, 1u, q.seed_hi, q.key0, q.key1, u);
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                const uint64_t idx = (uint64_t)(s0i + e) * q.n_paths + pr;
+                                z[e] = e < nsub ? R(L.normals[idx]) : R(0);
+                                u[e] = e < nsub ? L.uniforms[idx] : 0.0;
+                            }
+                        }
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            if (e < nsub) {
+                                const int nj = poisson_count(u[e], q);
+                                R zj = R(0);
+                                if (nj != 0) {
+                                    if constexpr (SRC == 0) {
+                                        R zz[4];
+                                        mp_normals4<R>((uint32_t)p, blk, 2u, q.seed_hi, q.key0, q.key1, zz);
+                                        zj = zz[e];
+                                    } else {
+                                        zj = R(L.jump_normals[(uint64_t)(s0i + e) * q.n_paths + pr]);
+                                    }
+                                }
+                                merton_apply(st[j], m, z[e], nj, zj);
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                if (pid[j] < pend) {
+#pragma unroll
+                    for (int sidx = 0; sidx < MP_NS_MAX; ++sidx) {
+                        if (sidx < ns) {
+                            const double pay = (double)payoff_of(st[j], m, q, n_steps, s0, kk[sidx]);
+                            sum[sidx] += pay;
+                            sq[sidx] += pay * pay;
+                        }
+                    }
+                    if (L.terminal_out) L.terminal_out[pid[j]] = (double)(s0 * mp_exp(st[j].x));
+                    if (L.payoff_out)
+                        L.payoff_out[pid[j]] = (double)payoff_of(st[j], m, q, n_steps, s0, kk[0]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < MP_NS_MAX; ++j) {
+            sum[j] = mp_wave_sum(sum[j]);
+            sq[j] = mp_wave_sum(sq[j]);
+        }
+        if (lane == 0) {
+            double* out = L.partial + (uint64_t)tile * MP_PARTIAL_STRIDE;
+#pragma unroll
+            for (int j = 0; j < MP_NS_MAX; ++j) {
+                out[2 * j] = sum[j];
+                out[2 * j + 1] = sq[j];
+            }
+        }
+    }
+}
+
+__global__ void reset_kernel(unsigned int* counter, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counter[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Finalize: one wave per request; tiles summed in tile order (Neumaier), then
+// price = discount * mean and the population-variance standard error of the
+// reference's summarize_payoffs (cpu.rs:7641-7652), all in FP64.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void finalize_kernel(const mp_dreq* reqs, const double* partial,
+                                                     double* out) {
+    __shared__ double red[2][MP_WAVE];
+    const mp_dreq& q = reqs[blockIdx.x];
+    const int lane = threadIdx.x;
+    const double n = (double)q.n_paths;
+    for (int j = 0; j < MP_NS_MAX; ++j) {
+        mp_neumaier a, b;
+        a.init();
+        b.init();
+        if (j < q.n_strikes) {
+            for (uint32_t t = lane; t < q.n_tiles; t += MP_WAVE) {
+                const double* p = partial + (uint64_t)(q.tile0 + t) * MP_PARTIAL_STRIDE;
+                a.add(p[2 * j]);
+                b.add(p[2 * j + 1]);
+            }
+        }
+        red[0][lane] = a.value();
+        red[1][lane] = b.value();
+        __syncthreads();
+        if (lane == 0) {
+            mp_neumaier s, s2;
+            s.init();
+            s2.init();
+            for (int i = 0; i < MP_WAVE; ++i) {
+                s.add(red[0][i]);
+                s2.add(red[1][i]);
+            }
+            double price = 0.0, se = 0.0;
+            if (j < q.n_strikes) {
+                const double mean = s.value() / n;
+                const double var = s2.value() / n - mean * mean;
+                price = q.discount * mean;
+                se = q.discount * sqrt(var > 0.0 ? var : 0.0) / sqrt(n);
+            }
+            out[(uint64_t)blockIdx.x * MP_PARTIAL_STRIDE + 2 * j] = price;
+            out[(uint64_t)blockIdx.x * MP_PARTIAL_STRIDE + 2 * j + 1] = se;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide FP64 sum of up to 8 values per lane into partial[tile][*].
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* dst) {
+    __shared__ double red[16][NV];
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = mp_wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[wave][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double acc = 0.0;
+        for (int w = 0; w < nw; ++w) acc += red[w][threadIdx.x];
+        dst[threadIdx.x] = acc;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Sobol + Brownian bridge kernel (GBM with F = 1, Heston with F = 2).
+// One block per tile of blockDim paths (grid-stride, static assignment);
+// nibble tables of the direction numbers in LDS, bridge stack per lane in LDS,
+// plan ops read with scalar loads.
+// ---------------------------------------------------------------------------
+template <class R>
+struct sobol_cursor {
+    uint32_t bseed;   // block seed: base + block * 0x9E3779B9
+    uint32_t local;   // index within the 2^16-point block
+    int off;          // current dimension offset (dim / 256)
+    uint32_t n0, n1, n2, n3;
+    __device__ __forceinline__ void set_offset(int o) {
+        off = o;
+        const uint32_t s = mp_sobol_shuffle(local, mp_hash32(bseed + (uint32_t)o));
+        n0 = s & 15u;
+        n1 = (s >> 4) & 15u;
+        n2 = (s >> 8) & 15u;
+        n3 = (s >> 12) & 15u;
+    }
+    __device__ __forceinline__ uint32_t raw(int g, const uint32_t* nib) {
+        const int o = g >> 8;
+        if (o != off) set_offset(o);
+        const uint32_t dd = (uint32_t)(g & 255);
+        const uint32_t* t = nib + dd * 64u;
+        uint32_t x = t[n0] ^ t[16 + n1] ^ t[32 + n2] ^ t[48 + n3];
+        const uint32_t hs = mp_hash32(dd ^ (bseed + (uint32_t)o));
+        return mp_rev32(mp_lk(mp_rev32(x), hs));
+    }
+    __device__ __forceinline__ R normal(int g, const uint32_t* nib) {
+        const uint32_t x = raw(g, nib);
+        const R u = R(x >> 9) * R(0x1p-23);
+        return mp_acklam<R>(u);
+    }
+};
+
+template <class R, int F, bool NIB_LDS>
+__global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ reqp,
+                                                  mp_qmc_params qp,
+                                                  const uint32_t* __restrict__ nib_g,
+                                                  const mp_bbop* __restrict__ plan,
+                                                  double* partial, double* terminal_out,
+                                                  double* payoff_out) {
+    extern __shared__ uint32_t smem[];
+    const uint32_t* nib = nib_g;
+    R* stack = reinterpret_cast<R*>(smem);
+    if constexpr (NIB_LDS) {
+        for (int i = threadIdx.x; i < kNibWords; i += blockDim.x) smem[i] = nib_g[i];
+        nib = smem;
+        stack = reinterpret_cast<R*>(smem + kNibWords);
+        __syncthreads();
+    }
+
+    const mp_dreq& q = *reqp;
+    const model_params<R> m = load_params<R>(q);
+    const int n_steps = q.n_steps;
+    const int ns = q.n_strikes;
+    const R s0 = R(q.s0);
+    R kk[MP_NS_MAX];
+#pragma unroll
+    for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = R(q.strikes[j]);
+    const R sqrt_t = R(qp.sqrt_t);
+    const R inv_sqrt_dt = R(qp.inv_sqrt_dt);
+    const int bd = blockDim.x;
+    const int tid = threadIdx.x;
+
+    for (uint32_t tile = blockIdx.x; tile < q.n_tiles; tile += gridDim.x) {
+        const uint64_t p = (uint64_t)tile * bd + tid;
+        const bool active = p < q.n_paths;
+        sobol_cursor<R> cur_s;
+        cur_s.bseed = qp.base_seed + (uint32_t)(p >> 16) * 0x9E3779B9u;
+        cur_s.local = (uint32_t)(p & 0xFFFFu);
+        cur_s.set_offset(0);
+
+        path_state<R> st;
+        state_init(st, q);
+        auto emit = [&](R i0, R i1) {
+            if constexpr (F == 1)
+                gbm_step(st, m, i0);
+            else
+                heston_step(st, m, i0, i1);
+        };
+        if (!qp.bridge) {
+            for (int s = 0; s < n_steps; ++s) {
+                const R z0 = cur_s.normal(s * F, nib);
+                const R z1 = F == 2 ? cur_s.normal(s * F + 1, nib) : R(0);
+                emit(z0, z1);
+            }
+        } else {
+            R cur[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const R wT = sqrt_t * cur_s.normal(f, nib);
+                stack[(0 * F + f) * bd + tid] = wT;
+                cur[f] = R(0);
+            }
+            if (qp.n_ops == 0) {
+                const R i0 = (stack[tid] - cur[0]) * inv_sqrt_dt;
+                const R i1 = F == 2 ? (stack[bd + tid] - cur[F - 1]) * inv_sqrt_dt : R(0);
+                emit(i0, i1);
+            }
+            for (int k = 0; k < qp.n_ops; ++k) {
+                const mp_bbop op = plan[k];
+                const R lw = R(op.lw), rw = R(op.rw), sd = R(op.sd);
+                R wm[F], wr[F];
+#pragma unroll
+                for (int f = 0; f < F; ++f) {
+                    const R z = cur_s.normal((k + 1) * F + f, nib);
+                    wr[f] = stack[(op.slot * F + f) * bd + tid];
+                    wm[f] = lw * cur[f] + rw * wr[f] + sd * z;
+                }
+                if (op.flags & 1) {
+                    emit((wm[0] - cur[0]) * inv_sqrt_dt,
+                         F == 2 ? (wm[F - 1] - cur[F - 1]) * inv_sqrt_dt : R(0));
+#pragma unroll
+                    for (int f = 0; f < F; ++f) cur[f] = wm[f];
+                    if (op.flags & 2) {
+                        emit((wr[0] - cur[0]) * inv_sqrt_dt,
+                             F == 2 ? (wr[F - 1] - cur[F - 1]) * inv_sqrt_dt : R(0));
+#pragma unroll
+                        for (int f = 0; f < F; ++f) cur[f] = wr[f];
+                    }
+                } else {
+#pragma unroll
+                    for (int f = 0; f < F; ++f) stack[((op.slot + 1) * F + f) * bd + tid] = wm[f];
+                }
+            }
+        }
+        double v[2 * MP_NS_MAX];
+#pragma unroll
+        for (int j = 0; j < MP_NS_MAX; ++j) {
+            double pay = 0.0;
+            if (active && j < ns) pay = (double)payoff_of(st, m, q, n_steps, s0, kk[j]);
+            v[2 * j] = pay;
+            v[2 * j + 1] = pay * pay;
+        }
+        if (active) {
+            if (terminal_out) terminal_out[p] = (double)(s0 * mp_exp(st.x));
+            if (payoff_out) payoff_out[p] = (double)payoff_of(st, m, q, n_steps, s0, kk[0]);
+        }
+        block_sum_store<2 * MP_NS_MAX>(v, partial + (uint64_t)tile * MP_PARTIAL_STRIDE);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// n-asset basket (stepped, Cholesky-correlated shocks), NA assets.
+// Philox stream a drives asset a's independent normals.
+// ---------------------------------------------------------------------------
+template <class R, int NA, int SRC>
+__global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__ reqp,
+                                                    const double* __restrict__ chol,
+                                                    const double* __restrict__ drift,
+                                                    const double* __restrict__ vol,
+                                                    const double* __restrict__ logs0,
+                                                    const double* __restrict__ weight,
+                                                    const double* __restrict__ normals,
+                                                    double* partial, double* terminal_out,
+                                                    double* payoff_out) {
+    const mp_dreq& q = *reqp;
+    R L[NA][NA], dr[NA], vo[NA], w[NA], s0a[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        dr[a] = R(drift[a]);
+        vo[a] = R(vol[a]);
+        w[a] = R(weight[a]);
+        s0a[a] = R(exp(logs0[a]));
+#pragma unroll
+        for (int b = 0; b < NA; ++b) L[a][b] = R(chol[a * NA + b]);
+    }
+    const int n_steps = q.n_steps;
+    const R kk = R(q.strikes[0]);
+    for (uint32_t tile = blockIdx.x; tile < q.n_tiles; tile += gridDim.x) {
+        const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
+        const bool active = p < q.n_paths;
+        const uint64_t pr = active ? p : q.n_paths - 1;
+        R x[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) x[a] = R(0);
+        for (int s0i = 0; s0i < n_steps; s0i += 4) {
+            const int nsub = min(4, n_steps - s0i);
+            R z[NA][4];
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                if constexpr (SRC == 0) {
+                    mp_normals4<R>((uint32_t)p, (uint32_t)(s0i >> 2), (uint32_t)a, q.seed_hi,
+                                   q.key0, q.key1, z[a]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        z[a][e] = e < nsub ? R(normals[((uint64_t)(s0i + e) * NA + a) * q.n_paths + pr]) : R(0);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (e < nsub) {
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) {
+                        R c = R(0);
+#pragma unroll
+                        for (int b = 0; b <= a; ++b) c += L[a][b] * z[b][e];
+                        x[a] += dr[a] + vo[a] * c;
+                    }
+                }
+            }
+        }
+        R basket = R(0);
+#pragma unroll
+        for (int a = 0; a < NA; ++a) basket += w[a] * (s0a[a] * mp_exp(x[a]));
+        const R pay = basket > kk ? basket - kk : R(0);
+        double v[2];
+        v[0] = active ? (double)pay : 0.0;
+        v[1] = v[0] * v[0];
+        if (active) {
+            if (terminal_out) terminal_out[p] = (double)basket;
+            if (payoff_out) payoff_out[p] = (double)pay;
+        }
+        double vv[2] = {v[0], v[1]};
+        block_sum_store<2>(vv, partial + (uint64_t)tile * MP_PARTIAL_STRIDE);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Longstaff-Schwartz put.  Paths stored step-major [step][path] (steps
+// 1..M-1), cash flows kept discounted to t0 in FP64, exercise step per path
+// in 16 bits.
+// ---------------------------------------------------------------------------
+template <class R, int SRC>
+__global__ __launch_bounds__(256) void lsm_paths_kernel(mp_lsm_params P,
+                                                       const double* __restrict__ normals,
+                                                       R* __restrict__ paths, double* cash,
+                                                       uint16_t* ex) {
+    const uint64_t n = P.n_paths;
+    const R drift = R(P.drift_dt), vol = R(P.vol_dt), s0 = R(P.s0);
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        R x = R(0);
+        R sT = s0;
+        for (int s0i = 0; s0i < P.n_steps; s0i += 4) {
+            R z[4];
+            const int nsub = min(4, P.n_steps - s0i);
+            if constexpr (SRC == 0) {
+                mp_normals4<R>((uint32_t)p, (uint32_t)(s0i >> 2), 0u, P.seed_hi, P.key0, P.key1, z);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    z[e] = e < nsub ? R(normals[(uint64_t)(s0i + e) * n + p]) : R(0);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (e < nsub) {
+                    x += drift + vol * z[e];
+                    const int step = s0i + e + 1;
+                    const R S = s0 * mp_exp(x);
+                    if (step < P.n_steps) paths[(uint64_t)(step - 1) * n + p] = S;
+                    sT = S;
+                }
+            }
+        }
+        const double d = P.k - (double)sT;
+        cash[p] = (d > 0.0 ? d : 0.0) * P.disc_m;  // terminal cash flow discounted to t0
+        ex[p] = (uint16_t)P.n_steps;
+    }
+}
+
+__device__ __forceinline__ void laguerre(double x, int degree, double b[4]) {
+    // laguerre_lsm_basis cpu.rs:7667-7682 (unweighted, x = S/K)
+    const double x2 = x * x;
+    const double x3 = x2 * x;
+    b[0] = 1.0;
+    b[1] = degree >= 1 ? 1.0 - x : 0.0;
+    b[2] = degree >= 2 ? 1.0 - 2.0 * x + 0.5 * x2 : 0.0;
+    b[3] = degree >= 3 ? 1.0 - 3.0 * x + 1.5 * x2 - x3 / 6.0 : 0.0;
+}
+
+// One pass per exercise date: apply the exercise rule of date step_update
+// (if >= 1) and accumulate the Gram matrix / right-hand side of date
+// step_gram (if >= 1) over in-the-money paths.  cash holds cash flows
+// discounted to t0; growth[s] = disc_step^-s brings them to date s.
+template <class R>
+__global__ __launch_bounds__(256) void lsm_step_kernel(mp_lsm_params P, const R* __restrict__ paths,
+                                                      double* cash, uint16_t* ex,
+                                                      const double* __restrict__ coeff_all,
+                                                      const int* __restrict__ valid_all,
+                                                      int step_update, int step_gram,
+                                                      const double* __restrict__ disc_pow,
+                                                      const double* __restrict__ growth,
+                                                      double* gram_partial) {
+    const uint64_t n = P.n_paths;
+    const double k = P.k, inv_k = P.inv_k;
+    const int deg = P.degree, nc = P.coeffs;
+    const bool do_update = step_update >= 1 && valid_all[step_update] != 0;
+    double beta[4] = {0.0, 0.0, 0.0, 0.0};
+    if (do_update) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) beta[i] = coeff_all[step_update * 4 + i];
+    }
+    const double dpow_u = step_update >= 1 ? disc_pow[step_update] : 0.0;
+    const double grow_g = step_gram >= 1 ? growth[step_gram] : 0.0;
+    // 10 unique Gram entries + 4 RHS + count
+    double acc[15];
+#pragma unroll
+    for (int i = 0; i < 15; ++i) acc[i] = 0.0;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        double c = cash[p];
+        if (do_update) {
+            const double s = (double)paths[(uint64_t)(step_update - 1) * n + p];
+            const double imm = k - s > 0.0 ? k - s : 0.0;
+            if (imm > 0.0) {
+                double b[4];
+                laguerre(s * inv_k, deg, b);
+                double cont = 0.0;
+                for (int i = 0; i < nc; ++i) cont += beta[i] * b[i];
+                if (imm > (cont > 0.0 ? cont : 0.0)) {
+                    c = imm * dpow_u;
+                    cash[p] = c;
+                    ex[p] = (uint16_t)step_update;
+                }
+            }
+        }
+        if (step_gram >= 1) {
+            const double s = (double)paths[(uint64_t)(step_gram - 1) * n + p];
+            const double imm = k - s > 0.0 ? k - s : 0.0;
+            if (imm > 0.0) {
+                double b[4];
+                laguerre(s * inv_k, deg, b);
+                const double cont = c * grow_g;
+                int e = 0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = i; j < 4; ++j) acc[e++] += b[i] * b[j];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[10 + i] += b[i] * cont;
+                acc[14] += 1.0;
+            }
+        }
+    }
+    if (step_gram >= 1) {
+        // block reduce 15 doubles
+        __shared__ double red[4][15];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int i = 0; i < 15; ++i) acc[i] = mp_wave_sum(acc[i]);
+        if (lane == 0)
+            for (int i = 0; i < 15; ++i) red[wave][i] = acc[i];
+        __syncthreads();
+        if (threadIdx.x < 15) {
+            double t = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w][threadIdx.x];
+            gram_partial[(uint64_t)blockIdx.x * 16 + threadIdx.x] = t;
+        }
+    }
+}
+
+// Gauss-Jordan with partial pivoting, failing when the best pivot <= 1e-12
+// (solve_lsm_normal_equations cpu.rs:7684-7736).  Single lane.
+__device__ bool lsm_solve4(double m[4][4], double rhs[4], int dim) {
+    for (int piv = 0; piv < dim; ++piv) {
+        int best = piv;
+        double best_abs = fabs(m[piv][piv]);
+        for (int row = piv + 1; row < dim; ++row) {
+            const double c = fabs(m[row][piv]);
+            if (c > best_abs) {
+                best = row;
+                best_abs = c;
+            }
+        }
+        if (best_abs <= 1e-12) return false;
+        if (best != piv) {
+            for (int j = 0; j < 4; ++j) {
+                const double t = m[piv][j];
+                m[piv][j] = m[best][j];
+                m[best][j] = t;
+            }
+            const double t = rhs[piv];
+            rhs[piv] = rhs[best];
+            rhs[best] = t;
+        }
+        const double pivot = m[piv][piv];
+        for (int c = piv; c < dim; ++c) m[piv][c] /= pivot;
+        rhs[piv] /= pivot;
+        for (int row = 0; row < dim; ++row) {
+            if (row == piv) continue;
+            const double f = m[row][piv];
+            if (f == 0.0) continue;
+            for (int c = piv; c < dim; ++c) m[row][c] -= f * m[piv][c];
+            rhs[row] -= f * rhs[piv];
+        }
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(64) void lsm_solve_kernel(mp_lsm_params P, const double* gram_partial,
+                                                      int n_blocks, int step, double* coeff_all,
+                                                      int* valid_all) {
+    __shared__ double red[15][MP_WAVE];
+    const int lane = threadIdx.x;
+    mp_neumaier acc[15];
+    for (int i = 0; i < 15; ++i) acc[i].init();
+    for (int b = lane; b < n_blocks; b += MP_WAVE)
+        for (int i = 0; i < 15; ++i) acc[i].add(gram_partial[(uint64_t)b * 16 + i]);
+    for (int i = 0; i < 15; ++i) red[i][lane] = acc[i].value();
+    __syncthreads();
+    if (lane == 0) {
+        double tot[15];
+        for (int i = 0; i < 15; ++i) {
+            mp_neumaier s;
+            s.init();
+            for (int l = 0; l < MP_WAVE; ++l) s.add(red[i][l]);
+            tot[i] = s.value();
+        }
+        double m[4][4], rhs[4];
+        int e = 0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = i; j < 4; ++j) {
+                m[i][j] = tot[e];
+                m[j][i] = tot[e];
+                ++e;
+            }
+        for (int i = 0; i < 4; ++i) rhs[i] = tot[10 + i];
+        const double itm = tot[14];
+        int ok = 0;
+        if (itm >= (double)P.coeffs) ok = lsm_solve4(m, rhs, P.coeffs) ? 1 : 0;
+        for (int i = 0; i < 4; ++i) coeff_all[step * 4 + i] = (ok && i < P.coeffs) ? rhs[i] : 0.0;
+        valid_all[step] = ok;
+    }
+}
+
+// Sums of discounted cash flows and exercise counts.
+__global__ __launch_bounds__(256) void lsm_final_kernel(mp_lsm_params P, const double* cash,
+                                                       const uint16_t* ex, double* partial) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P.n_paths;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        const double c = cash[p];
+        v[0] += c;
+        v[1] += c * c;
+        const int e = ex[p];
+        if (e < P.n_steps)
+            v[2] += 1.0;
+        else if (c > 0.0)
+            v[3] += 1.0;
+    }
+    block_sum_store<4>(v, partial + (uint64_t)blockIdx.x * 4);
+}
+
+__global__ __launch_bounds__(64) void lsm_reduce_kernel(mp_lsm_params P, const double* partial,
+                                                       int n_blocks, const int* valid_all,
+                                                       int allow_intrinsic, double* out) {
+    __shared__ double red[4][MP_WAVE];
+    __shared__ int regs[MP_WAVE];
+    const int lane = threadIdx.x;
+    mp_neumaier a[4];
+    for (int i = 0; i < 4; ++i) a[i].init();
+    for (int b = lane; b < n_blocks; b += MP_WAVE)
+        for (int i = 0; i < 4; ++i) a[i].add(partial[(uint64_t)b * 4 + i]);
+    for (int i = 0; i < 4; ++i) red[i][lane] = a[i].value();
+    int cnt = 0;
+    for (int s = lane; s <= P.n_steps; s += MP_WAVE) cnt += valid_all[s] ? 1 : 0;
+    regs[lane] = cnt;
+    __syncthreads();
+    if (lane == 0) {
+        double t[4];
+        for (int i = 0; i < 4; ++i) {
+            mp_neumaier s;
+            s.init();
+            for (int l = 0; l < MP_WAVE; ++l) s.add(red[i][l]);
+            t[i] = s.value();
+        }
+        int rs = 0;
+        for (int l = 0; l < MP_WAVE; ++l) rs += regs[l];
+        const double n = (double)P.n_paths;
+        const double mean = t[0] / n;
+        const double var = t[1] / n - mean * mean;
+        double price = mean, se = sqrt(var > 0.0 ? var : 0.0) / sqrt(n);
+        if (allow_intrinsic) {
+            const double intrinsic = P.k - P.s0 > 0.0 ? P.k - P.s0 : 0.0;
+            if (intrinsic > price) {
+                price = intrinsic;
+                se = 0.0;
+            }
+        }
+        out[0] = price;
+        out[1] = se;
+        out[2] = t[2];
+        out[3] = t[3];
+        out[4] = (double)rs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP64 comparator (shape of the reference's CUDA contract): one lane per
+// path, FP64 Philox normals and path, discounted payoff per path to HBM.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void plain_fp64_kernel(const mp_dreq* __restrict__ reqp,
+                                                        double* payoffs) {
+    const mp_dreq& q = *reqp;
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= q.n_paths) return;
+    const model_params<double> m = load_params<double>(q);
+    path_state<double> st;
+    state_init(st, q);
+    for (int s0i = 0; s0i < q.n_steps; s0i += 4) {
+        double z[4];
+        mp_normals4<double>((uint32_t)p, (uint32_t)(s0i >> 2), 0u, q.seed_hi, q.key0, q.key1, z);
+        const int nsub = min(4, q.n_steps - s0i);
+        for (int e = 0; e < nsub; ++e) gbm_step(st, m, z[e]);
+    }
+    payoffs[p] = q.discount * payoff_of(st, m, q, q.n_steps, q.s0, q.strikes[0]);
+}
+
+__global__ __launch_bounds__(256) void plain_reduce_kernel(const double* payoffs, uint64_t n,
+                                                          double* partial) {
+    double v[2] = {0.0, 0.0};
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        const double x = payoffs[p];
+        v[0] += x;
+        v[1] += x * x;
+    }
+    block_sum_store<2>(v, partial + (uint64_t)blockIdx.x * MP_PARTIAL_STRIDE);
+}
+
+// ---------------------------------------------------------------------------
+// Draw emission for parity tests.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void draws_kernel(const mp_dreq* __restrict__ reqp,
+                                                   mp_qmc_params qp,
+                                                   const uint32_t* __restrict__ nib, int kind,
+                                                   uint32_t stream, uint64_t path0,
+                                                   uint64_t n_paths, uint64_t n_draws,
+                                                   double* out) {
+    const mp_dreq& q = *reqp;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_paths) return;
+    const uint64_t p = path0 + i;
+    if (kind <= 1) {
+        for (uint64_t j = 0; j < n_draws; j += 4) {
+            double z[4];
+            if (kind == 0)
+                mp_normals4<double>((uint32_t)p, (uint32_t)(j >> 2), stream, q.seed_hi, q.key0, q.key1, z);
+            else
+                mp_uniforms4<double>((uint32_t)p, (uint32_t)(j >> 2), stream, q.seed_hi, q.key0, q.key1, z);
+            for (uint64_t e = 0; e < 4 && j + e < n_draws; ++e) out[i * n_draws + j + e] = z[e];
+        }
+    } else {
+        sobol_cursor<double> cs;
+        cs.bseed = qp.base_seed + (uint32_t)(p >> 16) * 0x9E3779B9u;
+        cs.local = (uint32_t)(p & 0xFFFFu);
+        cs.set_offset(0);
+        for (uint64_t j = 0; j < n_draws; ++j) {
+            if (kind == 2)
+                out[i * n_draws + j] = cs.normal((int)j, nib);
+            else
+                out[i * n_draws + j] = (double)cs.raw((int)j, nib);
+        }
+    }
+}
+
+// FP32 draws (normals of the FP32 generator) for parity of the hardware
+// transcendental path: kind 4.
+__global__ __launch_bounds__(256) void draws_f32_kernel(const mp_dreq* __restrict__ reqp,
+                                                       uint32_t stream, uint64_t path0,
+                                                       uint64_t n_paths, uint64_t n_draws,
+                                                       double* out) {
+    const mp_dreq& q = *reqp;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_paths) return;
+    const uint64_t p = path0 + i;
+    for (uint64_t j = 0; j < n_draws; j += 4) {
+        float z[4];
+        mp_normals4<float>((uint32_t)p, (uint32_t)(j >> 2), stream, q.seed_hi, q.key0, q.key1, z);
+        for (uint64_t e = 0; e < 4 && j + e < n_draws; ++e) out[i * n_draws + j + e] = (double)z[e];
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Launchers
+// ===========================================================================
+namespace mp {
+
+hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s) {
+    hipLaunchKernelGGL(reset_kernel, dim3((n + 63) / 64), dim3(64), 0, s, counter, n);
+    return hipGetLastError();
+}
+
+template <class R, int SRC>
+static hipError_t tiles_ppl(const tile_launch& L, hipStream_t s) {
+    switch (L.ppl) {
+        case 2:
+            hipLaunchKernelGGL((tile_kernel<R, SRC, 2>), dim3(L.grid), dim3(L.block), 0, s, L);
+            break;
+        case 4:
+            hipLaunchKernelGGL((tile_kernel<R, SRC, 4>), dim3(L.grid), dim3(L.block), 0, s, L);
+            break;
+        default:
+            hipLaunchKernelGGL((tile_kernel<R, SRC, 1>), dim3(L.grid), dim3(L.block), 0, s, L);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tiles(const tile_launch& L, int precision, hipStream_t s) {
+    const bool buf = L.normals != nullptr;
+    if (precision == 64) return buf ? tiles_ppl<double, 1>(L, s) : tiles_ppl<double, 0>(L, s);
+    return buf ? tiles_ppl<float, 1>(L, s) : tiles_ppl<float, 0>(L, s);
+}
+
+hipError_t launch_finalize(const mp_dreq* reqs, int n_reqs, const double* partial, double* out,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(n_reqs), dim3(64), 0, s, reqs, partial, out);
+    return hipGetLastError();
+}
+
+size_t qmc_lds_bytes(int precision, int factors, int n_slots, int block, bool nib_lds) {
+    const size_t rb = precision == 64 ? 8 : 4;
+    return (nib_lds ? (size_t)kNibWords * 4 : 0) + (size_t)n_slots * factors * block * rb;
+}
+
+template <class R, int F>
+static void qmc_launch_nl(bool nib_lds, dim3 g, dim3 b, size_t lds, hipStream_t s,
+                          const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
+                          const mp_bbop* plan, double* partial, double* term, double* pay) {
+    if (nib_lds)
+        hipLaunchKernelGGL((qmc_kernel<R, F, true>), g, b, lds, s, req, qp, nib, plan, partial,
+                           term, pay);
+    else
+        hipLaunchKernelGGL((qmc_kernel<R, F, false>), g, b, lds, s, req, qp, nib, plan, partial,
+                           term, pay);
+}
+
+hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
+                      const mp_bbop* plan, double* partial, double* terminal_out,
+                      double* payoff_out, int precision, int grid, int block, bool nib_lds,
+                      hipStream_t s) {
+    const size_t lds =
+        qmc_lds_bytes(precision, qp.factors, qp.n_slots > 0 ? qp.n_slots : 1, block, nib_lds);
+    const dim3 g(grid), b(block);
+    if (precision == 64) {
+        if (qp.factors == 2)
+            qmc_launch_nl<double, 2>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+        else
+            qmc_launch_nl<double, 1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+    } else {
+        if (qp.factors == 2)
+            qmc_launch_nl<float, 2>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+        else
+            qmc_launch_nl<float, 1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+    }
+    return hipGetLastError();
+}
+
+template <class R, int NA>
+static hipError_t basket_na(const mp_dreq* req, const double* chol, const double* drift,
+                            const double* vol, const double* logs0, const double* weight,
+                            const double* normals, double* partial, double* term, double* pay,
+                            int grid, int block, hipStream_t s) {
+    if (normals)
+        hipLaunchKernelGGL((basket_kernel<R, NA, 1>), dim3(grid), dim3(block), 0, s, req, chol,
+                           drift, vol, logs0, weight, normals, partial, term, pay);
+    else
+        hipLaunchKernelGGL((basket_kernel<R, NA, 0>), dim3(grid), dim3(block), 0, s, req, chol,
+                           drift, vol, logs0, weight, normals, partial, term, pay);
+    return hipGetLastError();
+}
+
+template <class R>
+static hipError_t basket_r(int na, const mp_dreq* req, const double* chol, const double* drift,
+                           const double* vol, const double* logs0, const double* weight,
+                           const double* normals, double* partial, double* term, double* pay,
+                           int grid, int block, hipStream_t s) {
+#define MP_BASKET_CASE(N)                                                                         \
+    case N:                                                                                       \
+        return basket_na<R, N>(req, chol, drift, vol, logs0, weight, normals, partial, term, pay, \
+                               grid, block, s);
+    switch (na) {
+        MP_BASKET_CASE(1)
+        MP_BASKET_CASE(2)
+        MP_BASKET_CASE(3)
+        MP_BASKET_CASE(4)
+        MP_BASKET_CASE(5)
+        MP_BASKET_CASE(6)
+        MP_BASKET_CASE(7)
+        MP_BASKET_CASE(8)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef MP_BASKET_CASE
+}
+
+hipError_t launch_basket(const mp_dreq* req, int n_assets, const double* chol,
+                         const double* drift, const double* vol, const double* log_s0,
+                         const double* weight, const double* normals, double* partial,
+                         double* terminal_out, double* payoff_out, int precision, int grid,
+                         int block, hipStream_t s) {
+    if (precision == 64)
+        return basket_r<double>(n_assets, req, chol, drift, vol, log_s0, weight, normals, partial,
+                                terminal_out, payoff_out, grid, block, s);
+    return basket_r<float>(n_assets, req, chol, drift, vol, log_s0, weight, normals, partial,
+                           terminal_out, payoff_out, grid, block, s);
+}
+
+hipError_t launch_lsm_paths(const mp_lsm_params& p, const double* normals, void* paths,
+                            double* cash, uint16_t* ex, int precision, int grid, int block,
+                            hipStream_t s) {
+    if (precision == 64) {
+        if (normals)
+            hipLaunchKernelGGL((lsm_paths_kernel<double, 1>), dim3(grid), dim3(block), 0, s, p,
+                               normals, (double*)paths, cash, ex);
+        else
+            hipLaunchKernelGGL((lsm_paths_kernel<double, 0>), dim3(grid), dim3(block), 0, s, p,
+                               normals, (double*)paths, cash, ex);
+    } else {
+        if (normals)
+            hipLaunchKernelGGL((lsm_paths_kernel<float, 1>), dim3(grid), dim3(block), 0, s, p,
+                               normals, (float*)paths, cash, ex);
+        else
+            hipLaunchKernelGGL((lsm_paths_kernel<float, 0>), dim3(grid), dim3(block), 0, s, p,
+                               normals, (float*)paths, cash, ex);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_lsm_step(const mp_lsm_params& p, const void* paths, double* cash,
+                           uint16_t* ex, const double* coeff_all, const int* valid_all,
+                           int step_update, int step_gram, const double* disc_pow,
+                           const double* growth, double* gram_partial, int precision, int grid,
+                           int block, hipStream_t s) {
+    if (precision == 64)
+        hipLaunchKernelGGL((lsm_step_kernel<double>), dim3(grid), dim3(block), 0, s, p,
+                           (const double*)paths, cash, ex, coeff_all, valid_all, step_update,
+                           step_gram, disc_pow, growth, gram_partial);
+    else
+        hipLaunchKernelGGL((lsm_step_kernel<float>), dim3(grid), dim3(block), 0, s, p,
+                           (const float*)paths, cash, ex, coeff_all, valid_all, step_update,
+                           step_gram, disc_pow, growth, gram_partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_lsm_solve(const mp_lsm_params& p, const double* gram_partial, int n_blocks,
+                            int step, double* coeff_all, int* valid_all, hipStream_t s) {
+    hipLaunchKernelGGL(lsm_solve_kernel, dim3(1), dim3(64), 0, s, p, gram_partial, n_blocks, step,
+                       coeff_all, valid_all);
+    return hipGetLastError();
+}
+
+hipError_t launch_lsm_final(const mp_lsm_params& p, const double* cash, const uint16_t* ex,
+                            double* partial, int grid, int block, hipStream_t s) {
+    hipLaunchKernelGGL(lsm_final_kernel, dim3(grid), dim3(block), 0, s, p, cash, ex, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_lsm_reduce(const mp_lsm_params& p, const double* partial, int n_blocks,
+                             const int* valid_all, int allow_intrinsic, double* out,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(lsm_reduce_kernel, dim3(1), dim3(64), 0, s, p, partial, n_blocks,
+                       valid_all, allow_intrinsic, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_fp64(const mp_dreq* req, double* payoffs, int grid, int block,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(plain_fp64_kernel, dim3(grid), dim3(block), 0, s, req, payoffs);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_reduce(const double* payoffs, uint64_t n, double* partial, int n_blocks,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(plain_reduce_kernel, dim3(n_blocks), dim3(256), 0, s, payoffs, n, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_draws(const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
+                        int kind, uint32_t stream, uint64_t path0, uint64_t n_paths,
+                        uint64_t n_draws, double* out, hipStream_t s) {
+    const int grid = (int)((n_paths + 255) / 256);
+    if (kind == 4)
+        hipLaunchKernelGGL(draws_f32_kernel, dim3(grid), dim3(256), 0, s, req, stream, path0,
+                           n_paths, n_draws, out);
+    else
+        hipLaunchKernelGGL(draws_kernel, dim3(grid), dim3(256), 0, s, req, qp, nib, kind, stream,
+                           path0, n_paths, n_draws, out);
+    return hipGetLastError();
+}
+
+}  // namespace mp

@@ -1,0 +1,570 @@
+"""ctypes bindings to the C oracle ``oracle/_build/libmp_oracle.so``.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).  Each wrapper names
+the reference function it restates; the restatements themselves, with
+file:line citations into ``crates/mc-core/src/runtime/cpu.rs``, live in
+``oracle/mp_oracle.c``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libmp_oracle.so")
+
+EUROPEAN, ASIAN, DOWN_AND_OUT, LOOKBACK = 0, 1, 2, 3
+PAYOFF_CODES = {
+    "european_call": EUROPEAN,
+    "arithmetic_asian_call": ASIAN,
+    "down_and_out_call": DOWN_AND_OUT,
+    "lookback_call": LOOKBACK,
+}
+HESTON_REFERENCE_FLOOR, HESTON_FULL_TRUNCATION = 0, 1
+
+
+class Result(C.Structure):
+    _fields_ = [("price", C.c_double), ("stderr", C.c_double)]
+
+
+class GbmCfg(C.Structure):
+    _fields_ = [
+        ("s0", C.c_double), ("k", C.c_double), ("r", C.c_double), ("sigma", C.c_double),
+        ("t", C.c_double), ("barrier", C.c_double),
+        ("n_paths", C.c_uint64), ("n_steps", C.c_uint64), ("seed", C.c_uint64),
+        ("n_threads", C.c_uint64),
+    ]
+
+
+class HestonCfg(C.Structure):
+    _fields_ = [
+        ("s0", C.c_double), ("k", C.c_double), ("r", C.c_double), ("v0", C.c_double),
+        ("kappa", C.c_double), ("theta", C.c_double), ("xi", C.c_double), ("rho", C.c_double),
+        ("t", C.c_double),
+        ("n_paths", C.c_uint64), ("n_steps", C.c_uint64), ("seed", C.c_uint64),
+        ("n_threads", C.c_uint64),
+    ]
+
+
+class MertonCfg(C.Structure):
+    _fields_ = [
+        ("s0", C.c_double), ("k", C.c_double), ("r", C.c_double), ("sigma", C.c_double),
+        ("lam", C.c_double), ("mu_j", C.c_double), ("sigma_j", C.c_double), ("t", C.c_double),
+        ("n_paths", C.c_uint64), ("seed", C.c_uint64), ("n_threads", C.c_uint64),
+    ]
+
+
+class Basket2Cfg(C.Structure):
+    _fields_ = [
+        ("s01", C.c_double), ("s02", C.c_double), ("k", C.c_double), ("r", C.c_double),
+        ("sigma1", C.c_double), ("sigma2", C.c_double), ("rho", C.c_double),
+        ("weight1", C.c_double), ("weight2", C.c_double), ("t", C.c_double),
+        ("n_paths", C.c_uint64), ("seed", C.c_uint64),
+    ]
+
+
+class LsmCfg(C.Structure):
+    _fields_ = [
+        ("s0", C.c_double), ("k", C.c_double), ("r", C.c_double), ("sigma", C.c_double),
+        ("t", C.c_double),
+        ("n_paths", C.c_uint64), ("n_steps", C.c_uint64), ("seed", C.c_uint64),
+        ("basis_degree", C.c_uint64), ("allow_intrinsic_now", C.c_int),
+    ]
+
+
+class LsmResult(C.Structure):
+    _fields_ = [
+        ("price", C.c_double), ("stderr", C.c_double),
+        ("early_exercise_count", C.c_uint64), ("maturity_exercise_count", C.c_uint64),
+        ("regression_steps", C.c_uint64),
+    ]
+
+
+class BbOp(C.Structure):
+    _fields_ = [
+        ("left", C.c_uint64), ("mid", C.c_uint64), ("right", C.c_uint64),
+        ("normal_index", C.c_uint64),
+        ("lw", C.c_double), ("rw", C.c_double), ("sd", C.c_double),
+    ]
+
+
+_lib = None
+
+_D = C.POINTER(C.c_double)
+_U64 = C.POINTER(C.c_uint64)
+_U32 = C.POINTER(C.c_uint32)
+
+
+def lib() -> C.CDLL:
+    """Load the oracle library (built by ``oracle/Makefile`` via build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"oracle library missing at {LIB_PATH}; run __graft_entry__.build() first"
+        )
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "mpo_splitmix64": (C.c_uint64, [C.c_uint64]),
+        "mpo_derive_chunk_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+        "mpo_rng_normals": (None, [C.c_uint64, C.c_uint64, _D]),
+        "mpo_rng_u64": (None, [C.c_uint64, C.c_uint64, _U64]),
+        "mpo_summarize": (Result, [C.c_uint64, C.c_double, C.c_double]),
+        "mpo_gbm_stepwise_price": (Result, [C.POINTER(GbmCfg), C.c_int]),
+        "mpo_european_terminal_price": (Result, [C.POINTER(GbmCfg)]),
+        "mpo_gbm_paths_from_normals": (None, [C.POINTER(GbmCfg), C.c_int, _D, _D, _D]),
+        "mpo_heston_price": (Result, [C.POINTER(HestonCfg), C.c_int]),
+        "mpo_heston_paths_from_normals": (None, [C.POINTER(HestonCfg), C.c_int, _D, _D, _D]),
+        "mpo_merton_price": (Result, [C.POINTER(MertonCfg)]),
+        "mpo_poisson_from_uniform": (C.c_uint64, [C.c_double, C.c_double]),
+        "mpo_merton_paths_from_draws": (
+            None, [C.POINTER(MertonCfg), C.c_uint64, _D, _D, _D, _D, _U64]),
+        "mpo_basket2_price": (Result, [C.POINTER(Basket2Cfg)]),
+        "mpo_basket_paths_from_normals": (
+            None, [C.c_uint64, _D, _D, _D, _D, C.c_double, C.c_double, C.c_uint64,
+                   C.c_uint64, _D, _D]),
+        "mpo_cholesky": (C.c_int, [C.c_uint64, _D, _D]),
+        "mpo_lsm_solve": (C.c_int, [_D, _D, C.c_uint64, _D]),
+        "mpo_lsm_from_paths": (
+            None, [C.POINTER(LsmCfg), _D, _U64, C.c_uint64, C.POINTER(LsmResult), _D, _U64]),
+        "mpo_lsm_paths": (None, [C.POINTER(LsmCfg), _D]),
+        "mpo_lsm_paths_from_normals": (None, [C.POINTER(LsmCfg), _D, _D]),
+        "mpo_lsm_price": (None, [C.POINTER(LsmCfg), _U64, C.c_uint64, C.POINTER(LsmResult)]),
+        "mpo_norm_cdf_as": (C.c_double, [C.c_double]),
+        "mpo_norm_cdf_exact": (C.c_double, [C.c_double]),
+        "mpo_bs_call": (C.c_double, [C.c_double] * 5),
+        "mpo_bs_call_ref": (C.c_double, [C.c_double] * 5),
+        "mpo_bs_put": (C.c_double, [C.c_double] * 5),
+        "mpo_merton_series": (
+            C.c_double, [C.POINTER(MertonCfg), C.c_uint64, C.c_double, C.c_int]),
+        "mpo_crr_put": (
+            C.c_double, [C.c_double] * 5 + [C.c_uint64, C.POINTER(C.c_uint8)]),
+        "mpo_inverse_normal": (C.c_double, [C.c_double]),
+        "mpo_bb_plan": (C.c_uint64, [C.c_uint64, C.c_double, C.POINTER(BbOp)]),
+        "mpo_bb_increments": (None, [C.c_uint64, C.c_double, _D, _D]),
+        "mpo_sobol_init": (C.c_int, [_U32, _U32, C.c_uint64, C.c_uint64]),
+        "mpo_sobol_directions": (None, [_U32]),
+        "mpo_sobol_u32": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+        "mpo_sobol_scrambled_u32": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+        "mpo_sobol_sample": (C.c_double, [C.c_uint32, C.c_uint32, C.c_uint32]),
+        "mpo_sobol_point_normals": (None, [C.c_uint32, C.c_uint64, C.c_uint64, _D]),
+        "mpo_gbm_qmc_price": (Result, [C.POINTER(GbmCfg), C.c_int, C.c_int, _D]),
+        "mpo_heston_qmc_paths": (None, [C.POINTER(HestonCfg), C.c_int, C.c_uint64, _D]),
+        "mpo_philox4x32_10": (None, [_U32, _U32, _U32]),
+        "mpo_philox_normals": (
+            None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, _D]),
+        "mpo_philox_uniforms": (
+            None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, _D]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    _init_sobol(L)
+    return L
+
+
+def _init_sobol(L) -> None:
+    """Feed the Joe-Kuo (new-joe-kuo-6.21201) polynomials and initial
+    direction values shipped with scipy to the oracle's own recurrence."""
+    import scipy.stats as st
+
+    path = os.path.join(os.path.dirname(st.__file__), "_sobol_direction_numbers.npz")
+    with np.load(path) as d:
+        poly = np.ascontiguousarray(d["poly"][:256], dtype=np.uint32)
+        vinit = np.ascontiguousarray(d["vinit"][:256], dtype=np.uint32)
+    rc = L.mpo_sobol_init(_p32(poly), _p32(vinit), vinit.shape[1], 256)
+    if rc != 0:
+        raise RuntimeError("oracle Sobol initialisation failed")
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def _p64(a: np.ndarray):
+    return a.ctypes.data_as(_U64)
+
+
+def _p32(a: np.ndarray):
+    return a.ctypes.data_as(_U32)
+
+
+# ------------------------------------------------------------------ configs
+@dataclass
+class Gbm:
+    s0: float = 100.0
+    k: float = 100.0
+    r: float = 0.03
+    sigma: float = 0.2
+    t: float = 1.0
+    n_paths: int = 100_000
+    n_steps: int = 64
+    seed: int = 42
+    n_threads: int = 1
+    barrier: float = 80.0
+
+    def c(self) -> GbmCfg:
+        return GbmCfg(self.s0, self.k, self.r, self.sigma, self.t, self.barrier,
+                      self.n_paths, self.n_steps, self.seed, self.n_threads)
+
+
+@dataclass
+class Heston:
+    s0: float = 100.0
+    k: float = 100.0
+    r: float = 0.03
+    v0: float = 0.04
+    kappa: float = 2.0
+    theta: float = 0.04
+    xi: float = 0.5
+    rho: float = -0.7
+    t: float = 1.0
+    n_paths: int = 10_000
+    n_steps: int = 64
+    seed: int = 42
+    n_threads: int = 1
+
+    def c(self) -> HestonCfg:
+        return HestonCfg(self.s0, self.k, self.r, self.v0, self.kappa, self.theta, self.xi,
+                         self.rho, self.t, self.n_paths, self.n_steps, self.seed, self.n_threads)
+
+
+@dataclass
+class Merton:
+    s0: float = 100.0
+    k: float = 100.0
+    r: float = 0.03
+    sigma: float = 0.2
+    lam: float = 0.4
+    mu_j: float = -0.08
+    sigma_j: float = 0.25
+    t: float = 1.0
+    n_paths: int = 100_000
+    seed: int = 42
+    n_threads: int = 1
+
+    def c(self) -> MertonCfg:
+        return MertonCfg(self.s0, self.k, self.r, self.sigma, self.lam, self.mu_j, self.sigma_j,
+                         self.t, self.n_paths, self.seed, self.n_threads)
+
+
+@dataclass
+class Lsm:
+    s0: float = 100.0
+    k: float = 100.0
+    r: float = 0.03
+    sigma: float = 0.2
+    t: float = 1.0
+    n_paths: int = 20_000
+    n_steps: int = 64
+    seed: int = 42
+    basis_degree: int = 2
+    allow_intrinsic_now: bool = True
+
+    def c(self) -> LsmCfg:
+        return LsmCfg(self.s0, self.k, self.r, self.sigma, self.t, self.n_paths, self.n_steps,
+                      self.seed, self.basis_degree, int(self.allow_intrinsic_now))
+
+
+# ------------------------------------------------------------------ wrappers
+def rng_normals(seed: int, n: int) -> np.ndarray:
+    """MonteCarloRng::standard_normal stream (cpu.rs:2992-3006)."""
+    out = np.empty(n, dtype=np.float64)
+    lib().mpo_rng_normals(seed, n, _pd(out))
+    return out
+
+
+def rng_u64(seed: int, n: int) -> np.ndarray:
+    """MonteCarloRng::next_u64 stream (cpu.rs:2975-2983)."""
+    out = np.empty(n, dtype=np.uint64)
+    lib().mpo_rng_u64(seed, n, _p64(out))
+    return out
+
+
+def splitmix64(x: int) -> int:
+    return int(lib().mpo_splitmix64(x))
+
+
+def derive_chunk_seed(base: int, idx: int) -> int:
+    return int(lib().mpo_derive_chunk_seed(base, idx))
+
+
+def gbm_price(cfg: Gbm, payoff: str = "european_call") -> tuple[float, float]:
+    """Stepwise GBM family pricer (cpu.rs:3228-3262 and siblings)."""
+    c = cfg.c()
+    r = lib().mpo_gbm_stepwise_price(C.byref(c), PAYOFF_CODES[payoff])
+    return r.price, r.stderr
+
+
+def european_terminal_price(cfg: Gbm) -> tuple[float, float]:
+    """european_call_price_mc_cpu_terminal (cpu.rs:3191-3226)."""
+    c = cfg.c()
+    r = lib().mpo_european_terminal_price(C.byref(c))
+    return r.price, r.stderr
+
+
+def gbm_paths_from_normals(cfg: Gbm, normals: np.ndarray, payoff: str = "european_call"):
+    """Per-path (S_T, discounted payoff) from a [path][step] normal buffer."""
+    normals = np.ascontiguousarray(normals, dtype=np.float64)
+    assert normals.size == cfg.n_paths * cfg.n_steps
+    term = np.empty(cfg.n_paths)
+    pay = np.empty(cfg.n_paths)
+    c = cfg.c()
+    lib().mpo_gbm_paths_from_normals(C.byref(c), PAYOFF_CODES[payoff], _pd(normals), _pd(term),
+                                     _pd(pay))
+    return term, pay
+
+
+def heston_price(cfg: Heston, scheme: int = HESTON_REFERENCE_FLOOR) -> tuple[float, float]:
+    """heston_european_call_price_mc_standard (cpu.rs:8869-8881)."""
+    c = cfg.c()
+    r = lib().mpo_heston_price(C.byref(c), scheme)
+    return r.price, r.stderr
+
+
+def heston_paths_from_normals(cfg: Heston, normals: np.ndarray,
+                              scheme: int = HESTON_REFERENCE_FLOOR):
+    """Per-path S_T from [path][step][2] normals (z_var, z_ind)."""
+    normals = np.ascontiguousarray(normals, dtype=np.float64)
+    assert normals.size == cfg.n_paths * cfg.n_steps * 2
+    term = np.empty(cfg.n_paths)
+    var = np.empty(cfg.n_paths)
+    c = cfg.c()
+    lib().mpo_heston_paths_from_normals(C.byref(c), scheme, _pd(normals), _pd(term), _pd(var))
+    return term, var
+
+
+def merton_price(cfg: Merton) -> tuple[float, float]:
+    """merton_jump_diffusion_call_price_mc_cpu (cpu.rs:3151-3189)."""
+    c = cfg.c()
+    r = lib().mpo_merton_price(C.byref(c))
+    return r.price, r.stderr
+
+
+def merton_paths_from_draws(cfg: Merton, n_steps: int, z, u, zj):
+    z, u, zj = (np.ascontiguousarray(a, dtype=np.float64) for a in (z, u, zj))
+    term = np.empty(cfg.n_paths)
+    jumps = np.empty(cfg.n_paths, dtype=np.uint64)
+    c = cfg.c()
+    lib().mpo_merton_paths_from_draws(C.byref(c), n_steps, _pd(z), _pd(u), _pd(zj), _pd(term),
+                                      _p64(jumps))
+    return term, jumps
+
+
+def poisson_from_uniform(u: float, mean: float) -> int:
+    return int(lib().mpo_poisson_from_uniform(u, mean))
+
+
+def merton_series(cfg: Merton, max_terms: int = 96, tail_tol: float = 1e-12,
+                  exact: bool = True) -> float:
+    """merton_jump_diffusion_call_reference_price (cpu.rs:3110-3149)."""
+    c = cfg.c()
+    return float(lib().mpo_merton_series(C.byref(c), max_terms, tail_tol, int(exact)))
+
+
+def basket2_price(s01=100.0, s02=100.0, k=100.0, r=0.03, sigma1=0.2, sigma2=0.25, rho=0.5,
+                  weight1=0.5, weight2=0.5, t=1.0, n_paths=100_000, seed=42):
+    """simulate_basket_pseudorandom_standard (cpu.rs:7503-7518)."""
+    c = Basket2Cfg(s01, s02, k, r, sigma1, sigma2, rho, weight1, weight2, t, n_paths, seed)
+    res = lib().mpo_basket2_price(C.byref(c))
+    return res.price, res.stderr
+
+
+def cholesky(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    out = np.empty_like(a)
+    if lib().mpo_cholesky(n, _pd(a), _pd(out)) != 0:
+        raise ValueError("matrix is not positive definite")
+    return out
+
+
+def basket_paths_from_normals(s0, sigma, weight, corr, r, t, n_steps, n_paths, normals):
+    s0, sigma, weight = (np.ascontiguousarray(x, dtype=np.float64) for x in (s0, sigma, weight))
+    chol = cholesky(np.asarray(corr, dtype=np.float64))
+    normals = np.ascontiguousarray(normals, dtype=np.float64)
+    n_assets = len(s0)
+    assert normals.size == n_paths * n_steps * n_assets
+    out = np.empty(n_paths)
+    lib().mpo_basket_paths_from_normals(n_assets, _pd(s0), _pd(sigma), _pd(weight), _pd(chol),
+                                        r, t, n_steps, n_paths, _pd(normals), _pd(out))
+    return out
+
+
+def lsm_solve(mat, rhs, dim):
+    mat = np.ascontiguousarray(mat, dtype=np.float64).reshape(16)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64).reshape(4)
+    out = np.zeros(4)
+    rc = lib().mpo_lsm_solve(_pd(mat), _pd(rhs), dim, _pd(out))
+    return None if rc != 0 else out
+
+
+def lsm_paths(cfg: Lsm) -> np.ndarray:
+    out = np.empty((cfg.n_paths, cfg.n_steps + 1))
+    c = cfg.c()
+    lib().mpo_lsm_paths(C.byref(c), _pd(out))
+    return out
+
+
+def lsm_paths_from_normals(cfg: Lsm, normals) -> np.ndarray:
+    normals = np.ascontiguousarray(normals, dtype=np.float64)
+    out = np.empty((cfg.n_paths, cfg.n_steps + 1))
+    c = cfg.c()
+    lib().mpo_lsm_paths_from_normals(C.byref(c), _pd(normals), _pd(out))
+    return out
+
+
+def lsm_from_paths(cfg: Lsm, paths: np.ndarray, schedule=None, want_detail=False):
+    """Backward induction of price_put_lsm_cpu (cpu.rs:3835-3924)."""
+    if schedule is None:
+        schedule = np.arange(1, cfg.n_steps + 1, dtype=np.uint64)
+    schedule = np.ascontiguousarray(schedule, dtype=np.uint64)
+    paths = np.ascontiguousarray(paths, dtype=np.float64)
+    res = LsmResult()
+    coeff = np.zeros((cfg.n_steps, 4))
+    ex = np.zeros(cfg.n_paths, dtype=np.uint64)
+    c = cfg.c()
+    lib().mpo_lsm_from_paths(C.byref(c), _pd(paths), _p64(schedule), len(schedule), C.byref(res),
+                             _pd(coeff), _p64(ex))
+    out = {
+        "price": res.price, "stderr": res.stderr,
+        "early_exercise_count": res.early_exercise_count,
+        "maturity_exercise_count": res.maturity_exercise_count,
+        "regression_steps": res.regression_steps,
+    }
+    if want_detail:
+        out["coefficients"] = coeff
+        out["exercise_steps"] = ex
+    return out
+
+
+def lsm_price(cfg: Lsm, schedule=None):
+    """american_put_price_lsm_cpu / bermudan_put_price_lsm_cpu (cpu.rs:3947-4020)."""
+    if schedule is None:
+        schedule = np.arange(1, cfg.n_steps + 1, dtype=np.uint64)
+    schedule = np.ascontiguousarray(schedule, dtype=np.uint64)
+    res = LsmResult()
+    c = cfg.c()
+    lib().mpo_lsm_price(C.byref(c), _p64(schedule), len(schedule), C.byref(res))
+    return {
+        "price": res.price, "stderr": res.stderr,
+        "early_exercise_count": res.early_exercise_count,
+        "maturity_exercise_count": res.maturity_exercise_count,
+        "regression_steps": res.regression_steps,
+    }
+
+
+def bs_call(s0, k, r, sigma, t, exact=True) -> float:
+    """black_scholes_european_call_price (cpu.rs:839-859); exact=False uses
+    the reference's A&S CDF."""
+    f = lib().mpo_bs_call if exact else lib().mpo_bs_call_ref
+    return float(f(s0, k, r, sigma, t))
+
+
+def bs_put(s0, k, r, sigma, t) -> float:
+    return float(lib().mpo_bs_put(s0, k, r, sigma, t))
+
+
+def norm_cdf_as(x: float) -> float:
+    return float(lib().mpo_norm_cdf_as(x))
+
+
+def crr_put(s0, k, r, sigma, t, steps, exercise_steps) -> float:
+    """binomial_put_reference_price (cpu.rs:5244-5305)."""
+    mask = np.zeros(steps + 1, dtype=np.uint8)
+    for s in exercise_steps:
+        if s <= steps:
+            mask[s] = 1
+    return float(lib().mpo_crr_put(s0, k, r, sigma, t, steps,
+                                   mask.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+
+def inverse_normal(p: float) -> float:
+    """inverse_standard_normal (cpu.rs:6012-6061)."""
+    return float(lib().mpo_inverse_normal(p))
+
+
+def bb_plan(n_steps: int, maturity: float):
+    """BrownianBridgePlan::new (cpu.rs:5923-5961) as a list of dicts."""
+    ops = (BbOp * max(n_steps, 1))()
+    n = lib().mpo_bb_plan(n_steps, maturity, ops)
+    return [
+        {"left": o.left, "mid": o.mid, "right": o.right, "normal_index": o.normal_index,
+         "lw": o.lw, "rw": o.rw, "sd": o.sd}
+        for o in ops[:n]
+    ]
+
+
+def bb_increments(n_steps: int, maturity: float, inputs) -> np.ndarray:
+    inputs = np.ascontiguousarray(inputs, dtype=np.float64)
+    out = np.empty(n_steps)
+    lib().mpo_bb_increments(n_steps, maturity, _pd(inputs), _pd(out))
+    return out
+
+
+def sobol_directions() -> np.ndarray:
+    out = np.empty((256, 32), dtype=np.uint32)
+    lib().mpo_sobol_directions(_p32(out))
+    return out
+
+
+def sobol_u32(index: int, dim: int) -> int:
+    return int(lib().mpo_sobol_u32(index, dim))
+
+
+def sobol_scrambled_u32(index: int, dim: int, seed: int) -> int:
+    return int(lib().mpo_sobol_scrambled_u32(index, dim, seed))
+
+
+def sobol_sample(index: int, dim: int, seed: int) -> float:
+    return float(lib().mpo_sobol_sample(index, dim, seed))
+
+
+def sobol_point_normals(seed: int, point: int, dims: int) -> np.ndarray:
+    """fill_scrambled_sobol_point_standard_normals (cpu.rs:5393-5432)."""
+    out = np.empty(dims)
+    lib().mpo_sobol_point_normals(seed & 0xFFFFFFFF, point, dims, _pd(out))
+    return out
+
+
+def gbm_qmc_price(cfg: Gbm, payoff: str = "european_call", bridge: bool = True):
+    """simulate_stepwise_qmc (cpu.rs:7113-7197) and siblings."""
+    term = np.empty(cfg.n_paths)
+    c = cfg.c()
+    r = lib().mpo_gbm_qmc_price(C.byref(c), PAYOFF_CODES[payoff], int(bridge), _pd(term))
+    return r.price, r.stderr, term
+
+
+def heston_qmc_paths(cfg: Heston, scheme: int, path0: int = 0) -> np.ndarray:
+    out = np.empty(cfg.n_paths)
+    c = cfg.c()
+    lib().mpo_heston_qmc_paths(C.byref(c), scheme, path0, _pd(out))
+    return out
+
+
+def philox4x32_10(ctr, key) -> list[int]:
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().mpo_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def philox_normals(seed: int, contract: int, stream: int, path0: int, n_paths: int,
+                   n_normals: int) -> np.ndarray:
+    out = np.empty((n_paths, n_normals))
+    lib().mpo_philox_normals(seed, contract, stream, path0, n_paths, n_normals, _pd(out))
+    return out
+
+
+def philox_uniforms(seed: int, contract: int, stream: int, path0: int, n_paths: int,
+                    n: int) -> np.ndarray:
+    out = np.empty((n_paths, n))
+    lib().mpo_philox_uniforms(seed, contract, stream, path0, n_paths, n, _pd(out))
+    return out
