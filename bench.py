@@ -1,0 +1,254 @@
+"""Benchmark of the rocm_hip Monte Carlo path engine on one MI355X.
+
+A step is one full pricing pass of the Heston precision workload: 2^22 paths
+x 256 full-truncation Euler steps, scrambled Sobol (512 dims, 2 per step)
+with Brownian-bridge ordering, three strikes (80, 100, 120) from one
+simulation, FP32 path arithmetic, FP64 compensated payoff sums.  The
+parameters (S0, v0, kappa, theta, xi, rho, r, T) are the published
+precision configuration's.  Prints one JSON line.
+
+Secondary measurements (same line, under "extra"): the 4,096-contract
+batched persistent launch against plain one-launch-per-request FP64 pricing
+of the same contracts, and a short Poisson serving trace with p50/p99.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_PATHS = 1 << 22
+N_STEPS = 256
+STRIKES = [80.0, 100.0, 120.0]
+HESTON = dict(s0=100.0, v0=0.04, kappa=2.0, theta=0.04, xi=0.5, rho=-0.7, r=0.03, t=1.0)
+# Algorithmic FP work per path-step of this workload: Heston full-truncation
+# step 20, plus per factor an Acklam inverse normal 12 and a bridge point 7.
+FLOP_PER_PATH_STEP = 20 + 2 * (12 + 7)
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def heston_request(precision: int, n_paths: int = N_PATHS, n_steps: int = N_STEPS):
+    from montepath_amd.engine import make_request
+
+    return make_request("heston_european_call", n_paths=n_paths, n_steps=n_steps, seed=42,
+                        precision=precision, heston_scheme="full_truncation",
+                        sampling="scrambled_sobol_brownian_bridge", strikes=STRIKES, **HESTON)
+
+
+def cpu_baseline(threads: int) -> dict:
+    """The reference's CPU Heston path (xorshift64*/Box-Muller, floored Euler,
+    std::thread chunks; runtime/cpu.rs:8632-8910) restated in C, timed on a
+    bounded sample of the workload's path-steps on this host's cores."""
+    from oracle import cpu as O
+
+    n, m = 1 << 21, N_STEPS
+    h = O.Heston(n_paths=n, n_steps=m, kappa=2.0, xi=0.5, rho=-0.7, n_threads=threads)
+    O.heston_price(O.Heston(n_paths=1 << 14, n_steps=m, n_threads=threads), 0)  # warm
+    t0 = time.perf_counter()
+    O.heston_price(h, O.HESTON_REFERENCE_FLOOR)
+    dt = time.perf_counter() - t0
+    return {"value": n * m / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"Heston floored Euler, pseudorandom, {n} paths x {m} steps "
+                      f"({dt:.2f} s wall on {threads} threads)"}
+
+
+def batch_contracts(n: int = 4096, seed: int = 7):
+    """4,096 contracts: half European, half arithmetic-average Asian calls under
+    GBM; K~U[80,120], sigma~U[0.1,0.6], T~U[0.1,3], r~U[0,0.05], paths
+    log-uniform in [2^12, 2^20], steps uniform in {16..365}; parameter draws
+    from one seeded host generator; Philox keyed by (seed 7, contract id)."""
+    from montepath_amd.engine import make_request
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        reqs.append(make_request(
+            "european_call" if i < n // 2 else "arithmetic_asian_call",
+            n_paths=int(2 ** rng.uniform(12, 20)), n_steps=int(rng.integers(16, 366)),
+            k=float(rng.uniform(80, 120)), sigma=float(rng.uniform(0.1, 0.6)),
+            t=float(rng.uniform(0.1, 3.0)), r=float(rng.uniform(0.0, 0.05)),
+            seed=7, contract_id=i, precision=32))
+    return reqs
+
+
+def bench_batch(eng, reps: int = 5) -> dict:
+    from montepath_amd.engine import make_request
+
+    reqs = batch_contracts()
+    pb = eng.prepare_batch(reqs, use_graph=True)
+    info = pb.info()
+    pb.launch()
+    pb.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pb.launch()
+    pb.sync()
+    dt = (time.perf_counter() - t0) / reps
+    batched = info["path_steps"] / dt
+    res = pb.results()
+    pb.free()
+    # plain FP64, one launch per request, on a sample of the same contracts
+    sample = list(range(0, len(reqs), 16))
+    ps, t_plain = 0, 0.0
+    for i in sample:
+        r = reqs[i]
+        r64 = make_request("european_call" if r.workload == 1 else "arithmetic_asian_call",
+                           n_paths=r.n_paths, n_steps=r.n_steps, k=r.k, sigma=r.sigma, t=r.t,
+                           r=r.r, seed=7, contract_id=i, precision=64)
+        t0 = time.perf_counter()
+        eng.price_plain_fp64(r64)
+        t_plain += time.perf_counter() - t0
+        ps += r.n_paths * r.n_steps
+    plain = ps / t_plain
+    return {"contracts": len(reqs), "path_steps": info["path_steps"], "tiles": info["n_tiles"],
+            "ms_per_launch": dt * 1e3, "path_steps_per_s": batched,
+            "plain_fp64_path_steps_per_s": plain, "speedup_vs_plain_fp64": batched / plain,
+            "mean_price": float(np.mean([x.price for x in res]))}
+
+
+def serve_mix(n: int, rate: float, seed: int = 11):
+    """Poisson arrivals of mixed requests (40% GBM European/Asian, 25% Heston,
+    20% Merton, 15% LSM American put) with target standard errors in
+    [1e-3, 1e-2] turned into path counts clipped to [1e4, 1e7]."""
+    from montepath_amd.engine import make_request
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    reqs = []
+    for i in range(n):
+        kind = rng.random()
+        se = 10 ** rng.uniform(-3, -2)
+        k = float(rng.uniform(80, 120))
+        sd = 14.0 if kind < 0.85 else 6.0  # pilot payoff standard deviations
+        paths = int(min(max((sd / se) ** 2, 1e4), 1e7))
+        if kind < 0.40:
+            reqs.append(make_request("european_call" if rng.random() < 0.5 else "arithmetic_asian_call",
+                                     n_paths=paths, n_steps=64, k=k, seed=11, contract_id=i))
+        elif kind < 0.65:
+            reqs.append(make_request("heston_european_call", n_paths=paths, n_steps=64, k=k,
+                                     seed=11, contract_id=i, heston_scheme="full_truncation",
+                                     **HESTON))
+        elif kind < 0.85:
+            reqs.append(make_request("merton_jump_diffusion_call", n_paths=paths, n_steps=0, k=k,
+                                     jump_intensity=1.0, jump_mean=-0.1, jump_volatility=0.15,
+                                     sigma=0.2, seed=11, contract_id=i))
+        else:
+            reqs.append(make_request("american_put", n_paths=min(paths, 1 << 20), n_steps=50,
+                                     basis_degree=3, r=0.05, sigma=0.25, seed=11, contract_id=i))
+    return reqs, arrivals
+
+
+def bench_serve(eng, seconds: float, rate: float) -> dict:
+    n = int(seconds * rate)
+    reqs, arrivals = serve_mix(n, rate)
+    _, lat, stats = eng.serve_trace(reqs, arrivals, max_inflight=2)
+    big = [lat[i] for i, r in enumerate(reqs) if r.n_paths >= 1_000_000]
+    return {"requests": n, "offered_rate_per_s": rate, "requests_per_s": stats["requests_per_s"],
+            "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"], "max_ms": stats["max_ms"],
+            "p99_ms_1e6plus_paths": float(np.percentile(big, 99)) if big else None,
+            "path_steps_per_s": stats["path_steps_per_s"], "launches": stats["launches"]}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip batch and serving")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this engine runs on one GPU only "
+              "(multi-GPU sharding by path range is a design note, not built)", file=sys.stderr)
+        return 2
+
+    from montepath_amd.engine import Engine
+
+    eng = Engine(0)
+    info = eng.device_info()
+    log(f"[bench] {info['device_id']} {info['name']} {info['arch']} CUs={info['compute_units']}")
+    req = heston_request(32)
+    for _ in range(args.warmup):
+        eng.price(req)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = eng.price(req)
+        kernel_ms.append(eng.last_timing()["main_kernel_ms"])
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    path_steps = N_PATHS * N_STEPS
+    value = path_steps / (dt / args.steps)
+    k_ms = float(np.mean(kernel_ms))
+    achieved = path_steps * FLOP_PER_PATH_STEP / (k_ms * 1e-3) / 1e12
+    log(f"[bench] {ms_per_step:.3f} ms/step, kernel {k_ms:.3f} ms, {value:.3e} path-steps/s")
+
+    # FP64 run on the same draws (not timed) and the characteristic-function check
+    res64 = eng.price(heston_request(64))
+    from oracle import closed_forms as cf
+
+    cf_prices = [cf.heston_call_lewis(100.0, k, 0.03, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
+                 for k in STRIKES]
+    out = {
+        "metric": "path-steps/sec and priced requests/sec + p99 latency at fixed stderr, 1 GPU",
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 paths, f64 accumulation",
+        "data": "synthetic (scrambled Sobol draws generated on device)",
+        "config": {"workload": "heston_full_truncation_sobol_bridge_3_strikes",
+                   "n_paths": N_PATHS, "n_steps": N_STEPS, "sobol_dims": 2 * N_STEPS,
+                   "strikes": STRIKES, "precision": "fp32_paths_fp64_kahan_sums",
+                   **HESTON},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "qmc_kernel<float,2,true>", "kernel_ms": k_ms,
+                     "flop_per_path_step": FLOP_PER_PATH_STEP,
+                     "note": "FP32 vector (VALU) work; MI355X FP32 peak 157.3 TFLOP/s; MFMA unused"},
+        "prices": res.strike_prices,
+        "stderrs": res.strike_stderrs,
+        "fp64_prices": res64.strike_prices,
+        "fp64_ref_error": [abs(a - b) for a, b in zip(res.strike_prices, res64.strike_prices)],
+        "characteristic_function": cf_prices,
+        "error_in_se": [(p - c) / s for p, c, s in zip(res.strike_prices, cf_prices, res.strike_stderrs)],
+    }
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = max(1, min(threads, 16))
+        out["cpu_baseline"] = cpu_baseline(threads)
+        log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3e} path-steps/s")
+    if not args.no_extra:
+        extra = {}
+        extra["batched_4096_contracts"] = bench_batch(eng)
+        log(f"[bench] batch {extra['batched_4096_contracts']}")
+        extra["serving"] = bench_serve(eng, args.serve_seconds, args.serve_rate)
+        log(f"[bench] serving {extra['serving']}")
+        out["extra"] = extra
+    eng.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -158,14 +158,29 @@ __device__ __forceinline__ R payoff_of(const path_state<R>& st, const model_para
 // ---------------------------------------------------------------------------
 // Work-queue tile kernel (GBM / Heston / Merton).  SRC 0: Philox, 1: buffer.
 // ---------------------------------------------------------------------------
+// Claim the next queue position for the whole wave.  Lane 0 takes it with one
+// atomic and parks it in the wave's LDS slot; every lane then reads the slot,
+// so all active lanes hold the same value by data flow, whatever exec mask the
+// compiler's loop structurisation leaves at this point.
+__device__ __forceinline__ unsigned int next_queue_position(unsigned int* counter,
+                                                            volatile unsigned int* slot,
+                                                            int lane) {
+    if (lane == 0) *slot = atomicAdd(counter, 1u);
+    __builtin_amdgcn_wave_barrier();
+    return __builtin_amdgcn_readfirstlane(*slot);
+}
+
 template <class R, int SRC, int PPL>
 __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
+    __shared__ unsigned int queue_slot[256 / MP_WAVE];
     const int lane = threadIdx.x & (MP_WAVE - 1);
-    for (;;) {
-        unsigned int qpos = 0;
-        if (lane == 0) qpos = atomicAdd(L.counter, 1u);
-        qpos = __builtin_amdgcn_readfirstlane(qpos);
-        if (qpos >= L.n_tiles) break;
+    volatile unsigned int* slot = &queue_slot[threadIdx.x >> 6];
+    // A wave claims at most n_tiles + 1 positions, so the loop ends even if the
+    // queue counter were left in a bad state.
+    unsigned int claims = 0;
+    for (unsigned int qpos = next_queue_position(L.counter, slot, lane);
+         qpos < L.n_tiles && claims <= L.n_tiles;
+         qpos = next_queue_position(L.counter, slot, lane), ++claims) {
         const uint32_t tile = L.order ? L.order[qpos] : qpos;
         const uint32_t ri = L.tile_req[tile];
         const mp_dreq& q = L.reqs[ri];

@@ -16,9 +16,12 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 from __future__ import annotations
 
 import math
+import warnings
 
 import numpy as np
 from scipy import integrate, special
+
+warnings.filterwarnings("ignore", category=integrate.IntegrationWarning)
 
 
 def norm_cdf(x: float) -> float:

@@ -1,0 +1,78 @@
+"""Writes tests/golden/reference_values.json from the reference's own data files.
+
+Sources (read as data, in a checkout of montepath):
+  benchmarks/reference-fixtures.json -- closed-form / CRR / series fixtures
+  benchmarks/release-results.json    -- published CPU benchmark rows
+
+Each published row is a mean over seeds 42..44 (9303..9305 for Merton) of one
+mc-core CPU pricer run on an 8-core Apple M2 with n_threads = 0 (all cores),
+at MC_PATHS = 100_000 and MC_STEPS = 64 (crates/mc-bench/src/harness.rs:42-44
+and the benchmark_mc_rust_cpu_* functions).  The configs below restate those
+functions; the values come from the JSON files.
+
+Usage: python tests/golden/make_reference_values.py [path/to/montepath]
+"""
+import json
+import os
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_values.json")
+
+# benchmark row -> (pricer, config), from crates/mc-bench/src/harness.rs
+ROWS = {
+    "mc_cpu_european_call_rust": ("gbm_stepwise", {"payoff": "european_call"}),  # :1729-1771
+    "mc_cpu_european_call_rust_terminal": ("european_terminal", {}),  # :2218-2260
+    "mc_cpu_arithmetic_asian_call_rust": ("gbm_stepwise", {"payoff": "arithmetic_asian_call"}),  # :2522
+    "mc_cpu_down_and_out_call_rust": ("gbm_stepwise", {"payoff": "down_and_out_call", "barrier": 80.0}),  # :3520
+    "mc_cpu_lookback_call_rust": ("gbm_stepwise", {"payoff": "lookback_call"}),  # :3643
+    "mc_cpu_heston_european_call_rust": (
+        "heston", {"kappa": 1.5, "theta": 0.04, "v0": 0.04, "xi": 0.3, "rho": -0.6}),  # :3923
+    "mc_cpu_merton_jump_diffusion_call_rust": (
+        "merton", {"lam": 0.4, "mu_j": -0.08, "sigma_j": 0.25, "seed": 9303}),  # :4004
+    "mc_cpu_basket_call_rust": (
+        "basket2", {"s01": 100.0, "s02": 95.0, "sigma1": 0.2, "sigma2": 0.25, "rho": 0.35,
+                    "weight1": 0.5, "weight2": 0.5}),  # :4456
+    "mc_cpu_american_put_lsm_rust": ("american_lsm", {"basis_degree": 2}),  # :3730
+    "mc_cpu_bermudan_put_lsm_rust": (
+        "bermudan_lsm", {"basis_degree": 2, "exercise_steps": [16, 32, 48, 64]}),  # :3773
+    # scrambled-Sobol rows: sobol_burley 0.5 bits, not reproducible without the crate
+    "mc_cpu_european_call_rust_scrambled_sobol": ("gbm_qmc", {"bridge": False}),  # :3356
+    "mc_cpu_european_call_rust_scrambled_sobol_brownian_bridge": ("gbm_qmc", {"bridge": True}),
+}
+
+COMMON = {"s0": 100.0, "k": 100.0, "r": 0.03, "sigma": 0.2, "t": 1.0, "n_paths": 100_000,
+          "n_steps": 64, "seed": 42, "n_threads": 8, "repeats": 3}
+
+
+def main() -> None:
+    with open(os.path.join(REF, "benchmarks", "reference-fixtures.json")) as f:
+        fixtures = json.load(f)
+    with open(os.path.join(REF, "benchmarks", "release-results.json")) as f:
+        release = json.load(f)
+    fx = {}
+    for item in fixtures["fixtures"]:
+        fx[item["fixture_id"]] = {"parameters": item["parameters"], "expected": item["expected"]}
+    rows = {}
+    for r in release["results"]:
+        name = r["benchmark_name"]
+        if name in ROWS:
+            kind, cfg = ROWS[name]
+            full = dict(COMMON)
+            full.update(cfg)
+            rows[name] = {"pricer": kind, "config": full, "metric_name": r["metric_name"],
+                          "published_value": r["metric_value"],
+                          "per_iteration_us": r["per_iteration_us"]}
+    out = {
+        "source": "montepath benchmarks/reference-fixtures.json and benchmarks/release-results.json",
+        "note": "published rows are means over seeds seed..seed+2 with n_threads=8",
+        "fixtures": fx,
+        "published_rows": rows,
+    }
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print(OUT, len(fx), "fixtures", len(rows), "rows")
+
+
+if __name__ == "__main__":
+    main()

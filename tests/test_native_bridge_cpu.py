@@ -1,0 +1,126 @@
+"""The price_*_rocm bridge contract on the CPU, with an engine double.
+
+Checks the parts that mirror crates/montepath-python/src/lib.rs: key aliases
+and defaults (fill_gbm_fields :652-678, heston/merton/basket/put parsers),
+the response schema of metal_pricing_response (:730-768) with backend
+rocm_hip, and ValueError on bad input -- the same style of test double the
+reference uses for its Python surface (python/tests/test_python_surface.py).
+"""
+import math
+
+import pytest
+
+from montepath_amd import _abi as A
+from montepath_amd import native
+from montepath_amd.engine import PriceResult
+
+
+class FakeEngine:
+    def __init__(self, fail_code=None):
+        self.requests = []
+        self.fail_code = fail_code
+
+    def price(self, req):
+        self.requests.append(req)
+        if self.fail_code:
+            raise A.MontepathHipError(self.fail_code, "device said no")
+        ns = max(req.n_strikes, 1)
+        return PriceResult(price=1.25, stderr=0.01, runtime_ms=0.5, fp64_ref_error=math.nan,
+                           strike_prices=[1.25] * ns, strike_stderrs=[0.01] * ns,
+                           n_paths=req.n_paths, n_steps=req.n_steps, regression_steps=7)
+
+
+@pytest.fixture
+def fake():
+    eng = FakeEngine()
+    native.set_engine_factory(lambda: eng)
+    yield eng
+    native.set_engine_factory(lambda: None)
+
+
+def test_aliases_and_defaults(fake):
+    out = native.price_european_call_rocm({"spot": 90.0, "strike": 95.0, "volatility": 0.3,
+                                           "maturity": 2.0, "rate": 0.01, "paths": 1000,
+                                           "steps": 16})
+    r = fake.requests[-1]
+    assert (r.s0, r.k, r.sigma, r.t, r.r, r.n_paths, r.n_steps) == (90.0, 95.0, 0.3, 2.0, 0.01, 1000, 16)
+    assert r.seed == 42 and r.workload == A.EUROPEAN_CALL and r.method == A.METHOD_STEPWISE
+    # primary keys win over aliases, as in lookup() (lib.rs:813-820)
+    native.price_european_call_rocm({"s0": 80.0, "spot": 90.0})
+    assert fake.requests[-1].s0 == 80.0
+    assert out["price"] == 1.25 and out["stderr"] == 0.01
+
+
+def test_response_schema(fake):
+    out = native.price_arithmetic_asian_call_rocm({"n_paths": 10})
+    assert set(out) == {"price", "stderr", "values", "manifest", "warnings"}
+    assert out["values"] == {"price": 1.25, "stderr": 0.01, "runtime_ms": 0.5}
+    m = out["manifest"]
+    assert m["schema_version"] == "python-native-runtime.v1"
+    assert m["backend"] == "rocm_hip" and m["execution_mode"] == "native_extension_rocm"
+    assert m["workload"] == "arithmetic_asian_call"
+    assert m["function"] == "price_arithmetic_asian_call_rocm"
+    assert m["config"]["n_paths"] == 10 and m["config"]["technique"] == "standard"
+
+
+def test_terminal_method_and_multi_strike(fake):
+    native.price_european_call_rocm({"method": "terminal-distribution"})
+    assert fake.requests[-1].method == A.METHOD_TERMINAL
+    out = native.price_heston_european_call_rocm({"strikes": [80, 100, 120], "vol_of_vol": 0.5,
+                                                  "scheme": "full_truncation"})
+    r = fake.requests[-1]
+    assert r.n_strikes == 3 and list(r.strikes)[:3] == [80.0, 100.0, 120.0]
+    assert r.heston_scheme == A.HESTON_FULL_TRUNCATION and r.xi == 0.5
+    assert len(out["values"]["strike_prices"]) == 3
+
+
+def test_put_details_and_schedule(fake):
+    out = native.price_bermudan_put_rocm({"exercise_steps": [8, 16], "steps": 16})
+    r = fake.requests[-1]
+    assert r.n_exercise == 2 and [r.exercise_steps[i] for i in range(2)] == [8, 16]
+    assert out["values"]["regression_steps"] == 7
+    native.price_american_put_rocm({})
+    r = fake.requests[-1]
+    assert r.workload == A.AMERICAN_PUT and r.n_steps == 64 and r.basis_degree == 2
+
+
+def test_basket_reference_keys(fake):
+    native.price_basket_call_rocm({"spot1": 100.0, "spot2": 90.0, "correlation": 0.5})
+    r = fake.requests[-1]
+    assert r.n_assets == 2 and r.asset_s0[1] == 90.0
+    assert r.correlation[1] == 0.5 and r.correlation[2] == 0.5 and r.correlation[0] == 1.0
+    assert r.asset_weight[0] == 0.5 and r.n_steps == 1
+
+
+def test_merton_defaults_are_terminal(fake):
+    native.price_merton_jump_diffusion_call_rocm({})
+    r = fake.requests[-1]
+    assert r.n_steps == 0 and r.jump_intensity == 0.4 and r.jump_mean == -0.08
+
+
+@pytest.mark.parametrize("fn,cfg,msg", [
+    (native.price_european_call_rocm, {"technique": "antithetic"}, "standard technique only"),
+    (native.price_european_call_rocm, {"technique": "bogus"}, "unsupported Monte Carlo technique"),
+    (native.price_european_call_rocm, {"sampling": "latin-hypercube"}, "does not offer sampling"),
+    (native.price_european_call_rocm, {"sampling": "bogus"}, "unsupported sampling method"),
+    (native.price_european_call_rocm, {"method": "bogus"}, "unsupported European call method"),
+    (native.price_european_call_rocm, {"n_paths": -5}, "non-negative integer"),
+    (native.price_european_call_rocm, {"precision": "fp16"}, "unsupported precision"),
+])
+def test_bad_configs_raise_value_error(fake, fn, cfg, msg):
+    with pytest.raises(ValueError, match=msg):
+        fn(cfg)
+
+
+def test_engine_errors_become_value_errors():
+    native.set_engine_factory(lambda: FakeEngine(fail_code=A.MP_E_INVALID_CONFIG))
+    try:
+        with pytest.raises(ValueError, match="native ROCm execution failed: MP_E_INVALID_CONFIG"):
+            native.price_european_call_rocm({})
+    finally:
+        native.set_engine_factory(lambda: None)
+
+
+def test_known_function_list_is_complete():
+    for name in native.KNOWN_ROCM_FUNCTIONS:
+        assert callable(getattr(native, name))

@@ -1,0 +1,289 @@
+"""The CPU oracle pinned against the reference's own numbers and mathematics.
+
+* published CPU benchmark rows (benchmarks/release-results.json) reproduce
+  bit-for-bit: they pin the xorshift64*/Box-Muller stream, the splitmix64
+  chunk seeds and thread partition, and every pricer's arithmetic;
+* reference fixtures (benchmarks/reference-fixtures.json): Black-Scholes,
+  CRR American/Bermudan, Merton series, Heston BS limit;
+* published known answers: Random123 Philox4x32-10 KAT vectors, the
+  splitmix64 sequence, scipy's unscrambled Joe-Kuo Sobol points, Acklam's
+  stated accuracy against scipy's ndtri;
+* invariants: Brownian-bridge order/covariance, Sobol stratification,
+  degenerate model limits.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_values.json")))
+ROWS = GOLD["published_rows"]
+FIX = GOLD["fixtures"]
+
+
+def _row_price(O, name):
+    kind, c = ROWS[name]["pricer"], ROWS[name]["config"]
+    prices = []
+    for i in range(c["repeats"]):
+        seed = c["seed"] + i
+        if kind == "gbm_stepwise":
+            g = O.Gbm(s0=c["s0"], k=c["k"], r=c["r"], sigma=c["sigma"], t=c["t"],
+                      n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed,
+                      n_threads=c["n_threads"], barrier=c.get("barrier", 80.0))
+            prices.append(O.gbm_price(g, c["payoff"])[0])
+        elif kind == "european_terminal":
+            g = O.Gbm(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed, n_threads=c["n_threads"])
+            prices.append(O.european_terminal_price(g)[0])
+        elif kind == "heston":
+            h = O.Heston(kappa=c["kappa"], theta=c["theta"], v0=c["v0"], xi=c["xi"], rho=c["rho"],
+                         n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed,
+                         n_threads=c["n_threads"])
+            prices.append(O.heston_price(h, O.HESTON_REFERENCE_FLOOR)[0])
+        elif kind == "merton":
+            m = O.Merton(lam=c["lam"], mu_j=c["mu_j"], sigma_j=c["sigma_j"], n_paths=c["n_paths"],
+                         seed=seed, n_threads=c["n_threads"])
+            prices.append(O.merton_price(m)[0])
+        elif kind == "basket2":
+            prices.append(O.basket2_price(s01=c["s01"], s02=c["s02"], sigma1=c["sigma1"],
+                                          sigma2=c["sigma2"], rho=c["rho"], weight1=c["weight1"],
+                                          weight2=c["weight2"], n_paths=c["n_paths"], seed=seed)[0])
+        elif kind in ("american_lsm", "bermudan_lsm"):
+            cfg = O.Lsm(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed,
+                        basis_degree=c["basis_degree"],
+                        allow_intrinsic_now=kind == "american_lsm")
+            prices.append(O.lsm_price(cfg, c.get("exercise_steps"))["price"])
+        else:
+            raise AssertionError(kind)
+    return float(np.mean(prices))
+
+
+PSEUDORANDOM_ROWS = [n for n, r in ROWS.items() if r["pricer"] != "gbm_qmc"]
+
+
+@pytest.mark.parametrize("name", PSEUDORANDOM_ROWS)
+def test_published_cpu_rows_reproduce_bit_for_bit(oracle, name):
+    got = _row_price(oracle, name)
+    want = ROWS[name]["published_value"]
+    # identical up to the last bit of the mean of three prices
+    assert abs(got - want) <= 4 * np.spacing(want), (name, got, want)
+
+
+def test_black_scholes_fixture(oracle):
+    want = FIX["black_scholes_european_call_atm_1y"]["expected"]["price"]
+    assert oracle.bs_call(100, 100, 0.03, 0.2, 1.0) == pytest.approx(want, abs=1e-14)
+    # the reference function itself uses the A&S 7.1.26 CDF (|error| < 1.5e-7)
+    assert abs(oracle.bs_call(100, 100, 0.03, 0.2, 1.0, exact=False) - want) < 2e-5
+    lb = FIX["black_scholes_european_put_atm_1y_lower_bound"]["expected"]["european_put_lower_bound"]
+    assert oracle.bs_put(100, 100, 0.03, 0.2, 1.0) == pytest.approx(lb, abs=1e-13)
+
+
+def test_crr_fixtures(oracle):
+    am = FIX["crr_american_put_atm_1y_reference"]["expected"]["crr_binomial_price"]
+    assert oracle.crr_put(100, 100, 0.03, 0.2, 1.0, 512, range(0, 513)) == pytest.approx(am, abs=1e-12)
+    be = FIX["crr_bermudan_put_atm_1y_quarterly_reference"]
+    steps = be["parameters"]["mapped_reference_exercise_steps"]
+    got = oracle.crr_put(100, 100, 0.03, 0.2, 1.0, 512, steps)
+    assert got == pytest.approx(be["expected"]["crr_binomial_price"], abs=1e-12)
+
+
+def test_merton_series_fixture_and_zero_intensity_limit(oracle):
+    want = FIX["merton_jump_diffusion_call_series_reference"]["expected"]["price"]
+    assert oracle.merton_series(oracle.Merton(), 96, 1e-12) == pytest.approx(want, abs=1e-12)
+    # lambda = 0 reduces to Black-Scholes (cpu_runtime.rs:57-73)
+    m0 = oracle.Merton(lam=0.0)
+    assert oracle.merton_series(m0) == pytest.approx(oracle.bs_call(100, 100, 0.03, 0.2, 1.0), abs=1e-10)
+    p, se = oracle.merton_price(oracle.Merton(lam=0.0, n_paths=200_000, n_threads=4))
+    assert abs(p - oracle.bs_call(100, 100, 0.03, 0.2, 1.0)) <= 4 * se + 0.05
+
+
+def test_heston_black_scholes_limit(oracle):
+    fx = FIX["heston_black_scholes_limit_atm_1y"]
+    bs = fx["expected"]["black_scholes_limit_price"]
+    h = oracle.Heston(xi=0.0, v0=0.04, theta=0.04, kappa=1.5, rho=-0.6, n_paths=100_000,
+                      n_steps=64, n_threads=4)
+    p, se = oracle.heston_price(h, oracle.HESTON_REFERENCE_FLOOR)
+    assert abs(p - bs) <= 4 * se + 0.05  # cpu_runtime.rs:1510-1532
+
+
+def test_heston_characteristic_function_two_quadratures(oracle):
+    from oracle import closed_forms as cf
+
+    # survey anchors for the 2^22 x 256 precision workload
+    anchors = {80.0: 23.8732403168, 100.0: 8.9294104536, 120.0: 1.2935081216}
+    for k, a in anchors.items():
+        p1 = cf.heston_call_p1p2(100, k, 0.03, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
+        p2 = cf.heston_call_lewis(100, k, 0.03, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
+        assert abs(p1 - p2) <= 1e-10
+        assert abs(p2 - a) <= 1e-9
+    # vanishing vol-of-vol with theta = v0 is Black-Scholes at sigma = sqrt(v0)
+    # (the correction is first order in xi)
+    bs = oracle.bs_call(100, 100, 0.03, 0.2, 1.0)
+    d3 = cf.heston_call_lewis(100, 100, 0.03, 0.04, 2.0, 0.04, 1e-3, -0.7, 1.0) - bs
+    d4 = cf.heston_call_lewis(100, 100, 0.03, 0.04, 2.0, 0.04, 1e-4, -0.7, 1.0) - bs
+    assert abs(d4) <= 5e-5 and 8.0 <= d3 / d4 <= 12.0
+    # put-call parity of the Lewis form against a direct P1/P2 put
+    c = cf.heston_call_lewis(100, 90, 0.03, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
+    assert c > max(100 - 90 * math.exp(-0.03), 0)
+
+
+def test_rng_seed_zero_and_splitmix_known_answers(oracle):
+    # seed 0 is replaced by the golden-ratio word (cpu.rs:2962-2967)
+    assert np.array_equal(oracle.rng_u64(0, 4), oracle.rng_u64(0x9E3779B97F4A7C15, 4))
+    # splitmix64 from state 0: the published first outputs of the generator
+    assert oracle.splitmix64(0) == 0xE220A8397B1DCDAF
+    assert oracle.splitmix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+    # chunk seeds: chunk i uses splitmix64(seed + i * golden)
+    assert oracle.derive_chunk_seed(42, 0) == oracle.splitmix64(42)
+
+
+def test_box_muller_pairs(oracle):
+    # pairs (r cos, r sin) share one radius: z0^2 + z1^2 = -2 ln u1
+    z = oracle.rng_normals(123, 20000).reshape(-1, 2)
+    r2 = (z ** 2).sum(axis=1)
+    assert np.all(r2 > 0)
+    assert abs(z.mean()) < 0.02 and abs(z.var() - 1) < 0.03
+
+
+def test_philox_known_answer_vectors(oracle):
+    # Random123 kat_vectors for philox4x32-10
+    assert oracle.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert oracle.philox4x32_10([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [
+        0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert oracle.philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                                [0xA4093822, 0x299F31D0]) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_inverse_normal_accuracy_and_symmetry(oracle):
+    from scipy.special import ndtri
+
+    ps = np.concatenate([np.linspace(1e-12, 0.02425, 500), np.linspace(0.02425, 0.97575, 2000),
+                         np.linspace(0.97575, 1 - 1e-12, 500)])
+    got = np.array([oracle.inverse_normal(p) for p in ps])
+    ref = ndtri(ps)
+    assert float(np.max(np.abs(got - ref) / np.abs(ref).clip(1e-3))) < 1.2e-9  # Acklam's bound
+    for p in (0.001, 0.1, 0.3, 0.49):
+        assert oracle.inverse_normal(1 - p) == pytest.approx(-oracle.inverse_normal(p), rel=1e-9)
+    # clamp at f64::MIN_POSITIVE: p = 0 is finite
+    assert math.isfinite(oracle.inverse_normal(0.0)) and oracle.inverse_normal(0.0) < -30
+
+
+def test_brownian_bridge_plan_is_dfs_preorder(oracle):
+    ops = oracle.bb_plan(8, 1.0)
+    assert [o["mid"] for o in ops] == [4, 2, 1, 3, 6, 5, 7]
+    assert [o["normal_index"] for o in ops] == list(range(1, 8))
+    assert len(oracle.bb_plan(256, 1.0)) == 255
+    assert oracle.bb_plan(1, 1.0) == []
+
+
+def test_brownian_bridge_increments_are_iid_standard_normal(oracle):
+    m, t = 16, 2.0
+    rng = np.random.default_rng(1)
+    inc = np.array([oracle.bb_increments(m, t, rng.standard_normal(m)) for _ in range(20000)])
+    cov = np.cov(inc.T)
+    assert np.max(np.abs(cov - np.eye(m))) < 0.05
+    # the increments telescope to W(T) = sqrt(T) * input[0]
+    x = rng.standard_normal(m)
+    w = oracle.bb_increments(m, t, x) * math.sqrt(t / m)
+    assert w.sum() == pytest.approx(math.sqrt(t) * x[0], abs=1e-12)
+
+
+def test_sobol_unscrambled_points_match_scipy(oracle):
+    from scipy.stats import qmc
+
+    dims, m = 256, 10
+    pts = qmc.Sobol(d=dims, scramble=False, bits=32).random_base2(m)
+    ints = np.rint(pts * 2.0 ** 32).astype(np.uint64)
+    for i in (1, 2, 3, 17, 500, 1023):
+        g = i ^ (i >> 1)  # scipy walks the sequence in Gray-code order
+        mine = [oracle.sobol_u32(g, d) for d in range(dims)]
+        assert mine == [int(v) for v in ints[i]], i
+
+
+def test_scrambled_sobol_keeps_net_stratification(oracle):
+    # every dimension of the first 2^m points of a block is stratified into
+    # 2^m equal intervals; dims (0, 1) form a (0, m, 2)-net
+    seed = 7
+    n = 1 << 10
+    u = np.array([[oracle.sobol_sample(i, d, seed) for d in (0, 1, 5, 200)] for i in range(n)])
+    for m in (4, 7, 10):
+        for d in range(4):
+            cells = np.floor(u[: 1 << m, d] * (1 << m)).astype(int)
+            assert sorted(cells) == list(range(1 << m))
+    for a in range(0, 11, 2):  # 2^a x 2^(10-a) boxes
+        cx = np.floor(u[:, 0] * (1 << a)).astype(int)
+        cy = np.floor(u[:, 1] * (1 << (10 - a))).astype(int)
+        assert len(set(zip(cx, cy))) == n
+    # different seeds give different scrambles of the same net
+    assert oracle.sobol_sample(3, 0, 1) != oracle.sobol_sample(3, 0, 2)
+
+
+def test_sobol_point_normals_follow_block_and_wrap_rules(oracle):
+    seed = 42
+    a = oracle.sobol_point_normals(seed, 70000, 300)
+    block, local = 70000 >> 16, 70000 & 0xFFFF
+    bseed = (seed + block * 0x9E3779B9) & 0xFFFFFFFF
+    assert a[10] == oracle.inverse_normal(oracle.sobol_sample(local, 10, bseed))
+    assert a[260] == oracle.inverse_normal(oracle.sobol_sample(local, 4, (bseed + 1) & 0xFFFFFFFF))
+
+
+def test_lsm_invariants(oracle):
+    cfg = oracle.Lsm(n_paths=50_000, n_steps=32, seed=11, basis_degree=2)
+    res = oracle.lsm_price(cfg)
+    assert res["regression_steps"] == 31  # cpu_runtime.rs:1334-1355
+    put = oracle.bs_put(100, 100, 0.03, 0.2, 1.0)
+    assert res["price"] >= put - 3 * res["stderr"]
+    crr = FIX["crr_american_put_atm_1y_reference"]["expected"]["crr_binomial_price"]
+    assert abs(res["price"] - crr) <= 5 * res["stderr"] + 0.15  # cpu_runtime.rs:1358-1377
+    # the solver refuses a singular system (pivot <= 1e-12)
+    assert oracle.lsm_solve(np.zeros((4, 4)), np.ones(4), 2) is None
+    sol = oracle.lsm_solve(np.diag([2.0, 4.0, 1.0, 1.0]), np.array([2.0, 8.0, 0, 0]), 2)
+    assert sol[:2] == pytest.approx([1.0, 2.0])
+
+
+def test_lookback_dominates_european_and_asian_deterministic_limit(oracle):
+    e = oracle.gbm_price(oracle.Gbm(n_paths=20000, n_steps=32, seed=5), "european_call")
+    lb = oracle.gbm_price(oracle.Gbm(n_paths=20000, n_steps=32, seed=5), "lookback_call")
+    assert lb[0] >= e[0]  # cpu_runtime.rs:1276-1295
+    # sigma -> 0: the Asian payoff is deterministic
+    m, s0, k, r = 12, 100.0, 95.0, 0.05
+    a = oracle.gbm_price(oracle.Gbm(sigma=1e-12, n_paths=100, n_steps=m, k=k, r=r), "arithmetic_asian_call")
+    avg = np.mean([s0 * math.exp(r * (i + 1) / m) for i in range(m)])
+    assert a[0] == pytest.approx((avg - k) * math.exp(-r), rel=1e-9)
+
+
+def test_two_asset_basket_reduces_to_reference_formula(oracle):
+    # the n-asset stepped basket with one step and two assets is the
+    # reference's two-asset terminal basket (cpu.rs:7622-7635)
+    n = 500
+    rho, s0, sig, w = 0.35, [100.0, 95.0], [0.2, 0.25], [0.5, 0.5]
+    z = np.random.default_rng(3).standard_normal((n, 1, 2))
+    got = oracle.basket_paths_from_normals(s0, sig, w, [[1, rho], [rho, 1]], 0.03, 1.0, 1, n, z)
+    z1, z2i = z[:, 0, 0], z[:, 0, 1]
+    z2 = rho * z1 + math.sqrt(1 - rho * rho) * z2i
+    s1 = 100 * np.exp((0.03 - 0.02) + 0.2 * z1)
+    s2 = 95 * np.exp((0.03 - 0.5 * 0.0625) + 0.25 * z2)
+    assert np.max(np.abs(got - (0.5 * s1 + 0.5 * s2)) / got) < 1e-13
+
+
+def test_stepped_merton_matches_series(oracle):
+    n, m = 200_000, 8
+    cfg = oracle.Merton(n_paths=n, lam=1.0, mu_j=-0.1, sigma_j=0.15)
+    rng = np.random.default_rng(11)
+    z, u, zj = rng.standard_normal((n, m)), rng.random((n, m)), rng.standard_normal((n, m))
+    term, _ = oracle.merton_paths_from_draws(cfg, m, z, u, zj)
+    pay = np.maximum(term - 100.0, 0) * math.exp(-0.03)
+    ref = oracle.merton_series(cfg)
+    assert abs(pay.mean() - ref) <= 3.5 * pay.std() / math.sqrt(n)
+
+
+def test_heston_full_truncation_oracle_near_characteristic_function(oracle):
+    from oracle import closed_forms as cf
+
+    h = oracle.Heston(n_paths=200_000, n_steps=64, kappa=2.0, xi=0.5, rho=-0.7, n_threads=8)
+    p, se = oracle.heston_price(h, oracle.HESTON_FULL_TRUNCATION)
+    ref = cf.heston_call_lewis(100, 100, 0.03, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
+    assert abs(p - ref) <= 4 * se
