@@ -473,42 +473,43 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
                 emit(z0, z1);
             }
         } else {
-            R cur[F];
+            // The bridge carries interval increments D = W(r) - W(l) rather than
+            // W itself: with lw + rw = 1 the reference's midpoint rule
+            // W(m) = lw W(l) + rw W(r) + sd z is W(m) - W(l) = rw D + sd z, so
+            // every stored quantity has the size of an increment and FP32 never
+            // subtracts two O(1) values.  Stack slot s holds D of the interval
+            // whose right end is owned by that slot.
 #pragma unroll
-            for (int f = 0; f < F; ++f) {
-                const R wT = sqrt_t * cur_s.normal(f, nib);
-                stack[(0 * F + f) * bd + tid] = wT;
-                cur[f] = R(0);
-            }
+            for (int f = 0; f < F; ++f) stack[(0 * F + f) * bd + tid] = sqrt_t * cur_s.normal(f, nib);
             if (qp.n_ops == 0) {
-                const R i0 = (stack[tid] - cur[0]) * inv_sqrt_dt;
-                const R i1 = F == 2 ? (stack[bd + tid] - cur[F - 1]) * inv_sqrt_dt : R(0);
+                const R i0 = stack[tid] * inv_sqrt_dt;
+                const R i1 = F == 2 ? stack[bd + tid] * inv_sqrt_dt : R(0);
                 emit(i0, i1);
             }
             for (int k = 0; k < qp.n_ops; ++k) {
                 const mp_bbop op = plan[k];
-                const R lw = R(op.lw), rw = R(op.rw), sd = R(op.sd);
-                R wm[F], wr[F];
+                const R rw = R(op.rw), sd = R(op.sd);
+                R dl[F], dr[F];
 #pragma unroll
                 for (int f = 0; f < F; ++f) {
                     const R z = cur_s.normal((k + 1) * F + f, nib);
-                    wr[f] = stack[(op.slot * F + f) * bd + tid];
-                    wm[f] = lw * cur[f] + rw * wr[f] + sd * z;
+                    const R d = stack[(op.slot * F + f) * bd + tid];
+                    dl[f] = rw * d + sd * z;
+                    dr[f] = d - dl[f];
                 }
                 if (op.flags & 1) {
-                    emit((wm[0] - cur[0]) * inv_sqrt_dt,
-                         F == 2 ? (wm[F - 1] - cur[F - 1]) * inv_sqrt_dt : R(0));
-#pragma unroll
-                    for (int f = 0; f < F; ++f) cur[f] = wm[f];
-                    if (op.flags & 2) {
-                        emit((wr[0] - cur[0]) * inv_sqrt_dt,
-                             F == 2 ? (wr[F - 1] - cur[F - 1]) * inv_sqrt_dt : R(0));
-#pragma unroll
-                        for (int f = 0; f < F; ++f) cur[f] = wr[f];
-                    }
+                    emit(dl[0] * inv_sqrt_dt, F == 2 ? dl[F - 1] * inv_sqrt_dt : R(0));
                 } else {
 #pragma unroll
-                    for (int f = 0; f < F; ++f) stack[((op.slot + 1) * F + f) * bd + tid] = wm[f];
+                    for (int f = 0; f < F; ++f) stack[((op.slot + 1) * F + f) * bd + tid] = dl[f];
+                }
+                if (op.flags & 2) {
+                    // right child is a leaf: it follows the left child in step
+                    // order, which is already complete when the left is a leaf
+                    emit(dr[0] * inv_sqrt_dt, F == 2 ? dr[F - 1] * inv_sqrt_dt : R(0));
+                } else {
+#pragma unroll
+                    for (int f = 0; f < F; ++f) stack[(op.slot * F + f) * bd + tid] = dr[f];
                 }
             }
         }

@@ -166,8 +166,16 @@ def test_lsm_fp64_matches_oracle_on_reference_normals(engine, oracle):
     assert abs(res.price - ref["price"]) <= 1e-10 * ref["price"]
     assert abs(res.stderr - ref["stderr"]) <= 1e-8 * ref["stderr"]
     assert np.mean(ex == ref["exercise_steps"]) >= 0.9999
-    nz = np.abs(ref["coefficients"]) > 0
-    assert relmax(coeff[nz], ref["coefficients"][nz]) <= 1e-8
+    # degree-3 Laguerre Gram matrices are ill-conditioned, so coefficients
+    # carry the summation order; the fitted continuation value does not
+    x = np.linspace(0.6, 1.0, 41)  # in-the-money moneyness S/K
+    basis = np.stack([np.ones_like(x), 1 - x, 1 - 2 * x + 0.5 * x * x,
+                      1 - 3 * x + 1.5 * x * x - x ** 3 / 6.0])
+    for d in range(1, m):
+        if np.any(ref["coefficients"][d]):
+            cont_ref = ref["coefficients"][d] @ basis
+            cont = coeff[d] @ basis
+            assert np.max(np.abs(cont - cont_ref)) <= 1e-7 * np.max(np.abs(cont_ref)), d
 
 
 def test_lsm_fp32_within_noise_of_fp64(engine):
