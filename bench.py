@@ -119,46 +119,74 @@ def bench_batch(eng, reps: int = 5) -> dict:
             "mean_price": float(np.mean([x.price for x in res]))}
 
 
-def serve_mix(n: int, rate: float, seed: int = 11):
-    """Poisson arrivals of mixed requests (40% GBM European/Asian, 25% Heston,
-    20% Merton, 15% LSM American put) with target standard errors in
-    [1e-3, 1e-2] turned into path counts clipped to [1e4, 1e7]."""
+SERVE_KINDS = ("european_call", "arithmetic_asian_call", "heston_european_call",
+               "merton_jump_diffusion_call", "american_put")
+
+
+def serve_request(kind: str, k: float, paths: int, contract: int, precision: int = 32):
     from montepath_amd.engine import make_request
 
+    common = dict(seed=11, contract_id=contract, precision=precision)
+    if kind in ("european_call", "arithmetic_asian_call"):
+        return make_request(kind, n_paths=paths, n_steps=64, k=k, **common)
+    if kind == "heston_european_call":
+        return make_request(kind, n_paths=paths, n_steps=64, k=k, heston_scheme="full_truncation",
+                            **HESTON, **common)
+    if kind == "merton_jump_diffusion_call":
+        return make_request(kind, n_paths=paths, n_steps=0, k=k, jump_intensity=1.0,
+                            jump_mean=-0.1, jump_volatility=0.15, sigma=0.2, **common)
+    return make_request("american_put", n_paths=paths, n_steps=50, k=k, basis_degree=3,
+                        r=0.05, sigma=0.25, **common)
+
+
+def serve_pilot(eng) -> dict:
+    """Coefficient of variation (payoff sd / price) per request kind and
+    strike, from 2^16-path pilots; interpolated in K."""
+    grid = np.linspace(80.0, 120.0, 9)
+    cv = {}
+    for kind in SERVE_KINDS:
+        vals = []
+        for k in grid:
+            r = eng.price(serve_request(kind, float(k), 1 << 16, 0))
+            vals.append(r.stderr * math.sqrt(1 << 16) / max(r.price, 1e-12))
+        cv[kind] = (grid, np.array(vals))
+    return cv
+
+
+def serve_mix(n: int, rate: float, cv: dict, seed: int = 11):
+    """Poisson arrivals of mixed requests -- 40% GBM European/Asian, 25% Heston,
+    20% Merton, 15% LSM American put; K~U[80,120] -- each with a target
+    standard error of 1e-3 to 1e-2 (log-uniform) relative to its price, which
+    the pilot's coefficient of variation turns into a path count clipped to
+    [1e4, 1e7]."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
     reqs = []
     for i in range(n):
-        kind = rng.random()
-        se = 10 ** rng.uniform(-3, -2)
+        u = rng.random()
+        kind = (SERVE_KINDS[0] if rng.random() < 0.5 else SERVE_KINDS[1]) if u < 0.40 else \
+            SERVE_KINDS[2] if u < 0.65 else SERVE_KINDS[3] if u < 0.85 else SERVE_KINDS[4]
         k = float(rng.uniform(80, 120))
-        sd = 14.0 if kind < 0.85 else 6.0  # pilot payoff standard deviations
-        paths = int(min(max((sd / se) ** 2, 1e4), 1e7))
-        if kind < 0.40:
-            reqs.append(make_request("european_call" if rng.random() < 0.5 else "arithmetic_asian_call",
-                                     n_paths=paths, n_steps=64, k=k, seed=11, contract_id=i))
-        elif kind < 0.65:
-            reqs.append(make_request("heston_european_call", n_paths=paths, n_steps=64, k=k,
-                                     seed=11, contract_id=i, heston_scheme="full_truncation",
-                                     **HESTON))
-        elif kind < 0.85:
-            reqs.append(make_request("merton_jump_diffusion_call", n_paths=paths, n_steps=0, k=k,
-                                     jump_intensity=1.0, jump_mean=-0.1, jump_volatility=0.15,
-                                     sigma=0.2, seed=11, contract_id=i))
-        else:
-            reqs.append(make_request("american_put", n_paths=min(paths, 1 << 20), n_steps=50,
-                                     basis_degree=3, r=0.05, sigma=0.25, seed=11, contract_id=i))
+        rel_se = 10 ** rng.uniform(-3, -2)
+        g, v = cv[kind]
+        paths = int(min(max((float(np.interp(k, g, v)) / rel_se) ** 2, 1e4), 1e7))
+        reqs.append(serve_request(kind, k, paths, i))
     return reqs, arrivals
 
 
 def bench_serve(eng, seconds: float, rate: float) -> dict:
     n = int(seconds * rate)
-    reqs, arrivals = serve_mix(n, rate)
+    cv = serve_pilot(eng)
+    reqs, arrivals = serve_mix(n, rate, cv)
     _, lat, stats = eng.serve_trace(reqs, arrivals, max_inflight=2)
-    big = [lat[i] for i, r in enumerate(reqs) if r.n_paths >= 1_000_000]
-    return {"requests": n, "offered_rate_per_s": rate, "requests_per_s": stats["requests_per_s"],
+    paths = np.array([r.n_paths for r in reqs])
+    big = lat[paths >= 1_000_000]
+    return {"requests": n, "offered_rate_per_s": rate, "seconds": seconds,
+            "requests_per_s": stats["requests_per_s"],
             "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"], "max_ms": stats["max_ms"],
-            "p99_ms_1e6plus_paths": float(np.percentile(big, 99)) if big else None,
+            "requests_1e6plus_paths": int(big.size),
+            "p99_ms_1e6plus_paths": float(np.percentile(big, 99)) if big.size else None,
+            "paths_p50": float(np.median(paths)), "paths_max": int(paths.max()),
             "path_steps_per_s": stats["path_steps_per_s"], "launches": stats["launches"]}
 
 

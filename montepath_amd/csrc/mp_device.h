@@ -17,16 +17,15 @@ __device__ __forceinline__ mp_u4 mp_philox(uint32_t c0, uint32_t c1, uint32_t c2
                                            uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int round = 0; round < 10; ++round) {
-        const uint32_t lo0 = 0xD2511F53u * c0;
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        // one v_mad_u64_u32 per 32x32 -> 64-bit product gives both halves
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
         c0 = n0;
-        c1 = lo1;
+        c1 = (uint32_t)p1;
         c2 = n2;
-        c3 = lo0;
+        c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -184,6 +183,48 @@ __device__ __forceinline__ R mp_acklam(R p) {
 }
 
 // ---------------------------------------------------------------------------
+// FP32 inverse normal for the FP32 paths: Phi^-1(p) = sqrt(2) erfinv(2p - 1)
+// with Giles' single-precision erfinv ("Approximating the erfinv function",
+// GPU Computing Gems, 2011; max relative error ~1.2e-7).  Acklam's rational
+// form, which the FP64 paths keep because it is the reference's, loses up to
+// 1e-4 relative in FP32 to cancellation between its alternating coefficients,
+// and its tail branch (4.85% of draws) diverges in almost every wave; this
+// tail (w >= 5, 0.34% of draws) is rare.  For p in [0,1) on the 2^-23 grid
+// 4p(1-p) is formed exactly up to one rounding.  p == 0 returns the FP64
+// reference value at f64::MIN_POSITIVE.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float mp_inv_normal_f32(float p) {
+    if (p <= 0.0f) return -37.51937936814848f;
+    const float x = 2.0f * p - 1.0f;
+    float w = -0.69314718055994531f * __builtin_amdgcn_logf(4.0f * p * (1.0f - p));
+    float r;
+    if (w < 5.0f) {
+        w -= 2.5f;
+        r = 2.81022636e-08f;
+        r = 3.43273939e-07f + r * w;
+        r = -3.5233877e-06f + r * w;
+        r = -4.39150654e-06f + r * w;
+        r = 0.00021858087f + r * w;
+        r = -0.00125372503f + r * w;
+        r = -0.00417768164f + r * w;
+        r = 0.246640727f + r * w;
+        r = 1.50140941f + r * w;
+    } else {
+        w = __builtin_amdgcn_sqrtf(w) - 3.0f;
+        r = -0.000200214257f;
+        r = 0.000100950558f + r * w;
+        r = 0.00134934322f + r * w;
+        r = -0.00367342844f + r * w;
+        r = 0.00573950773f + r * w;
+        r = -0.0076224613f + r * w;
+        r = 0.00943887047f + r * w;
+        r = 1.00167406f + r * w;
+        r = 2.83297682f + r * w;
+    }
+    return 1.41421356237309505f * r * x;
+}
+
+// ---------------------------------------------------------------------------
 // Owen-scrambled Sobol (hash-based nested uniform scrambling, Burley 2020).
 // lk_scramble lets lower bits influence only higher bits; applied to
 // bit-reversed values it is a nested uniform scramble of the digits.
@@ -205,6 +246,18 @@ __device__ __host__ __forceinline__ uint32_t mp_lk(uint32_t n, uint32_t hs) {
     n *= (hs >> 16) | 1u;
     n ^= n * 0x05526c56u;
     n ^= n * 0x53a22864u;
+    return n;
+}
+
+// The same permutation, exact in bits 0..23 only, with full-rate 24-bit
+// multiplies: every step maps bits 0..23 to bits 0..23 (products and sums
+// carry upward), and a uniform needs only 23 of those bits.
+__device__ __forceinline__ uint32_t mp_lk_low24(uint32_t n, uint32_t hs) {
+    n ^= __umul24(n, 0x3d20adeau);
+    n += hs;
+    n = __umul24(n, (hs >> 16) | 1u);
+    n ^= __umul24(n, 0x05526c56u);
+    n ^= __umul24(n, 0x53a22864u);
     return n;
 }
 

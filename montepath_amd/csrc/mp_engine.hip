@@ -413,10 +413,22 @@ struct sobol_cursor {
         const uint32_t hs = mp_hash32(dd ^ (bseed + (uint32_t)o));
         return mp_rev32(mp_lk(mp_rev32(x), hs));
     }
+    // Standard normal of dimension g: the top 23 bits of the scrambled point
+    // (computed with the 24-bit-exact permutation), then the inverse normal
+    // of the path precision.
     __device__ __forceinline__ R normal(int g, const uint32_t* nib) {
-        const uint32_t x = raw(g, nib);
-        const R u = R(x >> 9) * R(0x1p-23);
-        return mp_acklam<R>(u);
+        const int o = g >> 8;
+        if (o != off) set_offset(o);
+        const uint32_t dd = (uint32_t)(g & 255);
+        const uint32_t* t = nib + dd * 64u;
+        const uint32_t x = t[n0] ^ t[16 + n1] ^ t[32 + n2] ^ t[48 + n3];
+        const uint32_t hs = mp_hash32(dd ^ (bseed + (uint32_t)o));
+        const uint32_t top23 = mp_rev32(mp_lk_low24(mp_rev32(x), hs)) >> 9;
+        const R u = R(top23) * R(0x1p-23);
+        if constexpr (sizeof(R) == 4)
+            return mp_inv_normal_f32(u);
+        else
+            return mp_acklam<R>(u);
     }
 };
 
