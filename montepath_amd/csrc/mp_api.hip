@@ -134,7 +134,7 @@ struct mp_ctx {
     dbuf reqs, tile_req, order, partial, out, counter;
     dbuf normals, uniforms, jumps, term, pay;
     dbuf plan, basket_par;
-    dbuf lsm_paths, lsm_cash, lsm_ex, lsm_coeff, lsm_valid, lsm_gram, lsm_disc, lsm_grow,
+    dbuf lsm_paths, lsm_cash, lsm_ex, lsm_coeff, lsm_valid, lsm_gram, lsm_par, lsm_cnt,
         lsm_part, lsm_out, plain_pay;
     dbuf draws;
 };
@@ -815,6 +815,7 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     p.n_steps = m;
     p.degree = P.degree;
     p.coeffs = P.degree + 1;
+    p.allow_intrinsic = P.allow_intrinsic;
     p.n_paths = n;
     p.s0 = r.s0;
     p.log_s0 = std::log(r.s0);
@@ -829,57 +830,63 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     p.key0 = (uint32_t)r.seed;
     p.key1 = (uint32_t)r.contract_id;
     p.seed_hi = (uint32_t)(r.seed >> 32);
-    std::vector<double> disc_pow(m + 1), growth(m + 1);
+    // parameters followed by disc_step^s and disc_step^-s, one upload
+    const size_t blob_bytes = sizeof(mp_lsm_params) + 2 * (size_t)(m + 1) * 8;
+    std::vector<unsigned char> blob(blob_bytes);
+    std::memcpy(blob.data(), &p, sizeof(p));
+    double* disc_pow = reinterpret_cast<double*>(blob.data() + sizeof(mp_lsm_params));
+    double* growth = disc_pow + (m + 1);
     for (int s = 0; s <= m; ++s) {
         disc_pow[s] = powi_ref(p.disc_step, s);
         growth[s] = powi_ref(p.disc_step, -s);
     }
     const size_t rb = precision == 64 ? 8 : 4;
     const int block = 256;
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + block - 1) / block, (uint64_t)c->n_cu * 8));
+    const int grid = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>({(n + block - 1) / block, (uint64_t)c->n_cu * 2, (uint64_t)MP_LSM_MAX_BLOCKS}));
     ENSURE(c->lsm_paths, (size_t)(m - 1) * n * rb);
     ENSURE(c->lsm_cash, n * 8);
     ENSURE(c->lsm_ex, n * 2);
     ENSURE(c->lsm_coeff, (size_t)(m + 1) * 4 * 8);
     ENSURE(c->lsm_valid, (size_t)(m + 1) * 4);
     ENSURE(c->lsm_gram, (size_t)grid * 16 * 8);
-    ENSURE(c->lsm_disc, (size_t)(m + 1) * 8);
-    ENSURE(c->lsm_grow, (size_t)(m + 1) * 8);
+    ENSURE(c->lsm_par, blob_bytes);
+    ENSURE(c->lsm_cnt, (size_t)(m + 2) * 4);
     ENSURE(c->lsm_part, (size_t)grid * 4 * 8);
     ENSURE(c->lsm_out, 8 * 8);
     if (normals_host) ENSURE(c->normals, (size_t)m * n * 8);
-    HIPCHK(hipMemcpyAsync(c->lsm_disc.p, disc_pow.data(), (m + 1) * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->lsm_grow.p, growth.data(), (m + 1) * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemsetAsync(c->lsm_valid.p, 0, (size_t)(m + 1) * 4, c->stream));
-    HIPCHK(hipMemsetAsync(c->lsm_coeff.p, 0, (size_t)(m + 1) * 4 * 8, c->stream));
+    HIPCHK(hipMemcpyAsync(c->lsm_par.p, blob.data(), blob_bytes, hipMemcpyHostToDevice, c->stream));
     if (normals_host)
         HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, (size_t)m * n * 8, hipMemcpyHostToDevice, c->stream));
+    const mp_lsm_params* dp = c->lsm_par.as<mp_lsm_params>();
+    unsigned int* cnt = c->lsm_cnt.as<unsigned int>();
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(mp::launch_lsm_paths(p, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
+    HIPCHK(mp::launch_lsm_fill(cnt, m + 2, c->lsm_valid.as<int>(), c->lsm_coeff.as<double>(), m + 1,
+                               c->stream));
+    HIPCHK(mp::launch_lsm_paths(dp, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
                                 c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), precision, grid,
                                 block, c->stream));
-    // regression dates: schedule without its last element, latest first
+    // regression dates: schedule without its last element, latest first; each
+    // pass also applies the exercise rule of the date regressed before it
     const int ns = (int)P.schedule.size();
-    int prev = 0;
+    int prev = 0, launches = 3;
     for (int i = ns - 2; i >= 0; --i) {
         const int date = (int)P.schedule[i];
-        HIPCHK(mp::launch_lsm_step(p, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
+        HIPCHK(mp::launch_lsm_step(dp, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
                                    c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, date,
-                                   c->lsm_disc.as<double>(), c->lsm_grow.as<double>(),
-                                   c->lsm_gram.as<double>(), precision, grid, block, c->stream));
-        HIPCHK(mp::launch_lsm_solve(p, c->lsm_gram.as<double>(), grid, date, c->lsm_coeff.as<double>(),
-                                    c->lsm_valid.as<int>(), c->stream));
+                                   c->lsm_gram.as<double>(), cnt, precision, grid, block, c->stream));
         prev = date;
+        ++launches;
     }
-    if (prev >= 1)
-        HIPCHK(mp::launch_lsm_step(p, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
+    if (prev >= 1) {
+        HIPCHK(mp::launch_lsm_step(dp, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
                                    c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, 0,
-                                   c->lsm_disc.as<double>(), c->lsm_grow.as<double>(),
-                                   c->lsm_gram.as<double>(), precision, grid, block, c->stream));
-    HIPCHK(mp::launch_lsm_final(p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
-                                c->lsm_part.as<double>(), grid, block, c->stream));
-    HIPCHK(mp::launch_lsm_reduce(p, c->lsm_part.as<double>(), grid, c->lsm_valid.as<int>(),
-                                 P.allow_intrinsic, c->lsm_out.as<double>(), c->stream));
+                                   c->lsm_gram.as<double>(), cnt, precision, grid, block, c->stream));
+        ++launches;
+    }
+    HIPCHK(mp::launch_lsm_final(dp, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), c->lsm_valid.as<int>(),
+                                c->lsm_part.as<double>(), cnt + (m + 1), c->lsm_out.as<double>(), grid, block,
+                                c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     double res[8];
     HIPCHK(hipMemcpyAsync(res, c->lsm_out.p, 5 * 8, hipMemcpyDeviceToHost, c->stream));
@@ -898,7 +905,7 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     c->last.main_kernel_ms = ms;
     c->last.finalize_ms = 0.0;
-    c->last.main_launches = (uint64_t)(2 * std::max(ns - 1, 0) + 3);
+    c->last.main_launches = (uint64_t)launches;
     if (coeff_host)
         for (int s = 0; s < m; ++s)
             for (int i = 0; i < 4; ++i) coeff_host[s * 4 + i] = coeff[(size_t)s * 4 + i];

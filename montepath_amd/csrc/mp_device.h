@@ -267,3 +267,32 @@ __device__ __forceinline__ uint32_t mp_rev32(uint32_t x) { return __builtin_bitr
 __device__ __forceinline__ uint32_t mp_sobol_shuffle(uint32_t local, uint32_t hashed_seed) {
     return mp_rev32(mp_lk(mp_rev32(local & 0xFFFFu), hashed_seed)) & 0xFFFFu;
 }
+
+// Cross-block publication without agent-scope fences.  On gfx950 each XCD
+// has its own L2 and an agent-scope fence writes back and invalidates it,
+// which costs microseconds per wave.  Values meant for another block are
+// instead stored and loaded as relaxed agent-scope atomics (coherent across
+// XCDs, not held in a non-coherent L2), and only their order relative to
+// the signalling atomic is enforced: a workgroup fence plus a wait for all
+// outstanding vector-memory operations of the wave.
+__device__ __forceinline__ void mp_store_coherent(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double mp_load_coherent(const double* p) {
+    return __longlong_as_double((long long)__hip_atomic_load(
+        reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// s_waitcnt immediate for gfx9: vmcnt = 0 (bits 3:0 and 15:14), expcnt and
+// lgkmcnt at their maxima (no wait).
+__device__ __forceinline__ void mp_vmem_drain_release() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
+}
+
+__device__ __forceinline__ void mp_vmem_drain_acquire() {
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}

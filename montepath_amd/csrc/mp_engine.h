@@ -7,6 +7,9 @@
 #define MP_NS_MAX 4            // strikes priced from one simulation
 #define MP_POISSON_TERMS 24    // cached cumulative Poisson probabilities per request
 #define MP_PARTIAL_STRIDE (2 * MP_NS_MAX)  // doubles per tile partial: (sum, sumsq) per strike
+// LSM kernels run at most this many blocks: the last block reads every
+// block's partial sums in one round trip (two rows per thread).
+#define MP_LSM_MAX_BLOCKS 512
 
 enum mp_model { MP_MODEL_GBM = 0, MP_MODEL_HESTON = 1, MP_MODEL_MERTON = 2 };
 enum mp_payoff { MP_PAY_EUROPEAN = 0, MP_PAY_ASIAN = 1, MP_PAY_DOWN_AND_OUT = 2, MP_PAY_LOOKBACK = 3 };
@@ -57,11 +60,13 @@ struct mp_qmc_params {
     double inv_sqrt_dt;
 };
 
-struct mp_lsm_params {
+// Followed in device memory by disc_pow[n_steps+1] (disc_step^s) and
+// growth[n_steps+1] (disc_step^-s); the struct is padded to 128 bytes.
+struct alignas(128) mp_lsm_params {
     int32_t n_steps;
     int32_t degree;
     int32_t coeffs;
-    int32_t pad;
+    int32_t allow_intrinsic;
     uint64_t n_paths;
     double log_s0, s0, k, inv_k, drift_dt, vol_dt, disc_step, disc_m;
     uint32_t key0, key1, seed_hi, pad2;
@@ -101,21 +106,18 @@ hipError_t launch_basket(const mp_dreq* req, int n_assets, const double* chol,
                          const double* weight, const double* normals, double* partial,
                          double* terminal_out, double* payoff_out, int precision, int grid,
                          int block, hipStream_t s);
-hipError_t launch_lsm_paths(const mp_lsm_params& p, const double* normals, void* paths,
+hipError_t launch_lsm_fill(unsigned int* counters, int n_counters, int* valid, double* coeff,
+                           int n_dates, hipStream_t s);
+hipError_t launch_lsm_paths(const mp_lsm_params* p, const double* normals, void* paths,
                             double* cash, uint16_t* ex, int precision, int grid, int block,
                             hipStream_t s);
-hipError_t launch_lsm_step(const mp_lsm_params& p, const void* paths, double* cash,
-                           uint16_t* ex, const double* coeff_all, const int* valid_all,
-                           int step_update, int step_gram, const double* disc_pow,
-                           const double* growth, double* gram_partial, int precision, int grid,
-                           int block, hipStream_t s);
-hipError_t launch_lsm_solve(const mp_lsm_params& p, const double* gram_partial, int n_blocks,
-                            int step, double* coeff_all, int* valid_all, hipStream_t s);
-hipError_t launch_lsm_final(const mp_lsm_params& p, const double* cash, const uint16_t* ex,
-                            double* partial, int grid, int block, hipStream_t s);
-hipError_t launch_lsm_reduce(const mp_lsm_params& p, const double* partial, int n_blocks,
-                             const int* valid_all, int allow_intrinsic, double* out,
-                             hipStream_t s);
+hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* cash,
+                           uint16_t* ex, double* coeff_all, int* valid_all, int step_update,
+                           int step_gram, double* gram_partial, unsigned int* counters,
+                           int precision, int grid, int block, hipStream_t s);
+hipError_t launch_lsm_final(const mp_lsm_params* p, const double* cash, const uint16_t* ex,
+                            const int* valid_all, double* partial, unsigned int* counter,
+                            double* out, int grid, int block, hipStream_t s);
 hipError_t launch_plain_fp64(const mp_dreq* req, double* payoffs, int grid, int block,
                              hipStream_t s);
 hipError_t launch_plain_reduce(const double* payoffs, uint64_t n, double* partial,

@@ -13,6 +13,8 @@
 //   * path arithmetic in R = float or double; normals from Philox4x32-10 in
 //     registers, from a caller buffer (parity), or from Owen-scrambled Sobol
 //     with a streaming Brownian bridge (LDS).
+#include <algorithm>
+
 #include "mp_device.h"
 #include "mp_engine.h"
 
@@ -621,13 +623,24 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
 // ---------------------------------------------------------------------------
 // Longstaff-Schwartz put.  Paths stored step-major [step][path] (steps
 // 1..M-1), cash flows kept discounted to t0 in FP64, exercise step per path
-// in 16 bits.
+// in 16 bits.  Parameters live in device memory so the date loop can be
+// replayed as a graph; per-date reductions finish in the last block to
+// arrive (one atomic ticket per block, fixed-order sums, no waiting).
 // ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void lsm_fill_kernel(unsigned int* counters, int n_counters,
+                                                      int* valid, double* coeff, int n_dates) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_counters) counters[i] = 0u;
+    if (i < n_dates) valid[i] = 0;
+    if (i < 4 * n_dates) coeff[i] = 0.0;
+}
+
 template <class R, int SRC>
-__global__ __launch_bounds__(256) void lsm_paths_kernel(mp_lsm_params P,
+__global__ __launch_bounds__(256) void lsm_paths_kernel(const mp_lsm_params* __restrict__ Pp,
                                                        const double* __restrict__ normals,
                                                        R* __restrict__ paths, double* cash,
                                                        uint16_t* ex) {
+    const mp_lsm_params P = *Pp;
     const uint64_t n = P.n_paths;
     const R drift = R(P.drift_dt), vol = R(P.vol_dt), s0 = R(P.s0);
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
@@ -671,22 +684,145 @@ __device__ __forceinline__ void laguerre(double x, int degree, double b[4]) {
     b[3] = degree >= 3 ? 1.0 - 3.0 * x + 1.5 * x2 - x3 / 6.0 : 0.0;
 }
 
+// Gauss-Jordan with partial pivoting, failing when the best pivot <= 1e-12
+// (solve_lsm_normal_equations cpu.rs:7684-7736).  Single lane; every index
+// is static so the system stays in registers.  Rows and columns at or beyond
+// dim are zero (their basis functions are zero), so carrying them through
+// the eliminations changes nothing.
+__device__ bool lsm_solve4(double (&m)[4][4], double (&rhs)[4], int dim) {
+#pragma unroll
+    for (int piv = 0; piv < 4; ++piv) {
+        if (piv >= dim) break;
+        int best = piv;
+        double best_abs = fabs(m[piv][piv]);
+#pragma unroll
+        for (int row = piv + 1; row < 4; ++row) {
+            const double c = fabs(m[row][piv]);
+            if (row < dim && c > best_abs) {
+                best = row;
+                best_abs = c;
+            }
+        }
+        if (best_abs <= 1e-12) return false;
+#pragma unroll
+        for (int row = piv + 1; row < 4; ++row) {
+            if (row == best) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double t = m[piv][j];
+                    m[piv][j] = m[row][j];
+                    m[row][j] = t;
+                }
+                const double t = rhs[piv];
+                rhs[piv] = rhs[row];
+                rhs[row] = t;
+            }
+        }
+        const double pivot = m[piv][piv];
+#pragma unroll
+        for (int c = piv; c < 4; ++c) m[piv][c] /= pivot;
+        rhs[piv] /= pivot;
+#pragma unroll
+        for (int row = 0; row < 4; ++row) {
+            if (row == piv) continue;
+            const double f = m[row][piv];
+            if (f == 0.0) continue;
+#pragma unroll
+            for (int c = piv; c < 4; ++c) m[row][c] -= f * m[piv][c];
+            rhs[row] -= f * rhs[piv];
+        }
+    }
+    return true;
+}
+
+// Block sum of NV values published for other blocks (coherent stores),
+// then a ticket; true in the block that arrives last, which may then read
+// every block's partials with mp_load_coherent.
+template <int NV>
+__device__ __forceinline__ bool publish_and_ticket(double (&v)[NV], double* dst, unsigned int* counter) {
+    __shared__ double red[16][NV];
+    __shared__ bool is_last;
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = mp_wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[wave][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double acc = 0.0;
+        for (int w = 0; w < nw; ++w) acc += red[w][threadIdx.x];
+        mp_store_coherent(dst + threadIdx.x, acc);
+    }
+    // the stores above are all in wave 0; drain them before the ticket
+    if (threadIdx.x < MP_WAVE) mp_vmem_drain_release();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        is_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                  gridDim.x - 1;
+    __syncthreads();
+    if (is_last) mp_vmem_drain_acquire();
+    return is_last;
+}
+
+// Fixed-order FP64 reduction of NV values over nb <= MP_LSM_MAX_BLOCKS block
+// partials laid out [block][stride], 256 threads: thread t loads rows t and
+// t + 256 in one round trip and adds them; 16 threads per value then sum 16
+// of those in order, and one thread per value sums the 16 results.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* partial, int nb, int stride,
+                                                double* tot /* shared, NV */) {
+    __shared__ double row[256][NV + 1];
+    __shared__ double red[NV][17];
+    const int t = threadIdx.x;
+    double a[NV], b[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        a[i] = t < nb ? mp_load_coherent(partial + (uint64_t)t * stride + i) : 0.0;
+        b[i] = t + 256 < nb ? mp_load_coherent(partial + (uint64_t)(t + 256) * stride + i) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) row[t][i] = a[i] + b[i];
+    __syncthreads();
+    const int vi = t >> 4, j = t & 15;
+    if (vi < NV) {
+        mp_neumaier acc;
+        acc.init();
+        for (int q = 0; q < 16; ++q) acc.add(row[j * 16 + q][vi]);
+        red[vi][j] = acc.value();
+    }
+    __syncthreads();
+    if (t < NV) {
+        mp_neumaier s;
+        s.init();
+        for (int q = 0; q < 16; ++q) s.add(red[t][q]);
+        tot[t] = s.value();
+    }
+    __syncthreads();
+}
+
 // One pass per exercise date: apply the exercise rule of date step_update
 // (if >= 1) and accumulate the Gram matrix / right-hand side of date
-// step_gram (if >= 1) over in-the-money paths.  cash holds cash flows
-// discounted to t0; growth[s] = disc_step^-s brings them to date s.
+// step_gram (if >= 1) over in-the-money paths; the last block to finish
+// reduces all blocks' sums in fixed order and solves for the coefficients.
+// cash holds cash flows discounted to t0; growth[s] = disc_step^-s.
 template <class R>
-__global__ __launch_bounds__(256) void lsm_step_kernel(mp_lsm_params P, const R* __restrict__ paths,
+__global__ __launch_bounds__(256) void lsm_step_kernel(const mp_lsm_params* __restrict__ Pp,
+                                                      const R* __restrict__ paths,
                                                       double* cash, uint16_t* ex,
-                                                      const double* __restrict__ coeff_all,
-                                                      const int* __restrict__ valid_all,
+                                                      double* coeff_all, int* valid_all,
                                                       int step_update, int step_gram,
-                                                      const double* __restrict__ disc_pow,
-                                                      const double* __restrict__ growth,
-                                                      double* gram_partial) {
+                                                      double* gram_partial,
+                                                      unsigned int* counters) {
+    const mp_lsm_params& P = *Pp;
     const uint64_t n = P.n_paths;
     const double k = P.k, inv_k = P.inv_k;
     const int deg = P.degree, nc = P.coeffs;
+    const double* disc_pow = reinterpret_cast<const double*>(Pp + 1);
+    const double* growth = disc_pow + (P.n_steps + 1);
     const bool do_update = step_update >= 1 && valid_all[step_update] != 0;
     double beta[4] = {0.0, 0.0, 0.0, 0.0};
     if (do_update) {
@@ -699,116 +835,59 @@ __global__ __launch_bounds__(256) void lsm_step_kernel(mp_lsm_params P, const R*
     double acc[15];
 #pragma unroll
     for (int i = 0; i < 15; ++i) acc[i] = 0.0;
-    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-         p += (uint64_t)gridDim.x * blockDim.x) {
-        double c = cash[p];
-        if (do_update) {
-            const double s = (double)paths[(uint64_t)(step_update - 1) * n + p];
-            const double imm = k - s > 0.0 ? k - s : 0.0;
-            if (imm > 0.0) {
-                double b[4];
-                laguerre(s * inv_k, deg, b);
+    // four paths per thread per trip, all loads issued before any use
+    const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+    const R* pu = paths + (uint64_t)(step_update >= 1 ? step_update - 1 : 0) * n;
+    const R* pg = paths + (uint64_t)(step_gram >= 1 ? step_gram - 1 : 0) * n;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; base < n; base += 4 * T) {
+        double c[4], su[4], sg[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t p = base + u * T;
+            const bool in = p < n;
+            c[u] = in ? cash[p] : 0.0;
+            su[u] = in && do_update ? (double)pu[p] : k;
+            sg[u] = in && step_gram >= 1 ? (double)pg[p] : k;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t p = base + u * T;
+            const double imm_u = k - su[u] > 0.0 ? k - su[u] : 0.0;
+            if (imm_u > 0.0) {  // false when not updating or p >= n (su = k)
+                double bb[4];
+                laguerre(su[u] * inv_k, deg, bb);
+                // beta[i] = 0 for i >= nc, so the full sum equals the sum over nc
                 double cont = 0.0;
-                for (int i = 0; i < nc; ++i) cont += beta[i] * b[i];
-                if (imm > (cont > 0.0 ? cont : 0.0)) {
-                    c = imm * dpow_u;
-                    cash[p] = c;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cont += beta[i] * bb[i];
+                if (imm_u > (cont > 0.0 ? cont : 0.0)) {
+                    c[u] = imm_u * dpow_u;
+                    cash[p] = c[u];
                     ex[p] = (uint16_t)step_update;
                 }
             }
-        }
-        if (step_gram >= 1) {
-            const double s = (double)paths[(uint64_t)(step_gram - 1) * n + p];
-            const double imm = k - s > 0.0 ? k - s : 0.0;
-            if (imm > 0.0) {
-                double b[4];
-                laguerre(s * inv_k, deg, b);
-                const double cont = c * grow_g;
+            const double imm_g = k - sg[u] > 0.0 ? k - sg[u] : 0.0;
+            if (imm_g > 0.0) {  // false when not accumulating or p >= n
+                double bb[4];
+                laguerre(sg[u] * inv_k, deg, bb);
+                const double cont = c[u] * grow_g;
                 int e = 0;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = i; j < 4; ++j) acc[e++] += b[i] * b[j];
+                    for (int j = i; j < 4; ++j) acc[e++] += bb[i] * bb[j];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[10 + i] += b[i] * cont;
+                for (int i = 0; i < 4; ++i) acc[10 + i] += bb[i] * cont;
                 acc[14] += 1.0;
             }
         }
     }
-    if (step_gram >= 1) {
-        // block reduce 15 doubles
-        __shared__ double red[4][15];
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-        for (int i = 0; i < 15; ++i) acc[i] = mp_wave_sum(acc[i]);
-        if (lane == 0)
-            for (int i = 0; i < 15; ++i) red[wave][i] = acc[i];
-        __syncthreads();
-        if (threadIdx.x < 15) {
-            double t = 0.0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w][threadIdx.x];
-            gram_partial[(uint64_t)blockIdx.x * 16 + threadIdx.x] = t;
-        }
-    }
-}
-
-// Gauss-Jordan with partial pivoting, failing when the best pivot <= 1e-12
-// (solve_lsm_normal_equations cpu.rs:7684-7736).  Single lane.
-__device__ bool lsm_solve4(double m[4][4], double rhs[4], int dim) {
-    for (int piv = 0; piv < dim; ++piv) {
-        int best = piv;
-        double best_abs = fabs(m[piv][piv]);
-        for (int row = piv + 1; row < dim; ++row) {
-            const double c = fabs(m[row][piv]);
-            if (c > best_abs) {
-                best = row;
-                best_abs = c;
-            }
-        }
-        if (best_abs <= 1e-12) return false;
-        if (best != piv) {
-            for (int j = 0; j < 4; ++j) {
-                const double t = m[piv][j];
-                m[piv][j] = m[best][j];
-                m[best][j] = t;
-            }
-            const double t = rhs[piv];
-            rhs[piv] = rhs[best];
-            rhs[best] = t;
-        }
-        const double pivot = m[piv][piv];
-        for (int c = piv; c < dim; ++c) m[piv][c] /= pivot;
-        rhs[piv] /= pivot;
-        for (int row = 0; row < dim; ++row) {
-            if (row == piv) continue;
-            const double f = m[row][piv];
-            if (f == 0.0) continue;
-            for (int c = piv; c < dim; ++c) m[row][c] -= f * m[piv][c];
-            rhs[row] -= f * rhs[piv];
-        }
-    }
-    return true;
-}
-
-__global__ __launch_bounds__(64) void lsm_solve_kernel(mp_lsm_params P, const double* gram_partial,
-                                                      int n_blocks, int step, double* coeff_all,
-                                                      int* valid_all) {
-    __shared__ double red[15][MP_WAVE];
-    const int lane = threadIdx.x;
-    mp_neumaier acc[15];
-    for (int i = 0; i < 15; ++i) acc[i].init();
-    for (int b = lane; b < n_blocks; b += MP_WAVE)
-        for (int i = 0; i < 15; ++i) acc[i].add(gram_partial[(uint64_t)b * 16 + i]);
-    for (int i = 0; i < 15; ++i) red[i][lane] = acc[i].value();
-    __syncthreads();
-    if (lane == 0) {
-        double tot[15];
-        for (int i = 0; i < 15; ++i) {
-            mp_neumaier s;
-            s.init();
-            for (int l = 0; l < MP_WAVE; ++l) s.add(red[i][l]);
-            tot[i] = s.value();
-        }
+    if (step_gram < 1) return;
+    if (!publish_and_ticket<15>(acc, gram_partial + (uint64_t)blockIdx.x * 16, &counters[step_gram]))
+        return;
+    __shared__ double tot[15];
+    reduce_partials<15>(gram_partial, gridDim.x, 16, tot);
+    if (threadIdx.x == 0) {
         double m[4][4], rhs[4];
         int e = 0;
         for (int i = 0; i < 4; ++i)
@@ -818,17 +897,21 @@ __global__ __launch_bounds__(64) void lsm_solve_kernel(mp_lsm_params P, const do
                 ++e;
             }
         for (int i = 0; i < 4; ++i) rhs[i] = tot[10 + i];
-        const double itm = tot[14];
         int ok = 0;
-        if (itm >= (double)P.coeffs) ok = lsm_solve4(m, rhs, P.coeffs) ? 1 : 0;
-        for (int i = 0; i < 4; ++i) coeff_all[step * 4 + i] = (ok && i < P.coeffs) ? rhs[i] : 0.0;
-        valid_all[step] = ok;
+        if (tot[14] >= (double)nc) ok = lsm_solve4(m, rhs, nc) ? 1 : 0;
+        for (int i = 0; i < 4; ++i) coeff_all[step_gram * 4 + i] = (ok && i < nc) ? rhs[i] : 0.0;
+        valid_all[step_gram] = ok;
+        counters[step_gram] = 0u;  // ready for the next pricing
     }
 }
 
-// Sums of discounted cash flows and exercise counts.
-__global__ __launch_bounds__(256) void lsm_final_kernel(mp_lsm_params P, const double* cash,
-                                                       const uint16_t* ex, double* partial) {
+// Sums of discounted cash flows and exercise counts; the last block writes
+// price, standard error, counts and the number of regression dates.
+__global__ __launch_bounds__(256) void lsm_final_kernel(const mp_lsm_params* __restrict__ Pp,
+                                                       const double* cash, const uint16_t* ex,
+                                                       const int* valid_all, double* partial,
+                                                       unsigned int* counter, double* out) {
+    const mp_lsm_params& P = *Pp;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P.n_paths;
          p += (uint64_t)gridDim.x * blockDim.x) {
@@ -841,39 +924,17 @@ __global__ __launch_bounds__(256) void lsm_final_kernel(mp_lsm_params P, const d
         else if (c > 0.0)
             v[3] += 1.0;
     }
-    block_sum_store<4>(v, partial + (uint64_t)blockIdx.x * 4);
-}
-
-__global__ __launch_bounds__(64) void lsm_reduce_kernel(mp_lsm_params P, const double* partial,
-                                                       int n_blocks, const int* valid_all,
-                                                       int allow_intrinsic, double* out) {
-    __shared__ double red[4][MP_WAVE];
-    __shared__ int regs[MP_WAVE];
-    const int lane = threadIdx.x;
-    mp_neumaier a[4];
-    for (int i = 0; i < 4; ++i) a[i].init();
-    for (int b = lane; b < n_blocks; b += MP_WAVE)
-        for (int i = 0; i < 4; ++i) a[i].add(partial[(uint64_t)b * 4 + i]);
-    for (int i = 0; i < 4; ++i) red[i][lane] = a[i].value();
-    int cnt = 0;
-    for (int s = lane; s <= P.n_steps; s += MP_WAVE) cnt += valid_all[s] ? 1 : 0;
-    regs[lane] = cnt;
-    __syncthreads();
-    if (lane == 0) {
-        double t[4];
-        for (int i = 0; i < 4; ++i) {
-            mp_neumaier s;
-            s.init();
-            for (int l = 0; l < MP_WAVE; ++l) s.add(red[i][l]);
-            t[i] = s.value();
-        }
+    if (!publish_and_ticket<4>(v, partial + (uint64_t)blockIdx.x * 4, counter)) return;
+    __shared__ double tot[4];
+    reduce_partials<4>(partial, gridDim.x, 4, tot);
+    if (threadIdx.x == 0) {
         int rs = 0;
-        for (int l = 0; l < MP_WAVE; ++l) rs += regs[l];
+        for (int s = 1; s <= P.n_steps; ++s) rs += valid_all[s] ? 1 : 0;
         const double n = (double)P.n_paths;
-        const double mean = t[0] / n;
-        const double var = t[1] / n - mean * mean;
+        const double mean = tot[0] / n;
+        const double var = tot[1] / n - mean * mean;
         double price = mean, se = sqrt(var > 0.0 ? var : 0.0) / sqrt(n);
-        if (allow_intrinsic) {
+        if (P.allow_intrinsic) {
             const double intrinsic = P.k - P.s0 > 0.0 ? P.k - P.s0 : 0.0;
             if (intrinsic > price) {
                 price = intrinsic;
@@ -882,9 +943,10 @@ __global__ __launch_bounds__(64) void lsm_reduce_kernel(mp_lsm_params P, const d
         }
         out[0] = price;
         out[1] = se;
-        out[2] = t[2];
-        out[3] = t[3];
+        out[2] = tot[2];
+        out[3] = tot[3];
         out[4] = (double)rs;
+        *counter = 0u;
     }
 }
 
@@ -1101,7 +1163,15 @@ hipError_t launch_basket(const mp_dreq* req, int n_assets, const double* chol,
                            terminal_out, payoff_out, grid, block, s);
 }
 
-hipError_t launch_lsm_paths(const mp_lsm_params& p, const double* normals, void* paths,
+hipError_t launch_lsm_fill(unsigned int* counters, int n_counters, int* valid, double* coeff,
+                           int n_dates, hipStream_t s) {
+    const int n = std::max(n_counters, 4 * n_dates);
+    hipLaunchKernelGGL(lsm_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters,
+                       n_counters, valid, coeff, n_dates);
+    return hipGetLastError();
+}
+
+hipError_t launch_lsm_paths(const mp_lsm_params* p, const double* normals, void* paths,
                             double* cash, uint16_t* ex, int precision, int grid, int block,
                             hipStream_t s) {
     if (precision == 64) {
@@ -1122,40 +1192,28 @@ hipError_t launch_lsm_paths(const mp_lsm_params& p, const double* normals, void*
     return hipGetLastError();
 }
 
-hipError_t launch_lsm_step(const mp_lsm_params& p, const void* paths, double* cash,
-                           uint16_t* ex, const double* coeff_all, const int* valid_all,
-                           int step_update, int step_gram, const double* disc_pow,
-                           const double* growth, double* gram_partial, int precision, int grid,
-                           int block, hipStream_t s) {
+hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* cash,
+                           uint16_t* ex, double* coeff_all, int* valid_all, int step_update,
+                           int step_gram, double* gram_partial, unsigned int* counters,
+                           int precision, int grid, int block, hipStream_t s) {
+    if (grid < 1 || grid > MP_LSM_MAX_BLOCKS || block != 256) return hipErrorInvalidValue;
     if (precision == 64)
         hipLaunchKernelGGL((lsm_step_kernel<double>), dim3(grid), dim3(block), 0, s, p,
                            (const double*)paths, cash, ex, coeff_all, valid_all, step_update,
-                           step_gram, disc_pow, growth, gram_partial);
+                           step_gram, gram_partial, counters);
     else
         hipLaunchKernelGGL((lsm_step_kernel<float>), dim3(grid), dim3(block), 0, s, p,
                            (const float*)paths, cash, ex, coeff_all, valid_all, step_update,
-                           step_gram, disc_pow, growth, gram_partial);
+                           step_gram, gram_partial, counters);
     return hipGetLastError();
 }
 
-hipError_t launch_lsm_solve(const mp_lsm_params& p, const double* gram_partial, int n_blocks,
-                            int step, double* coeff_all, int* valid_all, hipStream_t s) {
-    hipLaunchKernelGGL(lsm_solve_kernel, dim3(1), dim3(64), 0, s, p, gram_partial, n_blocks, step,
-                       coeff_all, valid_all);
-    return hipGetLastError();
-}
-
-hipError_t launch_lsm_final(const mp_lsm_params& p, const double* cash, const uint16_t* ex,
-                            double* partial, int grid, int block, hipStream_t s) {
-    hipLaunchKernelGGL(lsm_final_kernel, dim3(grid), dim3(block), 0, s, p, cash, ex, partial);
-    return hipGetLastError();
-}
-
-hipError_t launch_lsm_reduce(const mp_lsm_params& p, const double* partial, int n_blocks,
-                             const int* valid_all, int allow_intrinsic, double* out,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(lsm_reduce_kernel, dim3(1), dim3(64), 0, s, p, partial, n_blocks,
-                       valid_all, allow_intrinsic, out);
+hipError_t launch_lsm_final(const mp_lsm_params* p, const double* cash, const uint16_t* ex,
+                            const int* valid_all, double* partial, unsigned int* counter,
+                            double* out, int grid, int block, hipStream_t s) {
+    if (grid < 1 || grid > MP_LSM_MAX_BLOCKS || block != 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(lsm_final_kernel, dim3(grid), dim3(block), 0, s, p, cash, ex, valid_all,
+                       partial, counter, out);
     return hipGetLastError();
 }
 
