@@ -34,6 +34,7 @@ HESTON = dict(s0=100.0, v0=0.04, kappa=2.0, theta=0.04, xi=0.5, rho=-0.7, r=0.03
 FLOP_PER_PATH_STEP = 20 + 2 * (12 + 7)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = "qmc_kernel_heston_sobol_bb_2p22x256_traffic.json"
 
 
 def log(msg: str) -> None:
@@ -190,6 +191,30 @@ def bench_serve(eng, seconds: float, rate: float) -> dict:
             "path_steps_per_s": stats["path_steps_per_s"], "launches": stats["launches"]}
 
 
+def copy_probe(nbytes: int) -> int:
+    import torch
+
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    src.fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}), flush=True)
+    return 0
+
+
+def measured_traffic() -> dict | None:
+    """HBM bytes per launch of the primary kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", TRAFFIC_FILE)
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,7 +224,12 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip batch and serving")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run three device-to-device copies of this many MiB (the "
+                         "known-size kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        return copy_probe(args.copy_probe_mib << 20)
     if args.gpus != 1:
         print("bench.py measures one MI355X: this engine runs on one GPU only "
               "(multi-GPU sharding by path range is a design note, not built)", file=sys.stderr)
@@ -230,6 +260,7 @@ def main() -> int:
     res64 = eng.price(heston_request(64))
     from oracle import closed_forms as cf
 
+    traffic = measured_traffic()
     cf_prices = [cf.heston_call_lewis(100.0, k, 0.03, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
                  for k in STRIKES]
     out = {
@@ -250,7 +281,9 @@ def main() -> int:
                    "strikes": STRIKES, "precision": "fp32_paths_fp64_kahan_sums",
                    **HESTON},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": f"profiles/{TRAFFIC_FILE}" if traffic else None,
                      "kernel": "qmc_kernel<float,2,true>", "kernel_ms": k_ms,
                      "flop_per_path_step": FLOP_PER_PATH_STEP,
                      "note": "FP32 vector (VALU) work; MI355X FP32 peak 157.3 TFLOP/s; MFMA unused"},

@@ -174,7 +174,8 @@ void sobol_nibbles(const uint32_t dirs[256][32], std::vector<uint32_t>& nib) {
                 uint32_t x = 0;
                 for (int b = 0; b < 4; ++b)
                     if ((n >> b) & 1) x ^= dirs[d][4 * j + b];
-                nib[d * 64 + j * 16 + n] = x;
+                // bit-reversed: the kernels scramble the reversed point
+                nib[d * 64 + j * 16 + n] = __builtin_bitreverse32(x);
             }
 }
 
@@ -198,9 +199,10 @@ int bridge_plan(uint64_t n_steps, double maturity, std::vector<mp_bbop>& ops, in
         const double tl = (double)it.l * dt, tm = (double)mid * dt, tr = (double)it.r * dt;
         const double denom = tr - tl;
         mp_bbop op;
-        op.lw = (tr - tm) / denom;
         op.rw = (tm - tl) / denom;
         op.sd = std::sqrt((tm - tl) * (tr - tm) / denom);
+        op.rwf = (float)op.rw;
+        op.sdf = (float)op.sd;
         op.slot = it.slot;
         const bool left_leaf = mid <= it.l + 1;
         const bool right_leaf = it.r <= mid + 1;

@@ -189,40 +189,51 @@ __device__ __forceinline__ R mp_acklam(R p) {
 // form, which the FP64 paths keep because it is the reference's, loses up to
 // 1e-4 relative in FP32 to cancellation between its alternating coefficients,
 // and its tail branch (4.85% of draws) diverges in almost every wave; this
-// tail (w >= 5, 0.34% of draws) is rare.  For p in [0,1) on the 2^-23 grid
-// 4p(1-p) is formed exactly up to one rounding.  p == 0 returns the FP64
-// reference value at f64::MIN_POSITIVE.
+// tail (w >= 5, 0.34% of draws) is rare.  p lies on the 2^-23 grid in
+// [0,1) and arrives as the bits of the float 1 + p; 4p(1-p) is formed
+// exactly up to one rounding.  p == 0 returns the FP64 reference value at
+// f64::MIN_POSITIVE.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float mp_inv_normal_f32(float p) {
-    if (p <= 0.0f) return -37.51937936814848f;
-    const float x = 2.0f * p - 1.0f;
-    float w = -0.69314718055994531f * __builtin_amdgcn_logf(4.0f * p * (1.0f - p));
+__device__ __forceinline__ float mp_inv_normal_f32(uint32_t fbits) {
+    // fbits = 0x3f800000 | t23 is the float 1 + p, p = t23 * 2^-23
+    const float f = __uint_as_float(fbits);
+    const float x = __builtin_fmaf(f, 2.0f, -3.0f);  // 2p - 1, exact
+    // 1 - x^2 = 4p(1-p) exactly; the FMA rounds it once.  l = log2 of it,
+    // w = -ln(4p(1-p)) = -ln2 * l.  The coefficients below are Giles'
+    // multiplied by sqrt(2), so that r * x is the normal itself.
+    const float l = __builtin_amdgcn_logf(__builtin_fmaf(-x, x, 1.0f));
     float r;
-    if (w < 5.0f) {
-        w -= 2.5f;
-        r = 2.81022636e-08f;
-        r = 3.43273939e-07f + r * w;
-        r = -3.5233877e-06f + r * w;
-        r = -4.39150654e-06f + r * w;
-        r = 0.00021858087f + r * w;
-        r = -0.00125372503f + r * w;
-        r = -0.00417768164f + r * w;
-        r = 0.246640727f + r * w;
-        r = 1.50140941f + r * w;
+    if (l > -7.2134752f) {  // w < 5
+        const float w = __builtin_fmaf(-0.69314718f, l, -2.5f);
+        r = 3.97426021e-08f;
+        r = 4.85462635e-07f + r * w;
+        r = -4.9828227e-06f + r * w;
+        r = -6.21052823e-06f + r * w;
+        r = 0.00030912002f + r * w;
+        r = -0.00177303492f + r * w;
+        r = -0.00590813393f + r * w;
+        r = 0.348802656f + r * w;
+        r = 2.12331367f + r * w;
     } else {
-        w = __builtin_amdgcn_sqrtf(w) - 3.0f;
-        r = -0.000200214257f;
-        r = 0.000100950558f + r * w;
-        r = 0.00134934322f + r * w;
-        r = -0.00367342844f + r * w;
-        r = 0.00573950773f + r * w;
-        r = -0.0076224613f + r * w;
-        r = 0.00943887047f + r * w;
-        r = 1.00167406f + r * w;
-        r = 2.83297682f + r * w;
+        const float w = __builtin_amdgcn_sqrtf(-0.69314718f * l) - 3.0f;
+        r = -0.000283145724f;
+        r = 0.000142765653f + r * w;
+        r = 0.00190825947f + r * w;
+        r = -0.00519501232f + r * w;
+        r = 0.00811688974f + r * w;
+        r = -0.0107797878f + r * w;
+        r = 0.0133485785f + r * w;
+        r = 1.41658103f + r * w;
+        r = 4.00643444f + r * w;
     }
-    return 1.41421356237309505f * r * x;
+    const float z = r * x;
+    return fbits == 0x3f800000u ? -37.51937936814848f : z;  // p == 0
 }
+
+// max(v, 0) for finite v in one instruction (v_med3 with FLT_MAX; a plain
+// max also quiets signalling NaNs first).
+__device__ __forceinline__ float mp_relu(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 3.40282347e38f); }
+__device__ __forceinline__ double mp_relu(double v) { return v > 0.0 ? v : 0.0; }
 
 // ---------------------------------------------------------------------------
 // Owen-scrambled Sobol (hash-based nested uniform scrambling, Burley 2020).
@@ -258,6 +269,26 @@ __device__ __forceinline__ uint32_t mp_lk_low24(uint32_t n, uint32_t hs) {
     n = __umul24(n, (hs >> 16) | 1u);
     n ^= __umul24(n, 0x05526c56u);
     n ^= __umul24(n, 0x53a22864u);
+    return n;
+}
+
+// v_mul_u32_u24 with a wave-uniform first operand: the low 32 bits of
+// (a mod 2^24) * (b mod 2^24) at full rate.  Inline asm because, seeing that
+// only low product bits are used, the compiler drops the operand masks of
+// __umul24 and falls back to the quarter-rate 32-bit multiply.
+__device__ __forceinline__ uint32_t mp_mul24_su(uint32_t a_uniform, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(a_uniform), "v"(b));
+    return r;
+}
+
+// mp_lk_low24 for a wave-uniform hashed seed.
+__device__ __forceinline__ uint32_t mp_lk_low24_su(uint32_t n, uint32_t hs_uniform) {
+    n ^= mp_mul24_su(0x3d20adeau, n);
+    n += hs_uniform;
+    n = mp_mul24_su((hs_uniform >> 16) | 1u, n);
+    n ^= mp_mul24_su(0x05526c56u, n);
+    n ^= mp_mul24_su(0x53a22864u, n);
     return n;
 }
 

@@ -41,11 +41,14 @@ struct mp_dreq {
     double cum[MP_POISSON_TERMS];
 };
 
-// Brownian-bridge op in streaming (preorder) form: wm = lw*cur + rw*stack[slot] + sd*z.
-// flags bit0: left child is a leaf (emit wm - cur), bit1: right child is a leaf
-// (emit stack[slot] - wm); otherwise wm is pushed to stack[slot + 1].
+// Brownian-bridge op in streaming (preorder) form on interval increments:
+// stack[slot] holds D = W(r) - W(l); the left part is dl = rw*D + sd*z and the
+// right part dr = D - dl.  flags bit0: left child is a leaf (emit dl, else push
+// it to stack[slot + 1]); bit1: right child is a leaf (emit dr, else store it
+// in stack[slot]).  rwf/sdf are rw/sd rounded to FP32 for the FP32 kernel.
 struct mp_bbop {
-    double lw, rw, sd;
+    float rwf, sdf;
+    double rw, sd;
     int32_t slot;
     int32_t flags;
 };

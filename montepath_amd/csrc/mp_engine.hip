@@ -38,6 +38,7 @@ template <class R>
 struct model_params {
     R drift_dt, vol_dt, xb;
     R kappa_dt, theta, xi_sqdt, rho, rho_perp, rate_dt, half_dt, sqrt_dt;
+    R sigma, xi;  // per unit of Brownian increment (increment-form steps)
     R mu_j, sigma_j;
     int payoff, scheme, model;
 };
@@ -56,6 +57,8 @@ __device__ __forceinline__ model_params<R> load_params(const mp_dreq& q) {
     m.rate_dt = R(q.rate * q.dt);
     m.half_dt = R(0.5 * q.dt);
     m.sqrt_dt = R(q.sqrt_dt);
+    m.sigma = R(q.vol_dt / q.sqrt_dt);
+    m.xi = R(q.xi);
     m.mu_j = R(q.mu_j);
     m.sigma_j = R(q.sigma_j);
     m.payoff = q.payoff;
@@ -103,6 +106,28 @@ __device__ __forceinline__ void heston_step(path_state<R>& st, const model_param
     st.x += (m.rate_dt - m.half_dt * vp) + sv * m.sqrt_dt * z_price;
     R v = st.v + m.kappa_dt * (m.theta - vp) + m.xi_sqdt * sv * z_var;
     if (m.scheme == 0) v = v > R(0) ? v : R(0);
+    st.v = v;
+    observe(st, m);
+}
+
+// The same steps driven by Brownian increments dW = sqrt(dt) z (the FP32
+// Sobol/bridge kernel, whose bridge produces increments): one multiply fewer
+// per factor, equal to the steps above up to rounding.
+template <class R>
+__device__ __forceinline__ void gbm_step_dw(path_state<R>& st, const model_params<R>& m, R dw) {
+    st.x += m.drift_dt + m.sigma * dw;
+    observe(st, m);
+}
+
+template <class R>
+__device__ __forceinline__ void heston_step_dw(path_state<R>& st, const model_params<R>& m, R dw_var,
+                                               R dw_ind) {
+    const R vp = mp_relu(st.v);
+    const R sv = mp_sqrt(vp);
+    const R dw_price = m.rho * dw_var + m.rho_perp * dw_ind;
+    st.x += (m.rate_dt - m.half_dt * vp) + sv * dw_price;
+    R v = st.v + m.kappa_dt * (m.theta - vp) + m.xi * sv * dw_var;
+    if (m.scheme == 0) v = mp_relu(v);
     st.v = v;
     observe(st, m);
 }
@@ -392,45 +417,72 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* dst) {
 // nibble tables of the direction numbers in LDS, bridge stack per lane in LDS,
 // plan ops read with scalar loads.
 // ---------------------------------------------------------------------------
-template <class R>
+// Scrambled Sobol points of one lane (one path index).  Dimensions come in
+// blocks of 256 ("offsets") that share a seed, seed_o = bseed + offset; the
+// point index is Owen-shuffled within its 2^16-point block once per offset,
+// which fixes four per-lane pointers into the nibble tables.  The tables hold
+// direction numbers bit-reversed, so the XOR of four table words is the
+// reversed point that the Laine-Karras permutation takes directly.  With
+// U = true the caller guarantees bseed is the same in every lane of the wave
+// (paths of one tile never straddle a 2^16 block), so the per-dimension
+// hash runs on the scalar unit and the permutation uses 24-bit multiplies
+// by a scalar operand.
+template <class R, bool U>
 struct sobol_cursor {
-    uint32_t bseed;   // block seed: base + block * 0x9E3779B9
-    uint32_t local;   // index within the 2^16-point block
-    int off;          // current dimension offset (dim / 256)
-    uint32_t n0, n1, n2, n3;
+    const uint32_t* nib;
+    const uint32_t *t0, *t1, *t2, *t3;
+    uint32_t bseed, seed_o, local;
+    int off;
+    __device__ __forceinline__ void init(const uint32_t* tables, uint32_t block_seed, uint32_t local_index) {
+        nib = tables;
+        bseed = U ? __builtin_amdgcn_readfirstlane(block_seed) : block_seed;
+        local = local_index;
+        set_offset(0);
+    }
     __device__ __forceinline__ void set_offset(int o) {
         off = o;
-        const uint32_t s = mp_sobol_shuffle(local, mp_hash32(bseed + (uint32_t)o));
-        n0 = s & 15u;
-        n1 = (s >> 4) & 15u;
-        n2 = (s >> 8) & 15u;
-        n3 = (s >> 12) & 15u;
+        seed_o = bseed + (uint32_t)o;
+        const uint32_t s = mp_sobol_shuffle(local, mp_hash32(seed_o));
+        t0 = nib + (s & 15u);
+        t1 = nib + 16 + ((s >> 4) & 15u);
+        t2 = nib + 32 + ((s >> 8) & 15u);
+        t3 = nib + 48 + (s >> 12);
     }
-    __device__ __forceinline__ uint32_t raw(int g, const uint32_t* nib) {
-        const int o = g >> 8;
-        if (o != off) set_offset(o);
-        const uint32_t dd = (uint32_t)(g & 255);
-        const uint32_t* t = nib + dd * 64u;
-        uint32_t x = t[n0] ^ t[16 + n1] ^ t[32 + n2] ^ t[48 + n3];
-        const uint32_t hs = mp_hash32(dd ^ (bseed + (uint32_t)o));
-        return mp_rev32(mp_lk(mp_rev32(x), hs));
+    __device__ __forceinline__ uint32_t xrev(uint32_t dd) const {
+        const uint32_t o = dd * 64u;
+        return t0[o] ^ t1[o] ^ t2[o] ^ t3[o];
     }
-    // Standard normal of dimension g: the top 23 bits of the scrambled point
-    // (computed with the 24-bit-exact permutation), then the inverse normal
-    // of the path precision.
-    __device__ __forceinline__ R normal(int g, const uint32_t* nib) {
-        const int o = g >> 8;
-        if (o != off) set_offset(o);
-        const uint32_t dd = (uint32_t)(g & 255);
-        const uint32_t* t = nib + dd * 64u;
-        const uint32_t x = t[n0] ^ t[16 + n1] ^ t[32 + n2] ^ t[48 + n3];
-        const uint32_t hs = mp_hash32(dd ^ (bseed + (uint32_t)o));
-        const uint32_t top23 = mp_rev32(mp_lk_low24(mp_rev32(x), hs)) >> 9;
-        const R u = R(top23) * R(0x1p-23);
-        if constexpr (sizeof(R) == 4)
-            return mp_inv_normal_f32(u);
+    // Top 23 bits of the scrambled point of dimension dd (< 256) of the
+    // current offset, from the 24-bit-exact permutation.
+    __device__ __forceinline__ uint32_t scrambled_rev(uint32_t dd) const {
+        const uint32_t hs = mp_hash32(dd ^ seed_o);
+        uint32_t y;
+        if constexpr (U)
+            y = mp_lk_low24_su(xrev(dd), hs);
         else
-            return mp_acklam<R>(u);
+            y = mp_lk_low24(xrev(dd), hs);
+        return mp_rev32(y);
+    }
+    // Standard normal of dimension dd of the current offset: inverse normal
+    // of the path precision at the 23-bit uniform p = top 23 bits * 2^-23.
+    __device__ __forceinline__ R normal_at(uint32_t dd) const {
+        if constexpr (sizeof(R) == 4) {
+            // (0x7f:point) >> 9 = the bits of the float 1 + p
+            return mp_inv_normal_f32(__builtin_amdgcn_alignbit(0x7fu, scrambled_rev(dd), 9));
+        } else {
+            return mp_acklam<R>(R(scrambled_rev(dd) >> 9) * R(0x1p-23));
+        }
+    }
+    // Any dimension g, moving the offset as needed.
+    __device__ __forceinline__ R normal(int g) {
+        if ((g >> 8) != off) set_offset(g >> 8);
+        return normal_at((uint32_t)(g & 255));
+    }
+    // Full 32-bit scrambled point of dimension g.
+    __device__ __forceinline__ uint32_t raw(int g) {
+        if ((g >> 8) != off) set_offset(g >> 8);
+        const uint32_t dd = (uint32_t)(g & 255);
+        return mp_rev32(mp_lk(xrev(dd), mp_hash32(dd ^ seed_o)));
     }
 };
 
@@ -465,26 +517,48 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
     const int tid = threadIdx.x;
 
     for (uint32_t tile = blockIdx.x; tile < q.n_tiles; tile += gridDim.x) {
-        const uint64_t p = (uint64_t)tile * bd + tid;
+        // bd is a power of two <= 1024, so a tile lies inside one 2^16 block
+        const uint64_t tile_base = (uint64_t)tile * bd;
+        const uint64_t p = tile_base + tid;
         const bool active = p < q.n_paths;
-        sobol_cursor<R> cur_s;
-        cur_s.bseed = qp.base_seed + (uint32_t)(p >> 16) * 0x9E3779B9u;
-        cur_s.local = (uint32_t)(p & 0xFFFFu);
-        cur_s.set_offset(0);
+        sobol_cursor<R, true> cur;
+        cur.init(nib, qp.base_seed + (uint32_t)(tile_base >> 16) * 0x9E3779B9u, (uint32_t)(p & 0xFFFFu));
 
         path_state<R> st;
         state_init(st, q);
+        // FP64 follows the reference (normals z = dW / sqrt(dt), then z
+        // sqrt(dt) inside the step); FP32 feeds the increments directly.
+        constexpr bool kInc = sizeof(R) == 4;
+        const R to_step = kInc ? R(1) : inv_sqrt_dt;  // bridge increment -> step input
+        const R z_to_step = kInc ? m.sqrt_dt : R(1);  // normal -> step input
         auto emit = [&](R i0, R i1) {
-            if constexpr (F == 1)
-                gbm_step(st, m, i0);
-            else
-                heston_step(st, m, i0, i1);
+            if constexpr (F == 1) {
+                if constexpr (kInc)
+                    gbm_step_dw(st, m, i0);
+                else
+                    gbm_step(st, m, i0);
+            } else {
+                if constexpr (kInc)
+                    heston_step_dw(st, m, i0, i1);
+                else
+                    heston_step(st, m, i0, i1);
+            }
         };
         if (!qp.bridge) {
-            for (int s = 0; s < n_steps; ++s) {
-                const R z0 = cur_s.normal(s * F, nib);
-                const R z1 = F == 2 ? cur_s.normal(s * F + 1, nib) : R(0);
-                emit(z0, z1);
+            // step s uses dimensions s*F + f; 256/F steps per offset
+            constexpr int spo = 256 / F;
+            for (int s0i = 0; s0i < n_steps; s0i += spo) {
+                if (s0i) cur.set_offset(s0i / spo);
+                const int s1 = min(n_steps, s0i + spo);
+                for (int s = s0i; s < s1; ++s) {
+                    const uint32_t dd = (uint32_t)((s - s0i) * F);
+                    const R z0 = cur.normal_at(dd);
+                    const R z1 = F == 2 ? cur.normal_at(dd + 1) : R(0);
+                    if constexpr (kInc)
+                        emit(z0 * z_to_step, z1 * z_to_step);
+                    else
+                        emit(z0, z1);
+                }
             }
         } else {
             // The bridge carries interval increments D = W(r) - W(l) rather than
@@ -494,36 +568,58 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
             // subtracts two O(1) values.  Stack slot s holds D of the interval
             // whose right end is owned by that slot.
 #pragma unroll
-            for (int f = 0; f < F; ++f) stack[(0 * F + f) * bd + tid] = sqrt_t * cur_s.normal(f, nib);
+            for (int f = 0; f < F; ++f) stack[(0 * F + f) * bd + tid] = sqrt_t * cur.normal_at((uint32_t)f);
             if (qp.n_ops == 0) {
-                const R i0 = stack[tid] * inv_sqrt_dt;
-                const R i1 = F == 2 ? stack[bd + tid] * inv_sqrt_dt : R(0);
+                const R i0 = kInc ? stack[tid] : stack[tid] * to_step;
+                const R i1 = F == 2 ? (kInc ? stack[bd + tid] : stack[bd + tid] * to_step) : R(0);
                 emit(i0, i1);
             }
-            for (int k = 0; k < qp.n_ops; ++k) {
-                const mp_bbop op = plan[k];
-                const R rw = R(op.rw), sd = R(op.sd);
-                R dl[F], dr[F];
+            // op k uses dimensions (k+1)*F + f; 256/F ops per offset
+            constexpr int opo = 256 / F;
+            int k = 0;
+            for (int o = 0; k < qp.n_ops; ++o) {
+                if (o) cur.set_offset(o);
+                const int k1 = min(qp.n_ops, (o + 1) * opo - 1);
+                for (; k < k1; ++k) {
+                    const mp_bbop& op = plan[k];
+                    R rw, sd;
+                    if constexpr (sizeof(R) == 4) {
+                        rw = op.rwf;
+                        sd = op.sdf;
+                    } else {
+                        rw = op.rw;
+                        sd = op.sd;
+                    }
+                    const int slot = op.slot, flags = op.flags;
+                    const uint32_t dd = (uint32_t)((k + 1) * F - o * 256);
+                    R dl[F], dr[F];
 #pragma unroll
-                for (int f = 0; f < F; ++f) {
-                    const R z = cur_s.normal((k + 1) * F + f, nib);
-                    const R d = stack[(op.slot * F + f) * bd + tid];
-                    dl[f] = rw * d + sd * z;
-                    dr[f] = d - dl[f];
-                }
-                if (op.flags & 1) {
-                    emit(dl[0] * inv_sqrt_dt, F == 2 ? dl[F - 1] * inv_sqrt_dt : R(0));
-                } else {
+                    for (int f = 0; f < F; ++f) {
+                        const R z = cur.normal_at(dd + f);
+                        const R d = stack[(slot * F + f) * bd + tid];
+                        dl[f] = rw * d + sd * z;
+                        dr[f] = d - dl[f];
+                    }
+                    if (flags & 1) {
+                        if constexpr (kInc)
+                            emit(dl[0], F == 2 ? dl[F - 1] : R(0));
+                        else
+                            emit(dl[0] * to_step, F == 2 ? dl[F - 1] * to_step : R(0));
+                    } else {
 #pragma unroll
-                    for (int f = 0; f < F; ++f) stack[((op.slot + 1) * F + f) * bd + tid] = dl[f];
-                }
-                if (op.flags & 2) {
-                    // right child is a leaf: it follows the left child in step
-                    // order, which is already complete when the left is a leaf
-                    emit(dr[0] * inv_sqrt_dt, F == 2 ? dr[F - 1] * inv_sqrt_dt : R(0));
-                } else {
+                        for (int f = 0; f < F; ++f) stack[((slot + 1) * F + f) * bd + tid] = dl[f];
+                    }
+                    if (flags & 2) {
+                        // right child is a leaf: it follows the left child in step
+                        // order, which is already complete when the left is a leaf
+                        if constexpr (kInc)
+                            emit(dr[0], F == 2 ? dr[F - 1] : R(0));
+                        else
+                            emit(dr[0] * to_step, F == 2 ? dr[F - 1] * to_step : R(0));
+                    } else {
 #pragma unroll
-                    for (int f = 0; f < F; ++f) stack[(op.slot * F + f) * bd + tid] = dr[f];
+                        for (int f = 0; f < F; ++f) stack[(slot * F + f) * bd + tid] = dr[f];
+                    }
                 }
             }
         }
@@ -1006,15 +1102,13 @@ __global__ __launch_bounds__(256) void draws_kernel(const mp_dreq* __restrict__ 
             for (uint64_t e = 0; e < 4 && j + e < n_draws; ++e) out[i * n_draws + j + e] = z[e];
         }
     } else {
-        sobol_cursor<double> cs;
-        cs.bseed = qp.base_seed + (uint32_t)(p >> 16) * 0x9E3779B9u;
-        cs.local = (uint32_t)(p & 0xFFFFu);
-        cs.set_offset(0);
+        sobol_cursor<double, false> cs;
+        cs.init(nib, qp.base_seed + (uint32_t)(p >> 16) * 0x9E3779B9u, (uint32_t)(p & 0xFFFFu));
         for (uint64_t j = 0; j < n_draws; ++j) {
             if (kind == 2)
-                out[i * n_draws + j] = cs.normal((int)j, nib);
+                out[i * n_draws + j] = cs.normal((int)j);
             else
-                out[i * n_draws + j] = (double)cs.raw((int)j, nib);
+                out[i * n_draws + j] = (double)cs.raw((int)j);
         }
     }
 }
