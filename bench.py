@@ -120,6 +120,56 @@ def bench_batch(eng, reps: int = 5) -> dict:
             "mean_price": float(np.mean([x.price for x in res]))}
 
 
+def bench_lsm(eng, reps: int = 5) -> dict:
+    """American put S0=100, K=100, r=0.05, sigma=0.25, T=1 with 50 exercise
+    dates, 2^20 paths, unweighted Laguerre degree 3, FP32 paths [date][path],
+    FP64 cash flows, Gram sums and 4x4 solves.  Roofline: the backward pass's
+    algorithmic HBM bytes, 1,776 per path (50 x 4 B path writes, 49 dates x
+    32 B: the date's price read twice and the FP64 discounted cash flow read,
+    read and written, plus 8 B terminal), over the measured time."""
+    from montepath_amd.engine import make_request
+
+    n = 1 << 20
+    req = make_request("american_put", n_paths=n, n_steps=50, k=100.0, r=0.05, sigma=0.25,
+                       t=1.0, basis_degree=3, precision=32, seed=42)
+    eng.price(req)
+    ms = []
+    for _ in range(reps):
+        res = eng.price(req)
+        ms.append(eng.last_timing()["main_kernel_ms"])
+    t = float(np.median(ms))
+    b_alg = 1776.0 * n
+    return {"n_paths": n, "dates": 50, "degree": 3, "ms": t, "launches": eng.last_timing()["main_launches"],
+            "price": res.price, "stderr": res.stderr, "regression_steps": res.regression_steps,
+            "crr_bermudan_50": 7.962057,
+            "hbm_alg_bytes": b_alg, "hbm_alg_gbs": b_alg / (t * 1e-3) / 1e9,
+            "hbm_frac": b_alg / (t * 1e-3) / 1e9 / PEAK_HBM_GBS}
+
+
+def bench_basket(eng, reps: int = 5) -> dict:
+    """Five-asset stepped basket call: S0=100, K=100, sigma=0.2, rho=0.5,
+    equal weights, 2^22 paths x 64 steps, Cholesky-correlated Philox normals,
+    FP32 paths.  F_alg per path-step: Cholesky apply 25 + five GBM steps 15 +
+    five Box-Muller normals 30 = 70."""
+    from montepath_amd.engine import make_request
+
+    na, n, m = 5, 1 << 22, 64
+    corr = np.full((na, na), 0.5) + 0.5 * np.eye(na)
+    req = make_request("basket_call", n_paths=n, n_steps=m, asset_s0=[100.0] * na,
+                       asset_sigma=[0.2] * na, asset_weight=[1.0 / na] * na, correlation=corr,
+                       k=100.0, precision=32, seed=42)
+    eng.price(req)
+    ms = []
+    for _ in range(reps):
+        res = eng.price(req)
+        ms.append(eng.last_timing()["main_kernel_ms"])
+    t = float(np.median(ms))
+    ps = n * m / (t * 1e-3)
+    return {"assets": na, "n_paths": n, "n_steps": m, "ms": t, "path_steps_per_s": ps,
+            "tflops_alg": ps * 70 / 1e12, "fp32_frac": ps * 70 / 1e12 / PEAK_FP32_TFLOPS,
+            "price": res.price, "stderr": res.stderr}
+
+
 SERVE_KINDS = ("european_call", "arithmetic_asian_call", "heston_european_call",
                "merton_jump_diffusion_call", "american_put")
 
@@ -224,6 +274,8 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip batch and serving")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--only", choices=["batch", "lsm", "basket", "serve"], default=None,
+                    help="run one secondary section by itself (for profiling) and print its JSON")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run three device-to-device copies of this many MiB (the "
                          "known-size kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
@@ -240,6 +292,13 @@ def main() -> int:
     eng = Engine(0)
     info = eng.device_info()
     log(f"[bench] {info['device_id']} {info['name']} {info['arch']} CUs={info['compute_units']}")
+    if args.only:
+        section = {"batch": lambda: bench_batch(eng), "lsm": lambda: bench_lsm(eng),
+                   "basket": lambda: bench_basket(eng),
+                   "serve": lambda: bench_serve(eng, args.serve_seconds, args.serve_rate)}[args.only]()
+        eng.close()
+        print(json.dumps({args.only: section}), flush=True)
+        return 0
     req = heston_request(32)
     for _ in range(args.warmup):
         eng.price(req)
@@ -284,9 +343,15 @@ def main() -> int:
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_source": f"profiles/{TRAFFIC_FILE}" if traffic else None,
+                     "valu_busy": traffic.get("valu_busy_per_simd") if traffic else None,
+                     "valu_insts_per_path_step": traffic.get("valu_insts_per_path_step") if traffic else None,
                      "kernel": "qmc_kernel<float,2,true>", "kernel_ms": k_ms,
                      "flop_per_path_step": FLOP_PER_PATH_STEP,
-                     "note": "FP32 vector (VALU) work; MI355X FP32 peak 157.3 TFLOP/s; MFMA unused"},
+                     "note": "FP32 vector (VALU) work priced at the packed-FP32 peak 157.3 TFLOP/s "
+                             "(78.6 without packed FMA, which this build does not emit); MFMA unused; "
+                             "valu_busy is rocprof SQ_ACTIVE_INST_VALU per SIMD: the kernel is "
+                             "issue-bound on integer (Sobol scramble) and transcendental work "
+                             "that the flop count excludes"},
         "prices": res.strike_prices,
         "stderrs": res.strike_stderrs,
         "fp64_prices": res64.strike_prices,
@@ -303,6 +368,10 @@ def main() -> int:
         extra = {}
         extra["batched_4096_contracts"] = bench_batch(eng)
         log(f"[bench] batch {extra['batched_4096_contracts']}")
+        extra["lsm_american_put_2p20x50"] = bench_lsm(eng)
+        log(f"[bench] lsm {extra['lsm_american_put_2p20x50']}")
+        extra["basket_5_assets_2p22x64"] = bench_basket(eng)
+        log(f"[bench] basket {extra['basket_5_assets_2p22x64']}")
         extra["serving"] = bench_serve(eng, args.serve_seconds, args.serve_rate)
         log(f"[bench] serving {extra['serving']}")
         out["extra"] = extra

@@ -20,8 +20,9 @@ __device__ __forceinline__ mp_u4 mp_philox(uint32_t c0, uint32_t c1, uint32_t c2
         // one v_mad_u64_u32 per 32x32 -> 64-bit product gives both halves
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        // three-input XOR in one v_bitop3_b32 (truth table 0x96)
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         c0 = n0;
         c1 = (uint32_t)p1;
         c2 = n2;
@@ -35,7 +36,9 @@ __device__ __forceinline__ mp_u4 mp_philox(uint32_t c0, uint32_t c1, uint32_t c2
 // Uniform in [2^-24, 1 - 2^-24] with 23 significant bits plus a half step:
 // exact in both float and double, so FP32 and FP64 runs see the same draws.
 __device__ __forceinline__ float mp_u01f(uint32_t x) {
-    return ((float)(x >> 9) + 0.5f) * 0x1p-23f;
+    // (0x7f:x) >> 9 is the float 1 + k 2^-23 (k = x >> 9); subtracting
+    // 1 - 2^-24 is exact (Sterbenz) and leaves (k + 1/2) 2^-23
+    return __uint_as_float(__builtin_amdgcn_alignbit(0x7fu, x, 9)) - 0.99999994039535522f;
 }
 __device__ __forceinline__ double mp_u01d(uint32_t x) {
     return ((double)(x >> 9) + 0.5) * 0x1p-23;
