@@ -708,7 +708,7 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     if (!ops.empty())
         HIPCHK(hipMemcpyAsync(c->plan.p, ops.data(), ops.size() * sizeof(mp_bbop), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(mp::launch_qmc(c->reqs.as<mp_dreq>(), qp, c->d_nib, c->plan.as<mp_bbop>(),
+    HIPCHK(mp::launch_qmc(c->reqs.as<mp_dreq>(), qp, d.payoff, d.scheme, c->d_nib, c->plan.as<mp_bbop>(),
                           c->partial.as<double>(), term_host ? c->term.as<double>() : nullptr,
                           pay_host ? c->pay.as<double>() : nullptr, precision, grid, block, nib_lds,
                           c->stream));

@@ -99,10 +99,11 @@ hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s);
 hipError_t launch_tiles(const tile_launch& L, int precision, hipStream_t s);
 hipError_t launch_finalize(const mp_dreq* reqs, int n_reqs, const double* partial, double* out,
                            hipStream_t s);
-hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
-                      const mp_bbop* plan, double* partial, double* terminal_out,
-                      double* payoff_out, int precision, int grid, int block, bool nib_lds,
-                      hipStream_t s);
+// payoff / scheme: the request's (they select compiled-in specialisations)
+hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, int scheme,
+                      const uint32_t* nib, const mp_bbop* plan, double* partial,
+                      double* terminal_out, double* payoff_out, int precision, int grid, int block,
+                      bool nib_lds, hipStream_t s);
 size_t qmc_lds_bytes(int precision, int factors, int n_slots, int block, bool nib_lds);
 hipError_t launch_basket(const mp_dreq* req, int n_assets, const double* chol,
                          const double* drift, const double* vol, const double* log_s0,

@@ -450,7 +450,7 @@ struct sobol_cursor {
     }
     __device__ __forceinline__ uint32_t xrev(uint32_t dd) const {
         const uint32_t o = dd * 64u;
-        return t0[o] ^ t1[o] ^ t2[o] ^ t3[o];
+        return __builtin_amdgcn_bitop3_b32(t0[o], t1[o], t2[o], 0x96) ^ t3[o];
     }
     // Top 23 bits of the scrambled point of dimension dd (< 256) of the
     // current offset, from the 24-bit-exact permutation.
@@ -486,7 +486,9 @@ struct sobol_cursor {
     }
 };
 
-template <class R, int F, bool NIB_LDS>
+// PAY / SCH >= 0 fix the payoff / Heston scheme at compile time (the
+// request's fields must then hold those values); -1 reads them at run time.
+template <class R, int F, bool NIB_LDS, int PAY, int SCH>
 __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ reqp,
                                                   mp_qmc_params qp,
                                                   const uint32_t* __restrict__ nib_g,
@@ -504,7 +506,9 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
     }
 
     const mp_dreq& q = *reqp;
-    const model_params<R> m = load_params<R>(q);
+    model_params<R> m = load_params<R>(q);
+    if constexpr (PAY >= 0) m.payoff = PAY;
+    if constexpr (SCH >= 0) m.scheme = SCH;
     const int n_steps = q.n_steps;
     const int ns = q.n_strikes;
     const R s0 = R(q.s0);
@@ -1174,35 +1178,44 @@ size_t qmc_lds_bytes(int precision, int factors, int n_slots, int block, bool ni
     return (nib_lds ? (size_t)kNibWords * 4 : 0) + (size_t)n_slots * factors * block * rb;
 }
 
-template <class R, int F>
+template <class R, int F, int PAY, int SCH>
 static void qmc_launch_nl(bool nib_lds, dim3 g, dim3 b, size_t lds, hipStream_t s,
                           const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
                           const mp_bbop* plan, double* partial, double* term, double* pay) {
     if (nib_lds)
-        hipLaunchKernelGGL((qmc_kernel<R, F, true>), g, b, lds, s, req, qp, nib, plan, partial,
-                           term, pay);
+        hipLaunchKernelGGL((qmc_kernel<R, F, true, PAY, SCH>), g, b, lds, s, req, qp, nib, plan,
+                           partial, term, pay);
     else
-        hipLaunchKernelGGL((qmc_kernel<R, F, false>), g, b, lds, s, req, qp, nib, plan, partial,
-                           term, pay);
+        hipLaunchKernelGGL((qmc_kernel<R, F, false, PAY, SCH>), g, b, lds, s, req, qp, nib, plan,
+                           partial, term, pay);
 }
 
-hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
-                      const mp_bbop* plan, double* partial, double* terminal_out,
-                      double* payoff_out, int precision, int grid, int block, bool nib_lds,
-                      hipStream_t s) {
+hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, int scheme,
+                      const uint32_t* nib, const mp_bbop* plan, double* partial,
+                      double* terminal_out, double* payoff_out, int precision, int grid, int block,
+                      bool nib_lds, hipStream_t s) {
     const size_t lds =
         qmc_lds_bytes(precision, qp.factors, qp.n_slots > 0 ? qp.n_slots : 1, block, nib_lds);
     const dim3 g(grid), b(block);
+    const bool euro = payoff == MP_PAY_EUROPEAN;
     if (precision == 64) {
         if (qp.factors == 2)
-            qmc_launch_nl<double, 2>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+            qmc_launch_nl<double, 2, -1, -1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
         else
-            qmc_launch_nl<double, 1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+            qmc_launch_nl<double, 1, -1, -1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+    } else if (qp.factors == 2) {
+        // FP32 European Heston: payoff and scheme compiled in
+        if (euro && scheme == 1)
+            qmc_launch_nl<float, 2, MP_PAY_EUROPEAN, 1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+        else if (euro && scheme == 0)
+            qmc_launch_nl<float, 2, MP_PAY_EUROPEAN, 0>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+        else
+            qmc_launch_nl<float, 2, -1, -1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
     } else {
-        if (qp.factors == 2)
-            qmc_launch_nl<float, 2>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+        if (euro)
+            qmc_launch_nl<float, 1, MP_PAY_EUROPEAN, -1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
         else
-            qmc_launch_nl<float, 1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+            qmc_launch_nl<float, 1, -1, -1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
     }
     return hipGetLastError();
 }
