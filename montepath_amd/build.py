@@ -36,6 +36,9 @@ def flags() -> list[str]:
         "-fvisibility=hidden",
         # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        # multiply-adds fuse only within one source expression, so every
+        # kernel specialisation computes bit-identical results
+        "-ffp-contract=on",
         "-I", INCLUDE,
         "-I", CSRC,
         "-Wall",

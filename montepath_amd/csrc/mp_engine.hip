@@ -197,6 +197,138 @@ __device__ __forceinline__ unsigned int next_queue_position(unsigned int* counte
     return __builtin_amdgcn_readfirstlane(*slot);
 }
 
+// One tile of one request.  MODEL / PAY / SCH >= 0 compile the model,
+// payoff and Heston scheme in (the request must hold those values); -1 reads
+// them from the request at run time.
+template <class R, int SRC, int PPL, int MODEL, int PAY, int SCH>
+__device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t tile, const mp_dreq& q,
+                                          int lane) {
+    model_params<R> m = load_params<R>(q);
+    if constexpr (MODEL >= 0) m.model = MODEL;
+    if constexpr (PAY >= 0) m.payoff = PAY;
+    if constexpr (SCH >= 0) m.scheme = SCH;
+    const int n_steps = q.n_steps;
+    const int ns = q.n_strikes;
+    const R s0 = R(q.s0);
+    R kk[MP_NS_MAX];
+#pragma unroll
+    for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = R(q.strikes[j]);
+    const uint64_t pbeg = (uint64_t)(tile - q.tile0) * q.tile_paths;
+    const uint64_t pend = min(pbeg + (uint64_t)q.tile_paths, q.n_paths);
+
+    double sum[MP_NS_MAX], sq[MP_NS_MAX];
+#pragma unroll
+    for (int j = 0; j < MP_NS_MAX; ++j) sum[j] = sq[j] = 0.0;
+
+    for (uint64_t base = pbeg; base < pend; base += (uint64_t)MP_WAVE * PPL) {
+        path_state<R> st[PPL];
+        uint64_t pid[PPL];
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            pid[j] = base + lane + (uint64_t)MP_WAVE * j;
+            state_init(st[j], q);
+        }
+        for (int s0i = 0; s0i < n_steps; s0i += 4) {
+            const uint32_t blk = (uint32_t)(s0i >> 2);
+            const int nsub = min(4, n_steps - s0i);
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                const uint64_t p = pid[j];
+                const uint64_t pr = p < pend ? p : pend - 1;  // clamp buffer reads
+                if (m.model == MP_MODEL_GBM) {
+                    R z[4];
+                    if constexpr (SRC == 0) {
+                        mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, z);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            z[e] = e < nsub ? R(L.normals[(uint64_t)(s0i + e) * q.n_paths + pr]) : R(0);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (e < nsub) gbm_step(st[j], m, z[e]);
+                } else if (m.model == MP_MODEL_HESTON) {
+                    R zv[4], zi[4];
+                    if constexpr (SRC == 0) {
+                        mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, zv);
+                        mp_normals4<R>((uint32_t)p, blk, 1u, q.seed_hi, q.key0, q.key1, zi);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const uint64_t row = (uint64_t)(s0i + e) * 2;
+                            zv[e] = e < nsub ? R(L.normals[row * q.n_paths + pr]) : R(0);
+                            zi[e] = e < nsub ? R(L.normals[(row + 1) * q.n_paths + pr]) : R(0);
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (e < nsub) heston_step(st[j], m, zv[e], zi[e]);
+                } else {  // Merton
+                    R z[4];
+                    double u[4];
+                    if constexpr (SRC == 0) {
+                        mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, z);
+                        mp_uniforms4<double>((uint32_t)p, blk, 1u, q.seed_hi, q.key0, q.key1, u);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const uint64_t idx = (uint64_t)(s0i + e) * q.n_paths + pr;
+                            z[e] = e < nsub ? R(L.normals[idx]) : R(0);
+                            u[e] = e < nsub ? L.uniforms[idx] : 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        if (e < nsub) {
+                            const int nj = poisson_count(u[e], q);
+                            R zj = R(0);
+                            if (nj != 0) {
+                                if constexpr (SRC == 0) {
+                                    R zz[4];
+                                    mp_normals4<R>((uint32_t)p, blk, 2u, q.seed_hi, q.key0, q.key1, zz);
+                                    zj = zz[e];
+                                } else {
+                                    zj = R(L.jump_normals[(uint64_t)(s0i + e) * q.n_paths + pr]);
+                                }
+                            }
+                            merton_apply(st[j], m, z[e], nj, zj);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            if (pid[j] < pend) {
+#pragma unroll
+                for (int sidx = 0; sidx < MP_NS_MAX; ++sidx) {
+                    if (sidx < ns) {
+                        const double pay = (double)payoff_of(st[j], m, q, n_steps, s0, kk[sidx]);
+                        sum[sidx] += pay;
+                        sq[sidx] += pay * pay;
+                    }
+                }
+                if (L.terminal_out) L.terminal_out[pid[j]] = (double)(s0 * mp_exp(st[j].x));
+                if (L.payoff_out)
+                    L.payoff_out[pid[j]] = (double)payoff_of(st[j], m, q, n_steps, s0, kk[0]);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MP_NS_MAX; ++j) {
+        sum[j] = mp_wave_sum(sum[j]);
+        sq[j] = mp_wave_sum(sq[j]);
+    }
+    if (lane == 0) {
+        double* out = L.partial + (uint64_t)tile * MP_PARTIAL_STRIDE;
+#pragma unroll
+        for (int j = 0; j < MP_NS_MAX; ++j) {
+            out[2 * j] = sum[j];
+            out[2 * j + 1] = sq[j];
+        }
+    }
+}
+
 template <class R, int SRC, int PPL>
 __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
     __shared__ unsigned int queue_slot[256 / MP_WAVE];
@@ -211,126 +343,33 @@ __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
         const uint32_t tile = L.order ? L.order[qpos] : qpos;
         const uint32_t ri = L.tile_req[tile];
         const mp_dreq& q = L.reqs[ri];
-        const model_params<R> m = load_params<R>(q);
-        const int n_steps = q.n_steps;
-        const int ns = q.n_strikes;
-        const R s0 = R(q.s0);
-        R kk[MP_NS_MAX];
-#pragma unroll
-        for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = R(q.strikes[j]);
-        const uint64_t pbeg = (uint64_t)(tile - q.tile0) * q.tile_paths;
-        const uint64_t pend = min(pbeg + (uint64_t)q.tile_paths, q.n_paths);
-
-        double sum[MP_NS_MAX], sq[MP_NS_MAX];
-#pragma unroll
-        for (int j = 0; j < MP_NS_MAX; ++j) sum[j] = sq[j] = 0.0;
-
-        for (uint64_t base = pbeg; base < pend; base += (uint64_t)MP_WAVE * PPL) {
-            path_state<R> st[PPL];
-            uint64_t pid[PPL];
-#pragma unroll
-            for (int j = 0; j < PPL; ++j) {
-                pid[j] = base + lane + (uint64_t)MP_WAVE * j;
-                state_init(st[j], q);
+        if constexpr (sizeof(R) == 4 && SRC == 0) {
+            // FP32 Philox tiles (batches, serving): a body per model and payoff
+            const int mdl = q.model, pay = q.payoff;
+            if (mdl == MP_MODEL_GBM) {
+                if (pay == MP_PAY_EUROPEAN)
+                    tile_body<R, SRC, PPL, MP_MODEL_GBM, MP_PAY_EUROPEAN, -1>(L, tile, q, lane);
+                else if (pay == MP_PAY_ASIAN)
+                    tile_body<R, SRC, PPL, MP_MODEL_GBM, MP_PAY_ASIAN, -1>(L, tile, q, lane);
+                else if (pay == MP_PAY_DOWN_AND_OUT)
+                    tile_body<R, SRC, PPL, MP_MODEL_GBM, MP_PAY_DOWN_AND_OUT, -1>(L, tile, q, lane);
+                else
+                    tile_body<R, SRC, PPL, MP_MODEL_GBM, MP_PAY_LOOKBACK, -1>(L, tile, q, lane);
+            } else if (mdl == MP_MODEL_HESTON) {
+                if (pay == MP_PAY_EUROPEAN && q.scheme == 1)
+                    tile_body<R, SRC, PPL, MP_MODEL_HESTON, MP_PAY_EUROPEAN, 1>(L, tile, q, lane);
+                else if (pay == MP_PAY_EUROPEAN)
+                    tile_body<R, SRC, PPL, MP_MODEL_HESTON, MP_PAY_EUROPEAN, 0>(L, tile, q, lane);
+                else
+                    tile_body<R, SRC, PPL, MP_MODEL_HESTON, -1, -1>(L, tile, q, lane);
+            } else {
+                if (pay == MP_PAY_EUROPEAN)
+                    tile_body<R, SRC, PPL, MP_MODEL_MERTON, MP_PAY_EUROPEAN, -1>(L, tile, q, lane);
+                else
+                    tile_body<R, SRC, PPL, MP_MODEL_MERTON, -1, -1>(L, tile, q, lane);
             }
-            for (int s0i = 0; s0i < n_steps; s0i += 4) {
-                const uint32_t blk = (uint32_t)(s0i >> 2);
-                const int nsub = min(4, n_steps - s0i);
-#pragma unroll
-                for (int j = 0; j < PPL; ++j) {
-                    const uint64_t p = pid[j];
-                    const uint64_t pr = p < pend ? p : pend - 1;  // clamp buffer reads
-                    if (m.model == MP_MODEL_GBM) {
-                        R z[4];
-                        if constexpr (SRC == 0) {
-                            mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, z);
-                        } else {
-#pragma unroll
-                            for (int e = 0; e < 4; ++e)
-                                z[e] = e < nsub ? R(L.normals[(uint64_t)(s0i + e) * q.n_paths + pr]) : R(0);
-                        }
-#pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            if (e < nsub) gbm_step(st[j], m, z[e]);
-                    } else if (m.model == MP_MODEL_HESTON) {
-                        R zv[4], zi[4];
-                        if constexpr (SRC == 0) {
-                            mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, zv);
-                            mp_normals4<R>((uint32_t)p, blk, 1u, q.seed_hi, q.key0, q.key1, zi);
-                        } else {
-#pragma unroll
-                            for (int e = 0; e < 4; ++e) {
-                                const uint64_t row = (uint64_t)(s0i + e) * 2;
-                                zv[e] = e < nsub ? R(L.normals[row * q.n_paths + pr]) : R(0);
-                                zi[e] = e < nsub ? R(L.normals[(row + 1) * q.n_paths + pr]) : R(0);
-                            }
-                        }
-#pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            if (e < nsub) heston_step(st[j], m, zv[e], zi[e]);
-                    } else {  // Merton
-                        R z[4];
-                        double u[4];
-                        if constexpr (SRC == 0) {
-                            mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, z);
-                            mp_uniforms4<double>((uint32_t)p, blk, 1u, q.seed_hi, q.key0, q.key1, u);
-                        } else {
-#pragma unroll
-                            for (int e = 0; e < 4; ++e) {
-                                const uint64_t idx = (uint64_t)(s0i + e) * q.n_paths + pr;
-                                z[e] = e < nsub ? R(L.normals[idx]) : R(0);
-                                u[e] = e < nsub ? L.uniforms[idx] : 0.0;
-                            }
-                        }
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            if (e < nsub) {
-                                const int nj = poisson_count(u[e], q);
-                                R zj = R(0);
-                                if (nj != 0) {
-                                    if constexpr (SRC == 0) {
-                                        R zz[4];
-                                        mp_normals4<R>((uint32_t)p, blk, 2u, q.seed_hi, q.key0, q.key1, zz);
-                                        zj = zz[e];
-                                    } else {
-                                        zj = R(L.jump_normals[(uint64_t)(s0i + e) * q.n_paths + pr]);
-                                    }
-                                }
-                                merton_apply(st[j], m, z[e], nj, zj);
-                            }
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < PPL; ++j) {
-                if (pid[j] < pend) {
-#pragma unroll
-                    for (int sidx = 0; sidx < MP_NS_MAX; ++sidx) {
-                        if (sidx < ns) {
-                            const double pay = (double)payoff_of(st[j], m, q, n_steps, s0, kk[sidx]);
-                            sum[sidx] += pay;
-                            sq[sidx] += pay * pay;
-                        }
-                    }
-                    if (L.terminal_out) L.terminal_out[pid[j]] = (double)(s0 * mp_exp(st[j].x));
-                    if (L.payoff_out)
-                        L.payoff_out[pid[j]] = (double)payoff_of(st[j], m, q, n_steps, s0, kk[0]);
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < MP_NS_MAX; ++j) {
-            sum[j] = mp_wave_sum(sum[j]);
-            sq[j] = mp_wave_sum(sq[j]);
-        }
-        if (lane == 0) {
-            double* out = L.partial + (uint64_t)tile * MP_PARTIAL_STRIDE;
-#pragma unroll
-            for (int j = 0; j < MP_NS_MAX; ++j) {
-                out[2 * j] = sum[j];
-                out[2 * j + 1] = sq[j];
-            }
+        } else {
+            tile_body<R, SRC, PPL, -1, -1, -1>(L, tile, q, lane);
         }
     }
 }

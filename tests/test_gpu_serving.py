@@ -89,6 +89,28 @@ def test_autotune_keeps_results(engine):
     engine.set_tuning(req, 256, 1)
 
 
+@pytest.mark.parametrize("kind,extra", [
+    ("european_call", {}),
+    ("arithmetic_asian_call", {}),
+    ("down_and_out_call", {"barrier": 85.0}),
+    ("heston_european_call", {"kappa": 2.0, "xi": 0.5, "rho": -0.7, "heston_scheme": "full_truncation"}),
+    ("heston_european_call", {"kappa": 2.0, "xi": 0.5, "rho": -0.7}),
+    ("merton_jump_diffusion_call", {"jump_intensity": 1.0, "jump_mean": -0.1, "jump_volatility": 0.15}),
+])
+def test_every_tuning_gives_identical_results(engine, kind, extra):
+    # block size and paths interleaved per lane select different compiled
+    # kernels; the per-path arithmetic and the summation order must not change
+    req = make_request(kind, n_paths=(1 << 16) + 77, n_steps=37, seed=4, **extra)
+    seen = set()
+    for bs in (128, 256):
+        for ppl in (1, 2, 4):
+            engine.set_tuning(req, bs, ppl)
+            r = engine.price(req)
+            seen.add((r.price, r.stderr))
+    engine.set_tuning(req, 256, 1)
+    assert len(seen) == 1, seen
+
+
 def test_terminal_method_matches_black_scholes(engine):
     req = make_request("european_call", n_paths=1 << 20, n_steps=64, method="terminal", seed=9)
     res = engine.price(req)
