@@ -343,14 +343,15 @@ def main() -> int:
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_source": f"profiles/{TRAFFIC_FILE}" if traffic else None,
-                     "valu_busy": traffic.get("valu_busy_per_simd") if traffic else None,
+                     "valu_issue": traffic.get("valu_issue_frac") if traffic else None,
                      "valu_insts_per_path_step": traffic.get("valu_insts_per_path_step") if traffic else None,
                      "kernel": "qmc_kernel<float,2,true>", "kernel_ms": k_ms,
                      "flop_per_path_step": FLOP_PER_PATH_STEP,
                      "note": "FP32 vector (VALU) work priced at the packed-FP32 peak 157.3 TFLOP/s "
                              "(78.6 without packed FMA, which this build does not emit); MFMA unused; "
-                             "valu_busy is rocprof SQ_ACTIVE_INST_VALU per SIMD: the kernel is "
-                             "issue-bound on integer (Sobol scramble) and transcendental work "
+                             "valu_issue is the rocprof-measured share of wave64 VALU issue "
+                             "slots used (transcendentals, which co-issue, excluded): the kernel "
+                             "is issue-bound on integer (Sobol scramble) and transcendental work "
                              "that the flop count excludes"},
         "prices": res.strike_prices,
         "stderrs": res.strike_stderrs,
