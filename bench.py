@@ -9,7 +9,11 @@ precision configuration's.  Prints one JSON line.
 
 Secondary measurements (same line, under "extra"): the 4,096-contract
 batched persistent launch against plain one-launch-per-request FP64 pricing
-of the same contracts, and a short Poisson serving trace with p50/p99.
+of the same contracts, the 2^20-path 50-date American put (LSM) against the
+HBM roofline, the five-asset 2^22 x 64 basket, and a short Poisson serving
+trace with p50/p99.  `--only <section>` runs one of them alone (for
+profiling); `--copy-probe-mib N` runs only a device-to-device copy of known
+size (to scale the FETCH_SIZE / WRITE_SIZE counters).
 """
 from __future__ import annotations
 
