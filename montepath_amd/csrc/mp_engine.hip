@@ -380,49 +380,64 @@ __global__ void reset_kernel(unsigned int* counter, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// Finalize: one wave per request; tiles summed in tile order (Neumaier), then
-// price = discount * mean and the population-variance standard error of the
-// reference's summarize_payoffs (cpu.rs:7641-7652), all in FP64.
+// Finalize: one block per request, one wave per strike; tiles summed in tile
+// order (Neumaier), then price = discount * mean and the population-variance
+// standard error of the reference's summarize_payoffs (cpu.rs:7641-7652), all
+// in FP64.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void finalize_kernel(const mp_dreq* reqs, const double* partial,
-                                                     double* out) {
-    __shared__ double red[2][MP_WAVE];
+__global__ __launch_bounds__(64 * MP_NS_MAX) void finalize_kernel(const mp_dreq* reqs,
+                                                                  const double* partial,
+                                                                  double* out) {
+    __shared__ double red[MP_NS_MAX][2][MP_WAVE];
     const mp_dreq& q = reqs[blockIdx.x];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    const int j = threadIdx.x >> 6;  // one wave per strike
     const double n = (double)q.n_paths;
-    for (int j = 0; j < MP_NS_MAX; ++j) {
-        mp_neumaier a, b;
-        a.init();
-        b.init();
+    // Lane l adds tiles l, l + 64, ... in that order; sixteen tiles' (sum,
+    // sum of squares) pairs are loaded before any is added.
+    mp_neumaier a, b;
+    a.init();
+    b.init();
+    const double2* col = reinterpret_cast<const double2*>(partial + (uint64_t)q.tile0 * MP_PARTIAL_STRIDE) + j;
+    constexpr int kRow2 = MP_PARTIAL_STRIDE / 2;  // double2 per tile row
+    constexpr int kAhead = 16;
+    if (j < q.n_strikes) {
+        for (uint32_t t0 = lane; t0 < q.n_tiles; t0 += MP_WAVE * kAhead) {
+            double2 v[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) {
+                const uint32_t t = t0 + (uint32_t)u * MP_WAVE;
+                v[u] = t < q.n_tiles ? col[(uint64_t)t * kRow2] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) {
+                if (t0 + (uint32_t)u * MP_WAVE < q.n_tiles) {
+                    a.add(v[u].x);
+                    b.add(v[u].y);
+                }
+            }
+        }
+    }
+    red[j][0][lane] = a.value();
+    red[j][1][lane] = b.value();
+    __syncthreads();
+    if (lane == 0) {
+        mp_neumaier s, s2;
+        s.init();
+        s2.init();
+        for (int i = 0; i < MP_WAVE; ++i) {
+            s.add(red[j][0][i]);
+            s2.add(red[j][1][i]);
+        }
+        double price = 0.0, se = 0.0;
         if (j < q.n_strikes) {
-            for (uint32_t t = lane; t < q.n_tiles; t += MP_WAVE) {
-                const double* p = partial + (uint64_t)(q.tile0 + t) * MP_PARTIAL_STRIDE;
-                a.add(p[2 * j]);
-                b.add(p[2 * j + 1]);
-            }
+            const double mean = s.value() / n;
+            const double var = s2.value() / n - mean * mean;
+            price = q.discount * mean;
+            se = q.discount * sqrt(var > 0.0 ? var : 0.0) / sqrt(n);
         }
-        red[0][lane] = a.value();
-        red[1][lane] = b.value();
-        __syncthreads();
-        if (lane == 0) {
-            mp_neumaier s, s2;
-            s.init();
-            s2.init();
-            for (int i = 0; i < MP_WAVE; ++i) {
-                s.add(red[0][i]);
-                s2.add(red[1][i]);
-            }
-            double price = 0.0, se = 0.0;
-            if (j < q.n_strikes) {
-                const double mean = s.value() / n;
-                const double var = s2.value() / n - mean * mean;
-                price = q.discount * mean;
-                se = q.discount * sqrt(var > 0.0 ? var : 0.0) / sqrt(n);
-            }
-            out[(uint64_t)blockIdx.x * MP_PARTIAL_STRIDE + 2 * j] = price;
-            out[(uint64_t)blockIdx.x * MP_PARTIAL_STRIDE + 2 * j + 1] = se;
-        }
-        __syncthreads();
+        out[(uint64_t)blockIdx.x * MP_PARTIAL_STRIDE + 2 * j] = price;
+        out[(uint64_t)blockIdx.x * MP_PARTIAL_STRIDE + 2 * j + 1] = se;
     }
 }
 
@@ -1208,7 +1223,7 @@ hipError_t launch_tiles(const tile_launch& L, int precision, hipStream_t s) {
 
 hipError_t launch_finalize(const mp_dreq* reqs, int n_reqs, const double* partial, double* out,
                            hipStream_t s) {
-    hipLaunchKernelGGL(finalize_kernel, dim3(n_reqs), dim3(64), 0, s, reqs, partial, out);
+    hipLaunchKernelGGL(finalize_kernel, dim3(n_reqs), dim3(64 * MP_NS_MAX), 0, s, reqs, partial, out);
     return hipGetLastError();
 }
 
