@@ -116,6 +116,12 @@ def lib() -> C.CDLL:
         "mpo_summarize": (Result, [C.c_uint64, C.c_double, C.c_double]),
         "mpo_gbm_stepwise_price": (Result, [C.POINTER(GbmCfg), C.c_int]),
         "mpo_european_terminal_price": (Result, [C.POINTER(GbmCfg)]),
+        "mpo_gbm_stepwise_tech_price": (Result, [C.POINTER(GbmCfg), C.c_int, C.c_int]),
+        "mpo_cv_record": (None, [C.c_void_p, C.c_double, C.c_double]),
+        "mpo_summarize_cv": (Result, [C.c_void_p, C.c_double]),
+        "mpo_gbm_tech_from_normals": (Result, [C.POINTER(GbmCfg), C.c_int, C.c_int, _D]),
+        "mpo_european_terminal_tech_price": (Result, [C.POINTER(GbmCfg), C.c_int]),
+        "mpo_terminal_tech_from_normals": (Result, [C.POINTER(GbmCfg), C.c_int, _D]),
         "mpo_gbm_paths_from_normals": (None, [C.POINTER(GbmCfg), C.c_int, _D, _D, _D]),
         "mpo_heston_price": (Result, [C.POINTER(HestonCfg), C.c_int]),
         "mpo_heston_paths_from_normals": (None, [C.POINTER(HestonCfg), C.c_int, _D, _D, _D]),
@@ -295,17 +301,76 @@ def derive_chunk_seed(base: int, idx: int) -> int:
     return int(lib().mpo_derive_chunk_seed(base, idx))
 
 
-def gbm_price(cfg: Gbm, payoff: str = "european_call") -> tuple[float, float]:
-    """Stepwise GBM family pricer (cpu.rs:3228-3262 and siblings)."""
-    c = cfg.c()
-    r = lib().mpo_gbm_stepwise_price(C.byref(c), PAYOFF_CODES[payoff])
+TECHNIQUE_CODES = {"standard": 0, "antithetic": 1, "control_variate": 2}
+
+
+class CvMoments(C.Structure):
+    """ControlVariateMoments (cpu.rs:6202-6230)."""
+    _fields_ = [("count", C.c_uint64), ("payoff_sum", C.c_double), ("payoff_sq_sum", C.c_double),
+                ("control_sum", C.c_double), ("control_sq_sum", C.c_double),
+                ("cross_sum", C.c_double)]
+
+
+def cv_record(m: CvMoments, payoff: float, control: float) -> None:
+    lib().mpo_cv_record(C.byref(m), payoff, control)
+
+
+def summarize_cv(m: CvMoments, expectation: float) -> tuple[float, float]:
+    """summarize_control_variate (cpu.rs:7738-7771)."""
+    r = lib().mpo_summarize_cv(C.byref(m), expectation)
     return r.price, r.stderr
 
 
-def european_terminal_price(cfg: Gbm) -> tuple[float, float]:
-    """european_call_price_mc_cpu_terminal (cpu.rs:3191-3226)."""
+def summarize(n: int, total: float, sq_total: float) -> tuple[float, float]:
+    """summarize_payoffs (cpu.rs:7641-7652): population variance."""
+    r = lib().mpo_summarize(n, total, sq_total)
+    return r.price, r.stderr
+
+
+def gbm_price(cfg: Gbm, payoff: str = "european_call",
+              technique: str = "standard") -> tuple[float, float]:
+    """Stepwise GBM family pricer (cpu.rs:3228-3262 and siblings), with the
+    antithetic and control-variate variants (cpu.rs:7807-7909 and siblings)."""
     c = cfg.c()
-    r = lib().mpo_european_terminal_price(C.byref(c))
+    if technique == "standard":
+        r = lib().mpo_gbm_stepwise_price(C.byref(c), PAYOFF_CODES[payoff])
+    else:
+        r = lib().mpo_gbm_stepwise_tech_price(C.byref(c), PAYOFF_CODES[payoff],
+                                              TECHNIQUE_CODES[technique])
+    return r.price, r.stderr
+
+
+def gbm_tech_from_normals(cfg: Gbm, normals: np.ndarray, payoff: str,
+                          technique: str) -> tuple[float, float]:
+    """The technique pricer on a [unit][step] normal buffer (units = pairs for
+    antithetic, paths for the control variate), one sequential pass."""
+    normals = np.ascontiguousarray(normals, dtype=np.float64)
+    units = (cfg.n_paths + 1) // 2 if technique == "antithetic" else cfg.n_paths
+    assert normals.size == units * cfg.n_steps
+    c = cfg.c()
+    r = lib().mpo_gbm_tech_from_normals(C.byref(c), PAYOFF_CODES[payoff],
+                                        TECHNIQUE_CODES[technique], _pd(normals))
+    return r.price, r.stderr
+
+
+def european_terminal_price(cfg: Gbm, technique: str = "standard") -> tuple[float, float]:
+    """european_call_price_mc_cpu_terminal (cpu.rs:3191-3226), with the
+    antithetic (cpu.rs:7785-7805) and control-variate (cpu.rs:7837-7858) variants."""
+    c = cfg.c()
+    if technique == "standard":
+        r = lib().mpo_european_terminal_price(C.byref(c))
+    else:
+        r = lib().mpo_european_terminal_tech_price(C.byref(c), TECHNIQUE_CODES[technique])
+    return r.price, r.stderr
+
+
+def terminal_tech_from_normals(cfg: Gbm, normals: np.ndarray, technique: str) -> tuple[float, float]:
+    """The terminal technique pricer on a caller buffer, one normal per unit."""
+    normals = np.ascontiguousarray(normals, dtype=np.float64)
+    units = (cfg.n_paths + 1) // 2 if technique == "antithetic" else cfg.n_paths
+    assert normals.size == units
+    c = cfg.c()
+    r = lib().mpo_terminal_tech_from_normals(C.byref(c), TECHNIQUE_CODES[technique], _pd(normals))
     return r.price, r.stderr
 
 

@@ -359,6 +359,308 @@ MPO_EXPORT void mpo_gbm_paths_from_normals(const mpo_gbm_cfg* cfg, int payoff,
 }
 
 /* ------------------------------------------------------------------ */
+/* Variance-reduction techniques for the GBM family                    */
+/* Antithetic pairs: stepwise European cpu.rs:7807-7835 (one stream),  */
+/* Asian :6982-7017 / :7912-7938, down-and-out :8048-8100 / :8255-8288,*/
+/* lookback :8355-8390 / :8569-8600; control variate: moments          */
+/* cpu.rs:6202-6230, summary :7738-7771, chunks European :6532-6559,   */
+/* Asian :6593-6623, down-and-out :8011-8046, lookback :8392-8420.      */
+/* ------------------------------------------------------------------ */
+enum { MPO_STANDARD = 0, MPO_ANTITHETIC = 1, MPO_CONTROL_VARIATE = 2 };
+
+typedef struct {
+    uint64_t count;
+    double payoff_sum, payoff_sq_sum, control_sum, control_sq_sum, cross_sum;
+} mpo_cv_moments;
+
+static void cv_record(mpo_cv_moments* m, double payoff, double control) {
+    m->count += 1;
+    m->payoff_sum += payoff;
+    m->payoff_sq_sum += payoff * payoff;
+    m->control_sum += control;
+    m->control_sq_sum += control * control;
+    m->cross_sum += payoff * control;
+}
+
+static void cv_merge(mpo_cv_moments* m, const mpo_cv_moments* o) {
+    m->count += o->count;
+    m->payoff_sum += o->payoff_sum;
+    m->payoff_sq_sum += o->payoff_sq_sum;
+    m->control_sum += o->control_sum;
+    m->control_sq_sum += o->control_sq_sum;
+    m->cross_sum += o->cross_sum;
+}
+
+MPO_EXPORT void mpo_cv_record(mpo_cv_moments* m, double payoff, double control) {
+    cv_record(m, payoff, control);
+}
+
+/* summarize_control_variate cpu.rs:7738-7771 */
+MPO_EXPORT mpo_result mpo_summarize_cv(const mpo_cv_moments* m, double expectation) {
+    const double n = (double)m->count;
+    const double payoff_mean = m->payoff_sum / n;
+    const double control_mean = m->control_sum / n;
+    const double control_var = (m->control_sq_sum / n) - (control_mean * control_mean);
+    if (control_var <= DBL_EPSILON) return mpo_summarize(m->count, m->payoff_sum, m->payoff_sq_sum);
+    const double cov = (m->cross_sum / n) - (payoff_mean * control_mean);
+    const double beta = cov / control_var;
+    const double adjusted_mean = payoff_mean - beta * (control_mean - expectation);
+    const double adjusted_sq_mean = (m->payoff_sq_sum / n)
+        - (2.0 * beta) * ((m->cross_sum / n) - expectation * payoff_mean)
+        + (beta * beta) * ((m->control_sq_sum / n) - (2.0 * expectation * control_mean)
+                           + (expectation * expectation));
+    double adjusted_var = adjusted_sq_mean - (adjusted_mean * adjusted_mean);
+    if (!(adjusted_var > 0.0)) adjusted_var = 0.0; /* f64::max(0.0) also maps NaN to 0 */
+    mpo_result r;
+    r.price = adjusted_mean;
+    r.stderr_ = sqrt(adjusted_var) / sqrt(n);
+    return r;
+}
+
+/* A normal source: the reference stream, or a caller buffer read in order. */
+typedef struct {
+    mpo_rng g;
+    const double* buf;
+    uint64_t pos;
+} normal_src;
+
+static double src_normal(normal_src* s) {
+    if (s->buf) return s->buf[s->pos++];
+    return rng_normal(&s->g);
+}
+
+typedef struct {
+    double log_s, acc, max_s;
+    int knocked;
+} gbm_state;
+
+static void gbm_state_init(gbm_state* st, double log_s0, double s0, double barrier) {
+    st->log_s = log_s0;
+    st->acc = 0.0;
+    st->max_s = s0;
+    st->knocked = s0 <= barrier;
+}
+
+/* log_s += shock, then the payoff's observation of exp(log_s) */
+static void gbm_state_step(gbm_state* st, int payoff, double shock, double barrier) {
+    st->log_s += shock;
+    if (payoff == MPO_ASIAN) st->acc += exp(st->log_s);
+    if (payoff == MPO_DOWN_AND_OUT && exp(st->log_s) <= barrier) st->knocked = 1;
+    if (payoff == MPO_LOOKBACK) st->max_s = fmax_ref(st->max_s, exp(st->log_s));
+}
+
+static double gbm_state_payoff(const gbm_state* st, int payoff, double k, uint64_t n_steps,
+                               double discount) {
+    if (payoff == MPO_ASIAN) return fmax_ref(st->acc / (double)n_steps - k, 0.0) * discount;
+    if (payoff == MPO_LOOKBACK) return fmax_ref(st->max_s - k, 0.0) * discount;
+    if (payoff == MPO_DOWN_AND_OUT && st->knocked) return 0.0;
+    return fmax_ref(exp(st->log_s) - k, 0.0) * discount;
+}
+
+typedef struct {
+    int payoff, technique;
+    uint64_t seed, n_units, n_steps;
+    double s0, k, barrier, drift_dt, vol_dt, discount;
+    const double* normals; /* [unit][step] or NULL for the reference stream */
+    double sum, sq_sum;
+    mpo_cv_moments m;
+} gbm_tech_chunk;
+
+static void* gbm_tech_chunk_run(void* arg) {
+    gbm_tech_chunk* c = (gbm_tech_chunk*)arg;
+    normal_src src;
+    memset(&src, 0, sizeof(src));
+    src.buf = c->normals;
+    if (!src.buf) rng_init(&src.g, c->seed);
+    const double log_s0 = log(c->s0);
+    double sum = 0.0, sq = 0.0;
+    mpo_cv_moments m;
+    memset(&m, 0, sizeof(m));
+    for (uint64_t u = 0; u < c->n_units; ++u) {
+        if (c->technique == MPO_ANTITHETIC) {
+            gbm_state a, b;
+            gbm_state_init(&a, log_s0, c->s0, c->barrier);
+            gbm_state_init(&b, log_s0, c->s0, c->barrier);
+            for (uint64_t s = 0; s < c->n_steps; ++s) {
+                double z = src_normal(&src);
+                gbm_state_step(&a, c->payoff, c->drift_dt + c->vol_dt * z, c->barrier);
+                gbm_state_step(&b, c->payoff, c->drift_dt - c->vol_dt * z, c->barrier);
+            }
+            double pa = gbm_state_payoff(&a, c->payoff, c->k, c->n_steps, c->discount);
+            double pb = gbm_state_payoff(&b, c->payoff, c->k, c->n_steps, c->discount);
+            double block = 0.5 * (pa + pb);
+            sum += block;
+            sq += block * block;
+        } else {
+            gbm_state st;
+            gbm_state_init(&st, log_s0, c->s0, c->barrier);
+            for (uint64_t s = 0; s < c->n_steps; ++s) {
+                double z = src_normal(&src);
+                gbm_state_step(&st, c->payoff, c->drift_dt + c->vol_dt * z, c->barrier);
+            }
+            double s_t = exp(st.log_s);
+            double pay = gbm_state_payoff(&st, c->payoff, c->k, c->n_steps, c->discount);
+            cv_record(&m, pay, c->discount * s_t);
+        }
+    }
+    c->sum = sum;
+    c->sq_sum = sq;
+    c->m = m;
+    return NULL;
+}
+
+/* Stepwise GBM family with a technique.  European antithetic runs one
+ * stream (cpu.rs:7807-7835); every other combination is chunked like the
+ * standard pricers, over pairs (antithetic) or paths (control variate). */
+static mpo_result gbm_tech(const mpo_gbm_cfg* cfg, int payoff, int technique,
+                           const double* normals) {
+    const double dt = cfg->t / (double)cfg->n_steps;
+    const double drift_dt = (cfg->r - 0.5 * cfg->sigma * cfg->sigma) * dt;
+    const double vol_dt = cfg->sigma * sqrt(dt);
+    const double discount = exp(-cfg->r * cfg->t);
+    const uint64_t units = technique == MPO_ANTITHETIC ? (cfg->n_paths + 1) / 2 : cfg->n_paths;
+    uint64_t threads = resolved_threads(cfg->n_threads);
+    uint64_t count = (threads <= 1 || units < threads * 2000) ? 1 : threads;
+    if (technique == MPO_ANTITHETIC && payoff == MPO_EUROPEAN) count = 1;
+    if (normals) count = 1;
+    gbm_tech_chunk* ch = (gbm_tech_chunk*)calloc(count, sizeof(gbm_tech_chunk));
+    uint64_t base = units / count, rem = units % count;
+    for (uint64_t i = 0; i < count; ++i) {
+        ch[i].payoff = payoff;
+        ch[i].technique = technique;
+        ch[i].seed = count == 1 ? cfg->seed : mpo_derive_chunk_seed(cfg->seed, i);
+        ch[i].n_units = count == 1 ? units : base + (i < rem ? 1 : 0);
+        ch[i].n_steps = cfg->n_steps;
+        ch[i].s0 = cfg->s0;
+        ch[i].k = cfg->k;
+        ch[i].barrier = cfg->barrier;
+        ch[i].drift_dt = drift_dt;
+        ch[i].vol_dt = vol_dt;
+        ch[i].discount = discount;
+        ch[i].normals = normals;
+    }
+    if (count == 1)
+        gbm_tech_chunk_run(&ch[0]);
+    else
+        run_chunks(ch, sizeof(gbm_tech_chunk), count, gbm_tech_chunk_run);
+    mpo_result r;
+    if (technique == MPO_ANTITHETIC) {
+        double sum = 0.0, sq = 0.0;
+        for (uint64_t i = 0; i < count; ++i) {
+            sum += ch[i].sum;
+            sq += ch[i].sq_sum;
+        }
+        r = mpo_summarize(units, sum, sq); /* summarize_block_estimates cpu.rs:7654-7665 */
+    } else {
+        mpo_cv_moments m;
+        memset(&m, 0, sizeof(m));
+        for (uint64_t i = 0; i < count; ++i) cv_merge(&m, &ch[i].m);
+        r = mpo_summarize_cv(&m, cfg->s0);
+    }
+    free(ch);
+    return r;
+}
+
+MPO_EXPORT mpo_result mpo_gbm_stepwise_tech_price(const mpo_gbm_cfg* cfg, int payoff, int technique) {
+    if (technique == MPO_STANDARD) return mpo_gbm_stepwise_price(cfg, payoff);
+    return gbm_tech(cfg, payoff, technique, NULL);
+}
+
+/* The same pricer on a caller buffer of normals laid out [unit][step]
+ * (units = pairs for antithetic, paths for the control variate). */
+MPO_EXPORT mpo_result mpo_gbm_tech_from_normals(const mpo_gbm_cfg* cfg, int payoff, int technique,
+                                                const double* normals) {
+    return gbm_tech(cfg, payoff, technique, normals);
+}
+
+typedef struct {
+    int technique;
+    uint64_t seed, n_units;
+    double s0, k, drift_t, vol_t, discount;
+    const double* normals;
+    double sum, sq_sum;
+    mpo_cv_moments m;
+} term_tech_chunk;
+
+/* terminal antithetic cpu.rs:7785-7805 (one stream) and control variate
+ * chunk cpu.rs:6509-6530, with european_call_payoff_from_shock :7773-7783 */
+static void* term_tech_chunk_run(void* arg) {
+    term_tech_chunk* c = (term_tech_chunk*)arg;
+    normal_src src;
+    memset(&src, 0, sizeof(src));
+    src.buf = c->normals;
+    if (!src.buf) rng_init(&src.g, c->seed);
+    double sum = 0.0, sq = 0.0;
+    mpo_cv_moments m;
+    memset(&m, 0, sizeof(m));
+    for (uint64_t u = 0; u < c->n_units; ++u) {
+        double z = src_normal(&src);
+        if (c->technique == MPO_ANTITHETIC) {
+            double pa = fmax_ref(c->s0 * exp(c->drift_t + c->vol_t * z) - c->k, 0.0) * c->discount;
+            double pb = fmax_ref(c->s0 * exp(c->drift_t + c->vol_t * -z) - c->k, 0.0) * c->discount;
+            double block = 0.5 * (pa + pb);
+            sum += block;
+            sq += block * block;
+        } else {
+            double s_t = c->s0 * exp(c->drift_t + c->vol_t * z);
+            cv_record(&m, fmax_ref(s_t - c->k, 0.0) * c->discount, c->discount * s_t);
+        }
+    }
+    c->sum = sum;
+    c->sq_sum = sq;
+    c->m = m;
+    return NULL;
+}
+
+static mpo_result term_tech(const mpo_gbm_cfg* cfg, int technique, const double* normals) {
+    const double drift_t = (cfg->r - 0.5 * cfg->sigma * cfg->sigma) * cfg->t;
+    const double vol_t = cfg->sigma * sqrt(cfg->t);
+    const double discount = exp(-cfg->r * cfg->t);
+    const uint64_t units = technique == MPO_ANTITHETIC ? (cfg->n_paths + 1) / 2 : cfg->n_paths;
+    uint64_t threads = resolved_threads(cfg->n_threads);
+    uint64_t count = (threads <= 1 || units < threads * 2000) ? 1 : threads;
+    if (technique == MPO_ANTITHETIC || normals) count = 1;
+    term_tech_chunk* ch = (term_tech_chunk*)calloc(count, sizeof(term_tech_chunk));
+    uint64_t base = units / count, rem = units % count;
+    for (uint64_t i = 0; i < count; ++i) {
+        ch[i].technique = technique;
+        ch[i].seed = count == 1 ? cfg->seed : mpo_derive_chunk_seed(cfg->seed, i);
+        ch[i].n_units = count == 1 ? units : base + (i < rem ? 1 : 0);
+        ch[i].s0 = cfg->s0;
+        ch[i].k = cfg->k;
+        ch[i].drift_t = drift_t;
+        ch[i].vol_t = vol_t;
+        ch[i].discount = discount;
+        ch[i].normals = normals;
+    }
+    if (count == 1)
+        term_tech_chunk_run(&ch[0]);
+    else
+        run_chunks(ch, sizeof(term_tech_chunk), count, term_tech_chunk_run);
+    mpo_result r;
+    if (technique == MPO_ANTITHETIC) {
+        r = mpo_summarize(units, ch[0].sum, ch[0].sq_sum);
+    } else {
+        mpo_cv_moments m;
+        memset(&m, 0, sizeof(m));
+        for (uint64_t i = 0; i < count; ++i) cv_merge(&m, &ch[i].m);
+        r = mpo_summarize_cv(&m, cfg->s0);
+    }
+    free(ch);
+    return r;
+}
+
+MPO_EXPORT mpo_result mpo_european_terminal_tech_price(const mpo_gbm_cfg* cfg, int technique) {
+    if (technique == MPO_STANDARD) return mpo_european_terminal_price(cfg);
+    return term_tech(cfg, technique, NULL);
+}
+
+MPO_EXPORT mpo_result mpo_terminal_tech_from_normals(const mpo_gbm_cfg* cfg, int technique,
+                                                     const double* normals) {
+    return term_tech(cfg, technique, normals);
+}
+
+/* ------------------------------------------------------------------ */
 /* Heston: cpu.rs:8632-8910                                            */
 /* ------------------------------------------------------------------ */
 typedef struct {

@@ -36,9 +36,43 @@ ROWS = {
     "mc_cpu_american_put_lsm_rust": ("american_lsm", {"basis_degree": 2}),  # :3730
     "mc_cpu_bermudan_put_lsm_rust": (
         "bermudan_lsm", {"basis_degree": 2, "exercise_steps": [16, 32, 48, 64]}),  # :3773
+    # variance-reduction techniques (same defaults, technique set)
+    "mc_cpu_european_call_rust_antithetic": (
+        "gbm_stepwise", {"payoff": "european_call", "technique": "antithetic"}),  # :2387
+    "mc_cpu_european_call_rust_control_variate": (
+        "gbm_stepwise", {"payoff": "european_call", "technique": "control_variate"}),  # :2342
+    "mc_cpu_european_call_rust_terminal_antithetic": (
+        "european_terminal", {"technique": "antithetic"}),  # :2432
+    "mc_cpu_european_call_rust_terminal_control_variate": (
+        "european_terminal", {"technique": "control_variate"}),  # :2477
+    "mc_cpu_arithmetic_asian_call_rust_control_variate": (
+        "gbm_stepwise", {"payoff": "arithmetic_asian_call", "technique": "control_variate"}),  # :2566
+    "mc_cpu_down_and_out_call_rust_control_variate": (
+        "gbm_stepwise", {"payoff": "down_and_out_call", "barrier": 80.0,
+                         "technique": "control_variate"}),  # :3563
+    "mc_cpu_lookback_call_rust_control_variate": (
+        "gbm_stepwise", {"payoff": "lookback_call", "technique": "control_variate"}),  # :3686
     # scrambled-Sobol rows: sobol_burley 0.5 bits, not reproducible without the crate
     "mc_cpu_european_call_rust_scrambled_sobol": ("gbm_qmc", {"bridge": False}),  # :3356
     "mc_cpu_european_call_rust_scrambled_sobol_brownian_bridge": ("gbm_qmc", {"bridge": True}),
+}
+
+# stderr(technique) / stderr(standard) on one seed (harness.rs:2712-2891, :3607-3641)
+QUALITY_ROWS = {
+    "mc_cpu_european_call_rust_antithetic_quality": (
+        "gbm_stepwise", {"payoff": "european_call", "technique": "antithetic", "seed": 7001}),
+    "mc_cpu_european_call_rust_terminal_antithetic_quality": (
+        "european_terminal", {"technique": "antithetic", "seed": 7002}),
+    "mc_cpu_european_call_rust_control_variate_quality": (
+        "gbm_stepwise", {"payoff": "european_call", "technique": "control_variate", "seed": 7003}),
+    "mc_cpu_european_call_rust_terminal_control_variate_quality": (
+        "european_terminal", {"technique": "control_variate", "seed": 7004}),
+    "mc_cpu_arithmetic_asian_call_rust_control_variate_quality": (
+        "gbm_stepwise", {"payoff": "arithmetic_asian_call", "technique": "control_variate",
+                         "seed": 7111}),
+    "mc_cpu_down_and_out_call_rust_control_variate_quality": (
+        "gbm_stepwise", {"payoff": "down_and_out_call", "barrier": 80.0,
+                         "technique": "control_variate", "seed": 7311}),
 }
 
 COMMON = {"s0": 100.0, "k": 100.0, "r": 0.03, "sigma": 0.2, "t": 1.0, "n_paths": 100_000,
@@ -63,15 +97,26 @@ def main() -> None:
             rows[name] = {"pricer": kind, "config": full, "metric_name": r["metric_name"],
                           "published_value": r["metric_value"],
                           "per_iteration_us": r["per_iteration_us"]}
+    quality = {}
+    for r in release["results"]:
+        name = r["benchmark_name"]
+        if name in QUALITY_ROWS:
+            kind, cfg = QUALITY_ROWS[name]
+            full = dict(COMMON)
+            full.update(cfg)
+            full["repeats"] = 1
+            quality[name] = {"pricer": kind, "config": full, "metric_name": r["metric_name"],
+                             "published_value": r["metric_value"]}
     out = {
         "source": "montepath benchmarks/reference-fixtures.json and benchmarks/release-results.json",
+        "quality_rows": quality,
         "note": "published rows are means over seeds seed..seed+2 with n_threads=8",
         "fixtures": fx,
         "published_rows": rows,
     }
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
-    print(OUT, len(fx), "fixtures", len(rows), "rows")
+    print(OUT, len(fx), "fixtures", len(rows), "rows", len(quality), "quality rows")
 
 
 if __name__ == "__main__":

@@ -33,10 +33,10 @@ def _row_price(O, name):
             g = O.Gbm(s0=c["s0"], k=c["k"], r=c["r"], sigma=c["sigma"], t=c["t"],
                       n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed,
                       n_threads=c["n_threads"], barrier=c.get("barrier", 80.0))
-            prices.append(O.gbm_price(g, c["payoff"])[0])
+            prices.append(O.gbm_price(g, c["payoff"], c.get("technique", "standard"))[0])
         elif kind == "european_terminal":
             g = O.Gbm(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed, n_threads=c["n_threads"])
-            prices.append(O.european_terminal_price(g)[0])
+            prices.append(O.european_terminal_price(g, c.get("technique", "standard"))[0])
         elif kind == "heston":
             h = O.Heston(kappa=c["kappa"], theta=c["theta"], v0=c["v0"], xi=c["xi"], rho=c["rho"],
                          n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed,
@@ -69,6 +69,46 @@ def test_published_cpu_rows_reproduce_bit_for_bit(oracle, name):
     want = ROWS[name]["published_value"]
     # identical up to the last bit of the mean of three prices
     assert abs(got - want) <= 4 * np.spacing(want), (name, got, want)
+
+
+QUALITY = GOLD["quality_rows"]
+
+
+@pytest.mark.parametrize("name", sorted(QUALITY))
+def test_published_technique_stderr_ratios(oracle, name):
+    # stderr(technique) / stderr(standard) on one seed.  The antithetic and
+    # terminal ratios reproduce bit for bit; the stepwise control-variate
+    # standard error subtracts nearly equal terms (E[C^2] - 2 E E[C] + E^2 ~
+    # 1e4 -> 1e2), which amplifies a single-ulp libm difference in one path's
+    # exp between the published host and glibc by ~100x, so the bound is 1e-12.
+    row = QUALITY[name]
+    c = row["config"]
+    tech = c["technique"]
+    g = oracle.Gbm(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"],
+                   n_threads=c["n_threads"], barrier=c.get("barrier", 80.0))
+    if row["pricer"] == "gbm_stepwise":
+        ratio = oracle.gbm_price(g, c["payoff"], tech)[1] / oracle.gbm_price(g, c["payoff"])[1]
+    else:
+        ratio = oracle.european_terminal_price(g, tech)[1] / oracle.european_terminal_price(g)[1]
+    want = row["published_value"]
+    assert abs(ratio - want) <= 1e-12 * want, (name, ratio, want)
+
+
+def test_control_variate_summary_special_cases(oracle):
+    # a control that is an exact linear function of the payoff removes all
+    # variance; a constant control falls back to the plain summary
+    import ctypes as C
+
+    m = oracle.CvMoments()
+    xs = np.array([1.0, 3.0, 4.0, 9.0])
+    for x in xs:
+        oracle.cv_record(m, x, 2.0 * x + 1.0)
+    price, se = oracle.summarize_cv(m, 2.0 * 5.0 + 1.0)
+    assert abs(price - 5.0) < 1e-12 and se < 1e-7
+    m2 = oracle.CvMoments()
+    for x in xs:
+        oracle.cv_record(m2, x, 7.0)
+    assert oracle.summarize_cv(m2, 7.0) == oracle.summarize(len(xs), xs.sum(), (xs * xs).sum())
 
 
 def test_black_scholes_fixture(oracle):
