@@ -79,6 +79,13 @@ typedef enum mp_method {
     MP_METHOD_TERMINAL = 1
 } mp_method;
 
+/* MonteCarloTechnique (runtime/cpu.rs:24-28) */
+typedef enum mp_technique_code {
+    MP_TECHNIQUE_STANDARD = 0,
+    MP_TECHNIQUE_ANTITHETIC = 1,
+    MP_TECHNIQUE_CONTROL_VARIATE = 2
+} mp_technique_code;
+
 typedef enum mp_heston_scheme {
     MP_HESTON_REFERENCE_FLOOR = 0,  /* variance floored after each update, cpu.rs:8682 */
     MP_HESTON_FULL_TRUNCATION = 1   /* Lord et al. (2010): only v+ enters drift/diffusion */
@@ -95,7 +102,9 @@ typedef struct mp_request {
     int32_t workload;        /* mp_workload */
     int32_t precision;       /* 32: FP32 paths + FP64 accumulation; 64: FP64 throughout */
     int32_t sampling;        /* mp_sampling */
-    int32_t technique;       /* 0 = standard (the only technique offered) */
+    int32_t technique;       /* 0 standard, 1 antithetic pairs, 2 control variate (discounted
+                                S_T, one strike); MonteCarloTechnique (runtime/cpu.rs:24-28).
+                                Not offered for Merton and LSM, like the reference. */
     int32_t method;          /* mp_method (European only) */
     int32_t heston_scheme;   /* mp_heston_scheme */
     int32_t n_strikes;       /* 0 or 1: price strike k; 2..4: price strikes[] from one simulation */
@@ -216,6 +225,14 @@ int mp_hip_price_plain_fp64(mp_ctx* ctx, const mp_request* req, mp_result* out);
  * value for a basket) and undiscounted payoffs per path for strike 0.
  * Reference counterpart: the same-normals test hooks
  * *_from_f32_normals (runtime/cpu.rs:5434-5652). */
+/* Parity hook: price a request (any technique) from caller-supplied draws
+ * laid out as for mp_hip_paths_from_normals, with n_paths units (pairs for an
+ * antithetic request: ceil(n_paths / 2); the pair's second path uses -z).
+ * Reference counterpart: the technique pricers of runtime/cpu.rs:7785-7909
+ * run on one normal stream. */
+int mp_hip_price_from_normals(mp_ctx* ctx, const mp_request* req, const double* normals,
+                              const double* uniforms, const double* jump_normals, mp_result* out);
+
 int mp_hip_paths_from_normals(mp_ctx* ctx, const mp_request* req, const double* normals,
                               const double* uniforms, const double* jump_normals,
                               double* terminal_out, double* payoff_out);

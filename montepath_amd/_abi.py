@@ -138,6 +138,7 @@ SIGNATURES = {
     "mp_hip_batch_free": (None, [_P]),
     "mp_hip_price_plain_fp64": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Result)]),
     "mp_hip_paths_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, _D, _D, _D, _D]),
+    "mp_hip_price_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, _D, _D, C.POINTER(Result)]),
     "mp_hip_price_with_paths": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Result), _D, _D]),
     "mp_hip_lsm_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, C.POINTER(Result), _D,
                                           _U32]),

@@ -112,6 +112,7 @@ struct prepared {
     int allow_intrinsic;
     int degree;
     uint64_t n_steps_cfg;  // steps as configured (terminal method walks 1)
+    uint64_t n_paths_req;  // paths as requested (antithetic units are pairs)
 };
 
 }  // namespace
@@ -233,8 +234,8 @@ int prepare_request(const mp_request& r, prepared& P) {
     P.precision = r.precision == 64 ? 64 : 32;
     if (r.precision != 0 && r.precision != 32 && r.precision != 64)
         return fail(MP_E_INVALID_CONFIG, "precision must be 32 or 64");
-    if (r.technique != 0)
-        return fail(MP_E_UNSUPPORTED, "only the standard Monte Carlo technique is offered on rocm_hip");
+    if (r.technique < MP_TECH_STANDARD || r.technique > MP_TECH_CONTROL_VARIATE)
+        return fail(MP_E_INVALID_CONFIG, "technique must be 0 (standard), 1 (antithetic) or 2 (control variate)");
     if (r.sampling != MP_PSEUDORANDOM && r.sampling != MP_SCRAMBLED_SOBOL &&
         r.sampling != MP_SCRAMBLED_SOBOL_BRIDGE)
         return fail(MP_E_UNSUPPORTED, "sampling method not offered on rocm_hip");
@@ -449,6 +450,26 @@ int prepare_request(const mp_request& r, prepared& P) {
         default:
             return fail(MP_E_UNSUPPORTED, "unknown workload " + std::to_string(r.workload));
     }
+    P.n_paths_req = r.n_paths;
+    if (r.technique != MP_TECH_STANDARD) {
+        // the reference offers techniques for the GBM family, Heston and the
+        // basket (MonteCarloTechnique, cpu.rs:24-28); Merton and LSM have none
+        if (d.model == MP_MODEL_MERTON || P.kind == KIND_LSM)
+            return fail(MP_E_UNSUPPORTED, "this workload is priced with the standard technique only");
+        if (r.technique == MP_TECH_CONTROL_VARIATE && d.n_strikes > 1)
+            return fail(MP_E_INVALID_CONFIG, "the control variate prices one strike per request");
+        d.technique = r.technique;
+        if (P.kind == KIND_BASKET) {
+            // basket_control_expectation cpu.rs:7637-7639: sum of w_i S0_i
+            double e = 0.0;
+            for (int a = 0; a < r.n_assets; ++a) e += r.asset_weight[a] * r.asset_s0[a];
+            d.cv_expect = e;
+        } else {
+            d.cv_expect = s0;  // E[discount * S_T] = S0 (summarize_control_variate(.., cfg.s0))
+        }
+        // antithetic: a unit is a pair, ceil(n / 2) of them (cpu.rs:7812)
+        if (r.technique == MP_TECH_ANTITHETIC) d.n_paths = (r.n_paths + 1) / 2;
+    }
     if (P.kind == KIND_TILE || P.kind == KIND_BASKET) {
         d.tile_paths = tile_paths_for(d.n_paths, (uint64_t)d.n_steps);
         d.n_tiles = (uint32_t)((d.n_paths + d.tile_paths - 1) / d.tile_paths);
@@ -471,7 +492,7 @@ void fill_result(mp_result* out, const double* fin, const prepared& P) {
     }
     out->price = fin[0];
     out->std_error = fin[1];
-    out->n_paths = P.d.n_paths;
+    out->n_paths = P.n_paths_req;
     out->n_steps = P.n_steps_cfg;
     out->fp64_ref_error = NAN;
 }
@@ -708,7 +729,9 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     if (!ops.empty())
         HIPCHK(hipMemcpyAsync(c->plan.p, ops.data(), ops.size() * sizeof(mp_bbop), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(mp::launch_qmc(c->reqs.as<mp_dreq>(), qp, d.payoff, d.scheme, c->d_nib, c->plan.as<mp_bbop>(),
+    // techniques run in the generic kernel (payoff -1 selects it)
+    const int spec_payoff = d.technique == MP_TECH_STANDARD ? d.payoff : -1;
+    HIPCHK(mp::launch_qmc(c->reqs.as<mp_dreq>(), qp, spec_payoff, d.scheme, c->d_nib, c->plan.as<mp_bbop>(),
                           c->partial.as<double>(), term_host ? c->term.as<double>() : nullptr,
                           pay_host ? c->pay.as<double>() : nullptr, precision, grid, block, nib_lds,
                           c->stream));
@@ -1243,8 +1266,9 @@ MP_EXPORT int mp_hip_price_plain_fp64(mp_ctx* c, const mp_request* req, mp_resul
     int rc = prepare_request(*req, P);
     if (rc != MP_OK) return rc;
     if (P.kind != KIND_TILE || P.d.model != MP_MODEL_GBM ||
-        (P.d.payoff != MP_PAY_EUROPEAN && P.d.payoff != MP_PAY_ASIAN) || P.d.n_strikes != 1)
-        return fail(MP_E_UNSUPPORTED, "the plain FP64 path prices single-strike GBM European/Asian calls");
+        (P.d.payoff != MP_PAY_EUROPEAN && P.d.payoff != MP_PAY_ASIAN) || P.d.n_strikes != 1 ||
+        P.d.technique != MP_TECH_STANDARD)
+        return fail(MP_E_UNSUPPORTED, "the plain FP64 path prices single-strike GBM European/Asian calls (standard technique)");
     mp_dreq d = P.d;
     d.discount = std::exp(-req->r * req->t);
     const uint64_t n = d.n_paths;
@@ -1303,6 +1327,24 @@ MP_EXPORT int mp_hip_paths_from_normals(mp_ctx* c, const mp_request* req, const 
     if (P.d.model == MP_MODEL_MERTON && (!uniforms || !jump_normals))
         return fail(MP_E_INVALID_CONFIG, "Merton needs uniforms and jump_normals buffers");
     return run_tiles_single(c, P, prec, &res, normals, uniforms, jump_normals, terminal_out, payoff_out);
+}
+
+MP_EXPORT int mp_hip_price_from_normals(mp_ctx* c, const mp_request* req, const double* normals,
+                                        const double* uniforms, const double* jump_normals,
+                                        mp_result* out) {
+    if (!c || !req || !normals || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    if (req->sampling != MP_PSEUDORANDOM)
+        return fail(MP_E_INCOMPATIBLE_INPUT, "normal buffers replace pseudorandom draws only");
+    if (P.kind == KIND_BASKET) return run_basket(c, *req, P, P.precision, normals, out, nullptr, nullptr);
+    if (P.kind != KIND_TILE)
+        return fail(MP_E_INCOMPATIBLE_INPUT, "use mp_hip_lsm_from_normals for LSM requests");
+    if (P.d.model == MP_MODEL_MERTON && (!uniforms || !jump_normals))
+        return fail(MP_E_INVALID_CONFIG, "Merton needs uniforms and jump_normals buffers");
+    return run_tiles_single(c, P, P.precision, out, normals, uniforms, jump_normals, nullptr, nullptr);
 }
 
 MP_EXPORT int mp_hip_price_with_paths(mp_ctx* c, const mp_request* req, mp_result* out,

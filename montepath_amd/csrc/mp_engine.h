@@ -7,6 +7,9 @@
 #define MP_NS_MAX 4            // strikes priced from one simulation
 #define MP_POISSON_TERMS 24    // cached cumulative Poisson probabilities per request
 #define MP_PARTIAL_STRIDE (2 * MP_NS_MAX)  // doubles per tile partial: (sum, sumsq) per strike
+// Control-variate tiles use the first five slots of a partial row for the
+// discounted moments sum P, sum P^2, sum C, sum C^2, sum P*C (one strike).
+enum mp_technique { MP_TECH_STANDARD = 0, MP_TECH_ANTITHETIC = 1, MP_TECH_CONTROL_VARIATE = 2 };
 // LSM kernels run at most this many blocks: the last block reads every
 // block's partial sums in one round trip (two rows per thread).
 #define MP_LSM_MAX_BLOCKS 512
@@ -33,9 +36,10 @@ struct mp_dreq {
     double drift_dt, vol_dt;  // GBM/Merton per-step log drift (compensated for Merton) and vol
     double log_barrier_rel;   // ln(B) - ln(S0): knock-out level in log-return units
     int32_t knocked_at_start; // S0 <= B
-    int32_t pad0;
+    int32_t technique;        // 0 standard, 1 antithetic pairs, 2 control variate
     double strikes[MP_NS_MAX];
     double discount;
+    double cv_expect;         // E[discounted control] (control variate)
     double v0, kappa, theta, xi, rho, rho_perp, rate, dt, sqrt_dt;
     double mu_j, sigma_j, jump_mean_dt;
     double cum[MP_POISSON_TERMS];

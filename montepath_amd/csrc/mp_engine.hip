@@ -197,6 +197,38 @@ __device__ __forceinline__ unsigned int next_queue_position(unsigned int* counte
     return __builtin_amdgcn_readfirstlane(*slot);
 }
 
+// Adds one unit's contribution to the FP64 sums.  Standard: the payoff per
+// strike.  Antithetic: the pair estimate (payoff(+z) + payoff(-z)) / 2 per
+// strike (the reference's block estimate, cpu.rs:7807-7835).  Control
+// variate (one strike): discounted payoff P and control C = discount * S_T
+// into sum[0] (P), sq[0] (P^2), sum[1] (C), sq[1] (C^2), sum[2] (P*C), which
+// land in partial slots 0..4 (moments of cpu.rs:6202-6230).
+template <class R>
+__device__ __forceinline__ void accumulate_unit(const path_state<R>& st, const path_state<R>& sa,
+                                                const model_params<R>& m, const mp_dreq& q,
+                                                int tech, int n_steps, R s0, const R* kk, int ns,
+                                                double* sum, double* sq) {
+    if (tech == MP_TECH_CONTROL_VARIATE) {
+        const double pd = q.discount * (double)payoff_of(st, m, q, n_steps, s0, kk[0]);
+        const double cd = q.discount * (double)(s0 * mp_exp(st.x));
+        sum[0] += pd;
+        sq[0] += pd * pd;
+        sum[1] += cd;
+        sq[1] += cd * cd;
+        sum[2] += pd * cd;
+        return;
+    }
+#pragma unroll
+    for (int sidx = 0; sidx < MP_NS_MAX; ++sidx) {
+        if (sidx < ns) {
+            double pay = (double)payoff_of(st, m, q, n_steps, s0, kk[sidx]);
+            if (tech == MP_TECH_ANTITHETIC) pay = 0.5 * (pay + (double)payoff_of(sa, m, q, n_steps, s0, kk[sidx]));
+            sum[sidx] += pay;
+            sq[sidx] += pay * pay;
+        }
+    }
+}
+
 // One tile of one request.  MODEL / PAY / SCH >= 0 compile the model,
 // payoff and Heston scheme in (the request must hold those values); -1 reads
 // them from the request at run time.
@@ -207,12 +239,15 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
     if constexpr (MODEL >= 0) m.model = MODEL;
     if constexpr (PAY >= 0) m.payoff = PAY;
     if constexpr (SCH >= 0) m.scheme = SCH;
+    // specialised bodies run the standard technique only
+    const int tech = MODEL < 0 ? q.technique : MP_TECH_STANDARD;
     const int n_steps = q.n_steps;
     const int ns = q.n_strikes;
     const R s0 = R(q.s0);
     R kk[MP_NS_MAX];
 #pragma unroll
     for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = R(q.strikes[j]);
+    // unit = path, or antithetic pair (the pair's second path takes -z)
     const uint64_t pbeg = (uint64_t)(tile - q.tile0) * q.tile_paths;
     const uint64_t pend = min(pbeg + (uint64_t)q.tile_paths, q.n_paths);
 
@@ -221,12 +256,13 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
     for (int j = 0; j < MP_NS_MAX; ++j) sum[j] = sq[j] = 0.0;
 
     for (uint64_t base = pbeg; base < pend; base += (uint64_t)MP_WAVE * PPL) {
-        path_state<R> st[PPL];
+        path_state<R> st[PPL], sa[PPL];
         uint64_t pid[PPL];
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             pid[j] = base + lane + (uint64_t)MP_WAVE * j;
             state_init(st[j], q);
+            state_init(sa[j], q);
         }
         for (int s0i = 0; s0i < n_steps; s0i += 4) {
             const uint32_t blk = (uint32_t)(s0i >> 2);
@@ -245,8 +281,12 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
                             z[e] = e < nsub ? R(L.normals[(uint64_t)(s0i + e) * q.n_paths + pr]) : R(0);
                     }
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (e < nsub) gbm_step(st[j], m, z[e]);
+                    for (int e = 0; e < 4; ++e) {
+                        if (e < nsub) {
+                            gbm_step(st[j], m, z[e]);
+                            if (tech == MP_TECH_ANTITHETIC) gbm_step(sa[j], m, -z[e]);
+                        }
+                    }
                 } else if (m.model == MP_MODEL_HESTON) {
                     R zv[4], zi[4];
                     if constexpr (SRC == 0) {
@@ -261,9 +301,13 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
                         }
                     }
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (e < nsub) heston_step(st[j], m, zv[e], zi[e]);
-                } else {  // Merton
+                    for (int e = 0; e < 4; ++e) {
+                        if (e < nsub) {
+                            heston_step(st[j], m, zv[e], zi[e]);
+                            if (tech == MP_TECH_ANTITHETIC) heston_step(sa[j], m, -zv[e], -zi[e]);
+                        }
+                    }
+                } else {  // Merton (standard technique only)
                     R z[4];
                     double u[4];
                     if constexpr (SRC == 0) {
@@ -300,14 +344,7 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             if (pid[j] < pend) {
-#pragma unroll
-                for (int sidx = 0; sidx < MP_NS_MAX; ++sidx) {
-                    if (sidx < ns) {
-                        const double pay = (double)payoff_of(st[j], m, q, n_steps, s0, kk[sidx]);
-                        sum[sidx] += pay;
-                        sq[sidx] += pay * pay;
-                    }
-                }
+                accumulate_unit(st[j], sa[j], m, q, tech, n_steps, s0, kk, ns, sum, sq);
                 if (L.terminal_out) L.terminal_out[pid[j]] = (double)(s0 * mp_exp(st[j].x));
                 if (L.payoff_out)
                     L.payoff_out[pid[j]] = (double)payoff_of(st[j], m, q, n_steps, s0, kk[0]);
@@ -346,7 +383,9 @@ __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
         if constexpr (sizeof(R) == 4 && SRC == 0) {
             // FP32 Philox tiles (batches, serving): a body per model and payoff
             const int mdl = q.model, pay = q.payoff;
-            if (mdl == MP_MODEL_GBM) {
+            if (q.technique != MP_TECH_STANDARD) {
+                tile_body<R, SRC, PPL, -1, -1, -1>(L, tile, q, lane);
+            } else if (mdl == MP_MODEL_GBM) {
                 if (pay == MP_PAY_EUROPEAN)
                     tile_body<R, SRC, PPL, MP_MODEL_GBM, MP_PAY_EUROPEAN, -1>(L, tile, q, lane);
                 else if (pay == MP_PAY_ASIAN)
@@ -380,28 +419,57 @@ __global__ void reset_kernel(unsigned int* counter, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// Finalize: one block per request, one wave per strike; tiles summed in tile
-// order (Neumaier), then price = discount * mean and the population-variance
-// standard error of the reference's summarize_payoffs (cpu.rs:7641-7652), all
-// in FP64.
-// ---------------------------------------------------------------------------
+// Finalize: one block per request; wave w sums partial columns 2w and 2w+1
+// over the request's tiles in tile order (Neumaier), then one thread forms
+// the result in FP64: price = discount * mean and the population-variance
+// standard error of the reference's summarize_payoffs (cpu.rs:7641-7652) per
+// strike (pairs for antithetic requests: summarize_block_estimates
+// cpu.rs:7654-7665), or for control-variate requests the beta-adjusted
+// estimate of summarize_control_variate (cpu.rs:7738-7771) from moments that
+// are already discounted.
+__device__ void control_variate_summary(const double* tot, double n, double expectation,
+                                        double* price, double* se) {
+    const double payoff_mean = tot[0] / n;
+    const double control_mean = tot[2] / n;
+    const double control_var = (tot[3] / n) - (control_mean * control_mean);
+    if (control_var <= 2.220446049250313e-16) {  // f64::EPSILON: plain summary
+        const double var = tot[1] / n - payoff_mean * payoff_mean;
+        *price = payoff_mean;
+        *se = sqrt(var > 0.0 ? var : 0.0) / sqrt(n);
+        return;
+    }
+    const double cov = (tot[4] / n) - (payoff_mean * control_mean);
+    const double beta = cov / control_var;
+    const double adjusted_mean = payoff_mean - beta * (control_mean - expectation);
+    const double adjusted_sq_mean = (tot[1] / n)
+        - (2.0 * beta) * ((tot[4] / n) - expectation * payoff_mean)
+        + (beta * beta) * ((tot[3] / n) - (2.0 * expectation * control_mean)
+                           + (expectation * expectation));
+    const double v = adjusted_sq_mean - adjusted_mean * adjusted_mean;
+    *price = adjusted_mean;
+    *se = sqrt(v > 0.0 ? v : 0.0) / sqrt(n);
+}
+
 __global__ __launch_bounds__(64 * MP_NS_MAX) void finalize_kernel(const mp_dreq* reqs,
                                                                   const double* partial,
                                                                   double* out) {
     __shared__ double red[MP_NS_MAX][2][MP_WAVE];
+    __shared__ double tot[MP_PARTIAL_STRIDE];
     const mp_dreq& q = reqs[blockIdx.x];
     const int lane = threadIdx.x & (MP_WAVE - 1);
-    const int j = threadIdx.x >> 6;  // one wave per strike
+    const int w = threadIdx.x >> 6;
     const double n = (double)q.n_paths;
-    // Lane l adds tiles l, l + 64, ... in that order; sixteen tiles' (sum,
-    // sum of squares) pairs are loaded before any is added.
+    const bool cv = q.technique == MP_TECH_CONTROL_VARIATE;
+    const int n_cols = cv ? 5 : 2 * q.n_strikes;
+    // Lane l adds tiles l, l + 64, ... in that order; sixteen tiles' column
+    // pairs are loaded before any is added.
     mp_neumaier a, b;
     a.init();
     b.init();
-    const double2* col = reinterpret_cast<const double2*>(partial + (uint64_t)q.tile0 * MP_PARTIAL_STRIDE) + j;
+    const double2* col = reinterpret_cast<const double2*>(partial + (uint64_t)q.tile0 * MP_PARTIAL_STRIDE) + w;
     constexpr int kRow2 = MP_PARTIAL_STRIDE / 2;  // double2 per tile row
     constexpr int kAhead = 16;
-    if (j < q.n_strikes) {
+    if (2 * w < n_cols) {
         for (uint32_t t0 = lane; t0 < q.n_tiles; t0 += MP_WAVE * kAhead) {
             double2 v[kAhead];
 #pragma unroll
@@ -418,26 +486,34 @@ __global__ __launch_bounds__(64 * MP_NS_MAX) void finalize_kernel(const mp_dreq*
             }
         }
     }
-    red[j][0][lane] = a.value();
-    red[j][1][lane] = b.value();
+    red[w][0][lane] = a.value();
+    red[w][1][lane] = b.value();
     __syncthreads();
     if (lane == 0) {
         mp_neumaier s, s2;
         s.init();
         s2.init();
         for (int i = 0; i < MP_WAVE; ++i) {
-            s.add(red[j][0][i]);
-            s2.add(red[j][1][i]);
+            s.add(red[w][0][i]);
+            s2.add(red[w][1][i]);
         }
-        double price = 0.0, se = 0.0;
-        if (j < q.n_strikes) {
-            const double mean = s.value() / n;
-            const double var = s2.value() / n - mean * mean;
-            price = q.discount * mean;
-            se = q.discount * sqrt(var > 0.0 ? var : 0.0) / sqrt(n);
+        tot[2 * w] = s.value();
+        tot[2 * w + 1] = s2.value();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* o = out + (uint64_t)blockIdx.x * MP_PARTIAL_STRIDE;
+        for (int j = 0; j < MP_PARTIAL_STRIDE; ++j) o[j] = 0.0;
+        if (cv) {
+            control_variate_summary(tot, n, q.cv_expect, &o[0], &o[1]);
+        } else {
+            for (int j = 0; j < q.n_strikes; ++j) {
+                const double mean = tot[2 * j] / n;
+                const double var = tot[2 * j + 1] / n - mean * mean;
+                o[2 * j] = q.discount * mean;
+                o[2 * j + 1] = q.discount * sqrt(var > 0.0 ? var : 0.0) / sqrt(n);
+            }
         }
-        out[(uint64_t)blockIdx.x * MP_PARTIAL_STRIDE + 2 * j] = price;
-        out[(uint64_t)blockIdx.x * MP_PARTIAL_STRIDE + 2 * j + 1] = se;
     }
 }
 
@@ -563,6 +639,8 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
     model_params<R> m = load_params<R>(q);
     if constexpr (PAY >= 0) m.payoff = PAY;
     if constexpr (SCH >= 0) m.scheme = SCH;
+    // specialised kernels run the standard technique only
+    const int tech = PAY < 0 ? q.technique : MP_TECH_STANDARD;
     const int n_steps = q.n_steps;
     const int ns = q.n_strikes;
     const R s0 = R(q.s0);
@@ -582,25 +660,32 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
         sobol_cursor<R, true> cur;
         cur.init(nib, qp.base_seed + (uint32_t)(tile_base >> 16) * 0x9E3779B9u, (uint32_t)(p & 0xFFFFu));
 
-        path_state<R> st;
+        // antithetic requests: the unit is a pair, its second path (sa) takes
+        // the negated draws -- for the bridge, the negated increments
+        path_state<R> st, sa;
         state_init(st, q);
+        state_init(sa, q);
         // FP64 follows the reference (normals z = dW / sqrt(dt), then z
         // sqrt(dt) inside the step); FP32 feeds the increments directly.
         constexpr bool kInc = sizeof(R) == 4;
         const R to_step = kInc ? R(1) : inv_sqrt_dt;  // bridge increment -> step input
         const R z_to_step = kInc ? m.sqrt_dt : R(1);  // normal -> step input
-        auto emit = [&](R i0, R i1) {
+        auto step1 = [&](path_state<R>& x, R i0, R i1) {
             if constexpr (F == 1) {
                 if constexpr (kInc)
-                    gbm_step_dw(st, m, i0);
+                    gbm_step_dw(x, m, i0);
                 else
-                    gbm_step(st, m, i0);
+                    gbm_step(x, m, i0);
             } else {
                 if constexpr (kInc)
-                    heston_step_dw(st, m, i0, i1);
+                    heston_step_dw(x, m, i0, i1);
                 else
-                    heston_step(st, m, i0, i1);
+                    heston_step(x, m, i0, i1);
             }
+        };
+        auto emit = [&](R i0, R i1) {
+            step1(st, i0, i1);
+            if (tech == MP_TECH_ANTITHETIC) step1(sa, -i0, -i1);
         };
         if (!qp.bridge) {
             // step s uses dimensions s*F + f; 256/F steps per offset
@@ -681,13 +766,15 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
                 }
             }
         }
+        double sum[MP_NS_MAX], sq[MP_NS_MAX];
+#pragma unroll
+        for (int j = 0; j < MP_NS_MAX; ++j) sum[j] = sq[j] = 0.0;
+        if (active) accumulate_unit(st, sa, m, q, tech, n_steps, s0, kk, ns, sum, sq);
         double v[2 * MP_NS_MAX];
 #pragma unroll
         for (int j = 0; j < MP_NS_MAX; ++j) {
-            double pay = 0.0;
-            if (active && j < ns) pay = (double)payoff_of(st, m, q, n_steps, s0, kk[j]);
-            v[2 * j] = pay;
-            v[2 * j + 1] = pay * pay;
+            v[2 * j] = sum[j];
+            v[2 * j + 1] = sq[j];
         }
         if (active) {
             if (terminal_out) terminal_out[p] = (double)(s0 * mp_exp(st.x));
@@ -724,13 +811,15 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
     }
     const int n_steps = q.n_steps;
     const R kk = R(q.strikes[0]);
+    const int tech = q.technique;
     for (uint32_t tile = blockIdx.x; tile < q.n_tiles; tile += gridDim.x) {
+        // unit = path, or antithetic pair (second path xa takes -z)
         const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
         const bool active = p < q.n_paths;
         const uint64_t pr = active ? p : q.n_paths - 1;
-        R x[NA];
+        R x[NA], xa[NA];
 #pragma unroll
-        for (int a = 0; a < NA; ++a) x[a] = R(0);
+        for (int a = 0; a < NA; ++a) x[a] = xa[a] = R(0);
         for (int s0i = 0; s0i < n_steps; s0i += 4) {
             const int nsub = min(4, n_steps - s0i);
             R z[NA][4];
@@ -754,23 +843,46 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
 #pragma unroll
                         for (int b = 0; b <= a; ++b) c += L[a][b] * z[b][e];
                         x[a] += dr[a] + vo[a] * c;
+                        // the correlated shock of -z is exactly -c
+                        if (tech == MP_TECH_ANTITHETIC) xa[a] += dr[a] + vo[a] * -c;
                     }
                 }
             }
         }
-        R basket = R(0);
+        R basket = R(0), basket_a = R(0);
 #pragma unroll
-        for (int a = 0; a < NA; ++a) basket += w[a] * (s0a[a] * mp_exp(x[a]));
+        for (int a = 0; a < NA; ++a) {
+            basket += w[a] * (s0a[a] * mp_exp(x[a]));
+            if (tech == MP_TECH_ANTITHETIC) basket_a += w[a] * (s0a[a] * mp_exp(xa[a]));
+        }
         const R pay = basket > kk ? basket - kk : R(0);
-        double v[2];
-        v[0] = active ? (double)pay : 0.0;
-        v[1] = v[0] * v[0];
+        // standard: payoff, payoff^2; antithetic: the pair estimate; control
+        // variate: discounted payoff and control (discount * basket at T) moments
+        double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (active) {
+            if (tech == MP_TECH_CONTROL_VARIATE) {
+                const double pd = q.discount * (double)pay, cd = q.discount * (double)basket;
+                v[0] = pd;
+                v[1] = pd * pd;
+                v[2] = cd;
+                v[3] = cd * cd;
+                v[4] = pd * cd;
+            } else {
+                double est = (double)pay;
+                if (tech == MP_TECH_ANTITHETIC)
+                    est = 0.5 * (est + (double)(basket_a > kk ? basket_a - kk : R(0)));
+                v[0] = est;
+                v[1] = est * est;
+            }
             if (terminal_out) terminal_out[p] = (double)basket;
             if (payoff_out) payoff_out[p] = (double)pay;
         }
-        double vv[2] = {v[0], v[1]};
-        block_sum_store<2>(vv, partial + (uint64_t)tile * MP_PARTIAL_STRIDE);
+        if (tech == MP_TECH_CONTROL_VARIATE) {
+            block_sum_store<5>(v, partial + (uint64_t)tile * MP_PARTIAL_STRIDE);
+        } else {
+            double vv[2] = {v[0], v[1]};
+            block_sum_store<2>(vv, partial + (uint64_t)tile * MP_PARTIAL_STRIDE);
+        }
     }
 }
 

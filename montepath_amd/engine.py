@@ -32,6 +32,10 @@ SAMPLING = {
 }
 
 
+# MonteCarloTechnique (runtime/cpu.rs:24-28)
+TECHNIQUES = {"standard": 0, "antithetic": 1, "control_variate": 2}
+
+
 def make_request(workload: str, **kw) -> A.Request:
     """Build an mp_request.  Unknown keys raise; omitted keys keep the
     reference defaults where the reference has one."""
@@ -42,7 +46,8 @@ def make_request(workload: str, **kw) -> A.Request:
     r.precision = int(kw.pop("precision", 32))
     sampling = kw.pop("sampling", "pseudorandom")
     r.sampling = SAMPLING[sampling] if isinstance(sampling, str) else int(sampling)
-    r.technique = int(kw.pop("technique", 0))
+    technique = kw.pop("technique", "standard")
+    r.technique = TECHNIQUES[technique] if isinstance(technique, str) else int(technique)
     method = kw.pop("method", "stepwise")
     r.method = A.METHOD_TERMINAL if method in ("terminal", "terminal_distribution", 1) else A.METHOD_STEPWISE
     scheme = kw.pop("heston_scheme", "reference_floor")
@@ -214,6 +219,18 @@ class Engine:
             self._ctx, C.byref(req), _pd(normals), None if u is None else _pd(u),
             None if j is None else _pd(j), _pd(term), _pd(pay)))
         return term, pay
+
+    def price_from_normals(self, req: A.Request, normals, uniforms=None, jump_normals=None):
+        """Price from caller draws (step-major, n_paths units: pairs for an
+        antithetic request), any technique."""
+        normals = np.ascontiguousarray(normals, dtype=np.float64)
+        u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64)
+        j = None if jump_normals is None else np.ascontiguousarray(jump_normals, dtype=np.float64)
+        res = A.Result()
+        A.check(self.lib.mp_hip_price_from_normals(
+            self._ctx, C.byref(req), _pd(normals), None if u is None else _pd(u),
+            None if j is None else _pd(j), C.byref(res)))
+        return PriceResult.of(res)
 
     def price_with_paths(self, req: A.Request):
         """Price and return (result, terminal values, undiscounted payoffs)."""

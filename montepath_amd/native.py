@@ -137,9 +137,6 @@ def _engine_sampling(s: str) -> str:
     return s
 
 
-def _engine_technique(t: str) -> None:
-    if t != "standard":
-        raise ValueError(f"rocm_hip offers the standard technique only, not '{t}'")
 
 
 # ---------------------------------------------------------------- response
@@ -193,9 +190,9 @@ def _gbm_family(workload: str, function: str, cfg: Mapping, barrier: bool = Fals
     c["sampling"] = _sampling(cfg)
     if barrier:
         c["barrier"] = _f(cfg, ["barrier"], 80.0)
-    _engine_technique(c["technique"])
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], s0=c["s0"], k=c["k"],
-              r=c["r"], sigma=c["sigma"], t=c["t"], sampling=_engine_sampling(c["sampling"]))
+              r=c["r"], sigma=c["sigma"], t=c["t"], sampling=_engine_sampling(c["sampling"]),
+              technique=c["technique"])
     if barrier:
         kw["barrier"] = c["barrier"]
     ek = _engine_keys(cfg)
@@ -246,7 +243,6 @@ def price_heston_european_call_rocm(config: Mapping) -> dict:
         "n_steps": _u(config, ["n_steps", "steps"], 252), "seed": _u(config, ["seed"], 42),
         "n_threads": _u(config, ["n_threads"], 0), "technique": _technique(config),
     }
-    _engine_technique(c["technique"])
     scheme = _normalize(str(config.get("scheme", "reference_floor")))
     if scheme not in ("reference_floor", "full_truncation"):
         raise ValueError(f"unsupported Heston scheme '{scheme}'")
@@ -255,7 +251,8 @@ def price_heston_european_call_rocm(config: Mapping) -> dict:
     c["sampling"] = sampling
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], s0=c["s0"], k=c["k"],
               r=c["r"], t=c["t"], v0=c["v0"], kappa=c["kappa"], theta=c["theta"],
-              xi=c["vol_of_vol"], rho=c["rho"], heston_scheme=scheme, sampling=sampling)
+              xi=c["vol_of_vol"], rho=c["rho"], heston_scheme=scheme, sampling=sampling,
+              technique=c["technique"])
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)
@@ -290,7 +287,6 @@ def price_basket_call_rocm(config: Mapping) -> dict:
     asset_weight / correlation lists.  n_steps (default 1) walks the assets
     stepwise; one step is the reference's terminal basket."""
     technique = _technique(config)
-    _engine_technique(technique)
     sampling = _engine_sampling(_sampling(config))
     if sampling != "pseudorandom":
         raise ValueError("rocm_hip prices baskets with pseudorandom draws only")
@@ -319,7 +315,8 @@ def price_basket_call_rocm(config: Mapping) -> dict:
         "n_threads": _u(config, ["n_threads"], 0), "technique": technique, "sampling": sampling,
     }
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], k=c["k"], r=c["r"],
-              t=c["t"], asset_s0=s0, asset_sigma=sig, asset_weight=w, correlation=corr)
+              t=c["t"], asset_s0=s0, asset_sigma=sig, asset_weight=w, correlation=corr,
+              technique=technique)
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)

@@ -124,6 +124,11 @@ def lib() -> C.CDLL:
         "mpo_terminal_tech_from_normals": (Result, [C.POINTER(GbmCfg), C.c_int, _D]),
         "mpo_gbm_paths_from_normals": (None, [C.POINTER(GbmCfg), C.c_int, _D, _D, _D]),
         "mpo_heston_price": (Result, [C.POINTER(HestonCfg), C.c_int]),
+        "mpo_heston_tech_price": (Result, [C.POINTER(HestonCfg), C.c_int, C.c_int]),
+        "mpo_heston_tech_from_normals": (Result, [C.POINTER(HestonCfg), C.c_int, C.c_int, _D]),
+        "mpo_basket2_tech_price": (Result, [C.c_void_p, C.c_int]),
+        "mpo_basket_tech_from_normals": (Result, [C.c_uint64, _D, _D, _D, _D, C.c_double, C.c_double,
+                                                  C.c_double, C.c_uint64, C.c_uint64, C.c_int, _D]),
         "mpo_heston_paths_from_normals": (None, [C.POINTER(HestonCfg), C.c_int, _D, _D, _D]),
         "mpo_merton_price": (Result, [C.POINTER(MertonCfg)]),
         "mpo_poisson_from_uniform": (C.c_uint64, [C.c_double, C.c_double]),
@@ -386,10 +391,27 @@ def gbm_paths_from_normals(cfg: Gbm, normals: np.ndarray, payoff: str = "europea
     return term, pay
 
 
-def heston_price(cfg: Heston, scheme: int = HESTON_REFERENCE_FLOOR) -> tuple[float, float]:
-    """heston_european_call_price_mc_standard (cpu.rs:8869-8881)."""
+def heston_price(cfg: Heston, scheme: int = HESTON_REFERENCE_FLOOR,
+                 technique: str = "standard") -> tuple[float, float]:
+    """heston_european_call_price_mc_standard (cpu.rs:8869-8881) and the
+    antithetic / control-variate drivers (cpu.rs:8883-8910)."""
     c = cfg.c()
-    r = lib().mpo_heston_price(C.byref(c), scheme)
+    if technique == "standard":
+        r = lib().mpo_heston_price(C.byref(c), scheme)
+    else:
+        r = lib().mpo_heston_tech_price(C.byref(c), scheme, TECHNIQUE_CODES[technique])
+    return r.price, r.stderr
+
+
+def heston_tech_from_normals(cfg: Heston, normals: np.ndarray, scheme: int,
+                             technique: str) -> tuple[float, float]:
+    """Heston technique pricer on a [unit][step][2] buffer of (z_var, z_ind)."""
+    normals = np.ascontiguousarray(normals, dtype=np.float64)
+    units = (cfg.n_paths + 1) // 2 if technique == "antithetic" else cfg.n_paths
+    assert normals.size == units * cfg.n_steps * 2
+    c = cfg.c()
+    r = lib().mpo_heston_tech_from_normals(C.byref(c), scheme, TECHNIQUE_CODES[technique],
+                                           _pd(normals))
     return r.price, r.stderr
 
 
@@ -434,10 +456,29 @@ def merton_series(cfg: Merton, max_terms: int = 96, tail_tol: float = 1e-12,
 
 
 def basket2_price(s01=100.0, s02=100.0, k=100.0, r=0.03, sigma1=0.2, sigma2=0.25, rho=0.5,
-                  weight1=0.5, weight2=0.5, t=1.0, n_paths=100_000, seed=42):
-    """simulate_basket_pseudorandom_standard (cpu.rs:7503-7518)."""
+                  weight1=0.5, weight2=0.5, t=1.0, n_paths=100_000, seed=42,
+                  technique="standard"):
+    """simulate_basket_pseudorandom_standard (cpu.rs:7503-7518) and its
+    antithetic / control-variate variants (cpu.rs:7520-7553)."""
     c = Basket2Cfg(s01, s02, k, r, sigma1, sigma2, rho, weight1, weight2, t, n_paths, seed)
-    res = lib().mpo_basket2_price(C.byref(c))
+    if technique == "standard":
+        res = lib().mpo_basket2_price(C.byref(c))
+    else:
+        res = lib().mpo_basket2_tech_price(C.byref(c), TECHNIQUE_CODES[technique])
+    return res.price, res.stderr
+
+
+def basket_tech_from_normals(s0, sigma, weight, corr, r, t, k, n_steps, n_units, technique,
+                             normals):
+    """n-asset stepped basket priced with a technique from [unit][step][asset]
+    normals (antithetic units are pairs)."""
+    s0, sigma, weight = (np.ascontiguousarray(x, dtype=np.float64) for x in (s0, sigma, weight))
+    chol = cholesky(np.asarray(corr, dtype=np.float64))
+    normals = np.ascontiguousarray(normals, dtype=np.float64)
+    assert normals.size == n_units * n_steps * len(s0)
+    res = lib().mpo_basket_tech_from_normals(len(s0), _pd(s0), _pd(sigma), _pd(weight), _pd(chol),
+                                             r, t, k, n_steps, n_units, TECHNIQUE_CODES[technique],
+                                             _pd(normals))
     return res.price, res.stderr
 
 

@@ -782,6 +782,108 @@ MPO_EXPORT void mpo_heston_paths_from_normals(const mpo_heston_cfg* cfg, int sch
     }
 }
 
+/* Heston with a technique: antithetic chunk cpu.rs:8714-8762 (pairs, the
+ * second path takes -z_var, -z_ind), control variate chunk cpu.rs:8764-8776
+ * with heston_discounted_payoff :8688-8698 (control = S_T * discount);
+ * drivers :8883-8910. */
+typedef struct {
+    heston_params h;
+    int technique;
+    uint64_t seed, n_units;
+    const double* normals; /* [unit][step][2] or NULL for the reference stream */
+    double sum, sq_sum;
+    mpo_cv_moments m;
+} heston_tech_chunk;
+
+static void* heston_tech_chunk_run(void* arg) {
+    heston_tech_chunk* c = (heston_tech_chunk*)arg;
+    const heston_params* h = &c->h;
+    normal_src src;
+    memset(&src, 0, sizeof(src));
+    src.buf = c->normals;
+    if (!src.buf) rng_init(&src.g, c->seed);
+    double sum = 0.0, sq = 0.0;
+    mpo_cv_moments m;
+    memset(&m, 0, sizeof(m));
+    for (uint64_t u = 0; u < c->n_units; ++u) {
+        if (c->technique == MPO_ANTITHETIC) {
+            double log_a = h->log_s0, var_a = h->v0, log_b = h->log_s0, var_b = h->v0;
+            for (uint64_t s = 0; s < h->n_steps; ++s) {
+                double z_var = src_normal(&src);
+                double z_ind = src_normal(&src);
+                heston_step(h, z_var, z_ind, &log_a, &var_a);
+                heston_step(h, -z_var, -z_ind, &log_b, &var_b);
+            }
+            double pa = fmax_ref(exp(log_a) - h->strike, 0.0) * h->discount;
+            double pb = fmax_ref(exp(log_b) - h->strike, 0.0) * h->discount;
+            double block = 0.5 * (pa + pb);
+            sum += block;
+            sq += block * block;
+        } else {
+            double log_s = h->log_s0, v = h->v0;
+            for (uint64_t s = 0; s < h->n_steps; ++s) {
+                double z_var = 1.0 * src_normal(&src);
+                double z_ind = 1.0 * src_normal(&src);
+                heston_step(h, z_var, z_ind, &log_s, &v);
+            }
+            double s_t = exp(log_s);
+            cv_record(&m, fmax_ref(s_t - h->strike, 0.0) * h->discount, s_t * h->discount);
+        }
+    }
+    c->sum = sum;
+    c->sq_sum = sq;
+    c->m = m;
+    return NULL;
+}
+
+static mpo_result heston_tech(const mpo_heston_cfg* cfg, int scheme, int technique,
+                              const double* normals) {
+    heston_params h = make_heston_params(cfg, scheme);
+    const uint64_t units = technique == MPO_ANTITHETIC ? (cfg->n_paths + 1) / 2 : cfg->n_paths;
+    uint64_t threads = resolved_threads(cfg->n_threads);
+    uint64_t count = (threads <= 1 || units < threads * 2000) ? 1 : threads;
+    if (normals) count = 1;
+    heston_tech_chunk* ch = (heston_tech_chunk*)calloc(count, sizeof(heston_tech_chunk));
+    uint64_t base = units / count, rem = units % count;
+    for (uint64_t i = 0; i < count; ++i) {
+        ch[i].h = h;
+        ch[i].technique = technique;
+        ch[i].seed = count == 1 ? cfg->seed : mpo_derive_chunk_seed(cfg->seed, i);
+        ch[i].n_units = count == 1 ? units : base + (i < rem ? 1 : 0);
+        ch[i].normals = normals;
+    }
+    if (count == 1)
+        heston_tech_chunk_run(&ch[0]);
+    else
+        run_chunks(ch, sizeof(heston_tech_chunk), count, heston_tech_chunk_run);
+    mpo_result r;
+    if (technique == MPO_ANTITHETIC) {
+        double sum = 0.0, sq = 0.0;
+        for (uint64_t i = 0; i < count; ++i) {
+            sum += ch[i].sum;
+            sq += ch[i].sq_sum;
+        }
+        r = mpo_summarize(units, sum, sq);
+    } else {
+        mpo_cv_moments m;
+        memset(&m, 0, sizeof(m));
+        for (uint64_t i = 0; i < count; ++i) cv_merge(&m, &ch[i].m);
+        r = mpo_summarize_cv(&m, cfg->s0);
+    }
+    free(ch);
+    return r;
+}
+
+MPO_EXPORT mpo_result mpo_heston_tech_price(const mpo_heston_cfg* cfg, int scheme, int technique) {
+    if (technique == MPO_STANDARD) return mpo_heston_price(cfg, scheme);
+    return heston_tech(cfg, scheme, technique, NULL);
+}
+
+MPO_EXPORT mpo_result mpo_heston_tech_from_normals(const mpo_heston_cfg* cfg, int scheme,
+                                                   int technique, const double* normals) {
+    return heston_tech(cfg, scheme, technique, normals);
+}
+
 /* ------------------------------------------------------------------ */
 /* Merton jump diffusion: cpu.rs:932-953, 3151-3189, 6408-6441         */
 /* ------------------------------------------------------------------ */
@@ -948,6 +1050,108 @@ MPO_EXPORT mpo_result mpo_basket2_price(const mpo_basket2_cfg* cfg) {
         sq += payoff * payoff;
     }
     return mpo_summarize(cfg->n_paths, sum, sq);
+}
+
+/* Two-asset terminal basket with a technique, one stream: antithetic
+ * cpu.rs:7520-7538 (pair (z1, z2) and (-z1, -z2)), control variate
+ * :7540-7553 with control = basket * discount and expectation
+ * w1 S01 + w2 S02 (basket_control_expectation :7637-7639). */
+MPO_EXPORT mpo_result mpo_basket2_tech_price(const mpo_basket2_cfg* cfg, int technique) {
+    if (technique == MPO_STANDARD) return mpo_basket2_price(cfg);
+    double drift1 = (cfg->r - 0.5 * cfg->sigma1 * cfg->sigma1) * cfg->t;
+    double drift2 = (cfg->r - 0.5 * cfg->sigma2 * cfg->sigma2) * cfg->t;
+    double vol1 = cfg->sigma1 * sqrt(cfg->t);
+    double vol2 = cfg->sigma2 * sqrt(cfg->t);
+    double rp = 1.0 - cfg->rho * cfg->rho;
+    double ortho = sqrt(rp > 0.0 ? rp : 0.0);
+    double discount = exp(-cfg->r * cfg->t);
+    mpo_rng g;
+    rng_init(&g, cfg->seed);
+    const uint64_t units = technique == MPO_ANTITHETIC ? (cfg->n_paths + 1) / 2 : cfg->n_paths;
+    double sum = 0.0, sq = 0.0;
+    mpo_cv_moments m;
+    memset(&m, 0, sizeof(m));
+    for (uint64_t u = 0; u < units; ++u) {
+        double z1 = rng_normal(&g);
+        double z2i = rng_normal(&g);
+        double pay[2], ctrl[2];
+        for (int side = 0; side < (technique == MPO_ANTITHETIC ? 2 : 1); ++side) {
+            double a = side ? -z1 : z1, b = side ? -z2i : z2i;
+            double z2 = cfg->rho * a + ortho * b;
+            double s1 = cfg->s01 * exp(drift1 + vol1 * a);
+            double s2 = cfg->s02 * exp(drift2 + vol2 * z2);
+            double basket = cfg->weight1 * s1 + cfg->weight2 * s2;
+            pay[side] = fmax_ref(basket - cfg->k, 0.0) * discount;
+            ctrl[side] = basket * discount;
+        }
+        if (technique == MPO_ANTITHETIC) {
+            double block = 0.5 * (pay[0] + pay[1]);
+            sum += block;
+            sq += block * block;
+        } else {
+            cv_record(&m, pay[0], ctrl[0]);
+        }
+    }
+    if (technique == MPO_ANTITHETIC) return mpo_summarize(units, sum, sq);
+    return mpo_summarize_cv(&m, cfg->weight1 * cfg->s01 + cfg->weight2 * cfg->s02);
+}
+
+/* The n-asset stepped basket (below) with a technique on a caller buffer
+ * [unit][step][asset]: antithetic pairs use -z for the second path, the
+ * control variate records (payoff, basket * discount) with expectation
+ * sum_i w_i S0_i. */
+static double basket_value(uint64_t n_assets, const double* s0, const double* drift,
+                           const double* vol, const double* weight, const double* chol,
+                           uint64_t n_steps, const double* z, double sign) {
+    double x[16];
+    for (uint64_t a = 0; a < n_assets; ++a) x[a] = log(s0[a]);
+    for (uint64_t s = 0; s < n_steps; ++s) {
+        const double* zs = z + s * n_assets;
+        for (uint64_t a = 0; a < n_assets; ++a) {
+            double c = 0.0;
+            for (uint64_t b = 0; b <= a; ++b) c += chol[a * n_assets + b] * (sign * zs[b]);
+            x[a] += drift[a] + vol[a] * c;
+        }
+    }
+    double basket = 0.0;
+    for (uint64_t a = 0; a < n_assets; ++a) basket += weight[a] * exp(x[a]);
+    return basket;
+}
+
+MPO_EXPORT mpo_result mpo_basket_tech_from_normals(uint64_t n_assets, const double* s0,
+                                                   const double* sigma, const double* weight,
+                                                   const double* chol, double r, double t,
+                                                   double k, uint64_t n_steps, uint64_t n_units,
+                                                   int technique, const double* normals) {
+    double dt = t / (double)n_steps;
+    double drift[16], vol[16];
+    for (uint64_t a = 0; a < n_assets; ++a) {
+        drift[a] = (r - 0.5 * sigma[a] * sigma[a]) * dt;
+        vol[a] = sigma[a] * sqrt(dt);
+    }
+    const double discount = exp(-r * t);
+    double sum = 0.0, sq = 0.0, expect = 0.0;
+    for (uint64_t a = 0; a < n_assets; ++a) expect += weight[a] * s0[a];
+    mpo_cv_moments m;
+    memset(&m, 0, sizeof(m));
+    for (uint64_t u = 0; u < n_units; ++u) {
+        const double* z = normals + u * n_steps * n_assets;
+        double basket = basket_value(n_assets, s0, drift, vol, weight, chol, n_steps, z, 1.0);
+        double pay = fmax_ref(basket - k, 0.0) * discount;
+        if (technique == MPO_ANTITHETIC) {
+            double bb = basket_value(n_assets, s0, drift, vol, weight, chol, n_steps, z, -1.0);
+            double block = 0.5 * (pay + fmax_ref(bb - k, 0.0) * discount);
+            sum += block;
+            sq += block * block;
+        } else if (technique == MPO_CONTROL_VARIATE) {
+            cv_record(&m, pay, basket * discount);
+        } else {
+            sum += pay;
+            sq += pay * pay;
+        }
+    }
+    if (technique == MPO_CONTROL_VARIATE) return mpo_summarize_cv(&m, expect);
+    return mpo_summarize(n_units, sum, sq);
 }
 
 /* n-asset stepped basket (extension generalising cpu.rs:7622-7635):

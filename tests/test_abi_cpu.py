@@ -76,7 +76,13 @@ def test_valid_requests_report_footprints():
     ("european_call", {"sigma": 0.0}, A.MP_E_INVALID_CONFIG, "sigma must be > 0"),
     ("european_call", {"k": 0.0}, A.MP_E_INVALID_CONFIG, "k must be > 0"),
     ("european_call", {"n_steps": 0}, A.MP_E_INVALID_CONFIG, "n_steps must be > 0"),
-    ("european_call", {"technique": 1}, A.MP_E_UNSUPPORTED, "standard Monte Carlo technique"),
+    ("european_call", {"technique": 3}, A.MP_E_INVALID_CONFIG, "technique must be"),
+    ("merton_jump_diffusion_call", {"technique": "antithetic"}, A.MP_E_UNSUPPORTED,
+     "standard technique only"),
+    ("american_put", {"technique": "control_variate"}, A.MP_E_UNSUPPORTED, "standard technique only"),
+    ("heston_european_call", {"technique": "control_variate", "strikes": [80.0, 100.0, 120.0]},
+     A.MP_E_INVALID_CONFIG,
+     "one strike per request"),
     ("european_call", {"sampling": 1}, A.MP_E_UNSUPPORTED, "sampling"),
     ("european_call", {"precision": 16}, A.MP_E_INVALID_CONFIG, "precision"),
     ("european_call", {"n_paths": 1 << 33}, A.MP_E_UNSUPPORTED, "2^32"),
@@ -95,6 +101,17 @@ def test_invalid_requests_return_status_codes(workload, kw, code, msg):
     rc, _, err = footprint(make_request(workload, **kw))
     assert rc == code
     assert msg in err
+
+
+def test_techniques_are_accepted_where_the_reference_offers_them():
+    basket = dict(asset_s0=[100.0, 95.0], asset_sigma=[0.2, 0.25], asset_weight=[0.5, 0.5],
+                  correlation=[[1.0, 0.35], [0.35, 1.0]])
+    for workload in ("european_call", "arithmetic_asian_call", "down_and_out_call", "lookback_call",
+                     "heston_european_call", "basket_call"):
+        for tech in ("antithetic", "control_variate"):
+            kw = basket if workload == "basket_call" else {}
+            rc, b, err = footprint(make_request(workload, technique=tech, **kw))
+            assert rc == A.MP_OK, (workload, tech, err)
 
 
 def test_basket_validation():

@@ -99,7 +99,6 @@ def test_merton_defaults_are_terminal(fake):
 
 
 @pytest.mark.parametrize("fn,cfg,msg", [
-    (native.price_european_call_rocm, {"technique": "antithetic"}, "standard technique only"),
     (native.price_european_call_rocm, {"technique": "bogus"}, "unsupported Monte Carlo technique"),
     (native.price_european_call_rocm, {"sampling": "latin-hypercube"}, "does not offer sampling"),
     (native.price_european_call_rocm, {"sampling": "bogus"}, "unsupported sampling method"),
@@ -110,6 +109,15 @@ def test_merton_defaults_are_terminal(fake):
 def test_bad_configs_raise_value_error(fake, fn, cfg, msg):
     with pytest.raises(ValueError, match=msg):
         fn(cfg)
+
+
+def test_technique_reaches_the_request(fake):
+    native.price_european_call_rocm({"technique": "antithetic"})
+    assert fake.requests[-1].technique == 1
+    native.price_heston_european_call_rocm({"technique": "control-variate"})
+    assert fake.requests[-1].technique == 2
+    native.price_basket_call_rocm({"technique": "antithetic"})
+    assert fake.requests[-1].technique == 1
 
 
 def test_engine_errors_become_value_errors():

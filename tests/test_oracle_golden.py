@@ -111,6 +111,49 @@ def test_control_variate_summary_special_cases(oracle):
     assert oracle.summarize_cv(m2, 7.0) == oracle.summarize(len(xs), xs.sum(), (xs * xs).sum())
 
 
+@pytest.mark.parametrize("technique", ["antithetic", "control_variate"])
+@pytest.mark.parametrize("rho,factor", [(1.0, 0), (0.0, 1)])
+def test_heston_techniques_reduce_to_gbm_techniques(oracle, technique, rho, factor):
+    # xi = 0, v0 = theta = sigma^2: every Heston path is the GBM path driven by
+    # its z_var draws (rho = 1) or its z_ind draws (rho = 0), so the Heston
+    # pair / control-variate code must reproduce the GBM technique pricers
+    # (pinned by the published rows) on that factor's draws
+    n, m, sigma = 6001, 24, 0.2
+    units = (n + 1) // 2 if technique == "antithetic" else n
+    z = oracle.rng_normals(13, units * m * 2).reshape(units, m, 2)
+    h = oracle.Heston(n_paths=n, n_steps=m, v0=sigma * sigma, theta=sigma * sigma, kappa=1.5,
+                      xi=0.0, rho=rho)
+    g = oracle.Gbm(n_paths=n, n_steps=m, sigma=sigma)
+    zg = np.ascontiguousarray(z[:, :, factor])
+    for scheme in (0, 1):
+        ph, sh = oracle.heston_tech_from_normals(h, z, scheme, technique)
+        pg, sg = oracle.gbm_tech_from_normals(g, zg, "european_call", technique)
+        assert abs(ph - pg) <= 1e-12 * pg and abs(sh - sg) <= 1e-9 * sg
+
+
+@pytest.mark.parametrize("technique", ["antithetic", "control_variate"])
+def test_basket_techniques_pinned(oracle, technique):
+    # one asset, one step: the terminal European technique pricer
+    n = 8001
+    units = (n + 1) // 2 if technique == "antithetic" else n
+    z = oracle.rng_normals(17, units)
+    pb, sb = oracle.basket_tech_from_normals([100.0], [0.2], [1.0], [[1.0]], 0.03, 1.0, 100.0, 1,
+                                             units, technique, z)
+    pt, st = oracle.terminal_tech_from_normals(oracle.Gbm(n_paths=n, n_steps=1), z, technique)
+    assert abs(pb - pt) <= 1e-12 * pt and abs(sb - st) <= 1e-9 * st
+    # the reference's two-asset basket on its own stream equals the n-asset
+    # code fed that stream (Cholesky of [[1, rho], [rho, 1]] is the
+    # reference's rho z1 + sqrt(1 - rho^2) z2)
+    kw = dict(s01=100.0, s02=95.0, sigma1=0.2, sigma2=0.25, rho=0.35, weight1=0.5, weight2=0.5,
+              k=100.0, r=0.03, t=1.0, n_paths=n, seed=17)
+    p2, s2 = oracle.basket2_price(technique=technique, **kw)
+    z2 = oracle.rng_normals(17, units * 2)
+    pn, sn = oracle.basket_tech_from_normals([100.0, 95.0], [0.2, 0.25], [0.5, 0.5],
+                                             [[1.0, 0.35], [0.35, 1.0]], 0.03, 1.0, 100.0, 1,
+                                             units, technique, z2)
+    assert abs(p2 - pn) <= 1e-12 * p2 and abs(s2 - sn) <= 1e-9 * s2
+
+
 def test_black_scholes_fixture(oracle):
     want = FIX["black_scholes_european_call_atm_1y"]["expected"]["price"]
     assert oracle.bs_call(100, 100, 0.03, 0.2, 1.0) == pytest.approx(want, abs=1e-14)
