@@ -1,0 +1,168 @@
+"""GPU parity of the variance-reduction techniques (antithetic pairs and the
+control variate, MonteCarloTechnique in runtime/cpu.rs:24-28).
+
+The oracle's technique pricers reproduce the reference's published technique
+rows bit for bit (tests/test_oracle_golden.py).  Here the engine prices from
+the same draws -- the reference's normal stream fed as a buffer, or for QMC
+the oracle's scrambled-Sobol normals -- and must agree to 1e-12 in FP64 and
+to FP32 rounding in FP32.  Pseudorandom (Philox) runs are checked against
+Black-Scholes and against the standard technique's standard error.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from montepath_amd.engine import make_request
+
+pytestmark = pytest.mark.gpu
+
+GBM = [("european_call", {}), ("arithmetic_asian_call", {}),
+       ("down_and_out_call", {"barrier": 90.0}), ("lookback_call", {})]
+TECHS = ["antithetic", "control_variate"]
+BS_ATM = 9.413403383853016  # S0 = K = 100, r = 0.03, sigma = 0.2, T = 1
+
+
+def _units(n, technique):
+    return (n + 1) // 2 if technique == "antithetic" else n
+
+
+def _close(got, want, tol):
+    return abs(got - want) <= tol * abs(want)
+
+
+def _tol(precision, payoff="european_call"):
+    # FP32: per-path log-returns carry ~1e-6 relative error; a path grazing
+    # the barrier within FP32 rounding can flip its knock-out flag
+    if precision == 64:
+        return 1e-12, 1e-10
+    if payoff == "down_and_out_call":
+        return 3e-3, 3e-3
+    return 5e-5, 5e-4
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("technique", TECHS)
+@pytest.mark.parametrize("payoff,extra", GBM)
+def test_gbm_techniques_from_reference_normals(engine, oracle, payoff, extra, technique, precision):
+    n, m = 5001, 37  # odd n: ceil(n / 2) pairs; 37 steps: a partial 4-step block
+    u = _units(n, technique)
+    z = oracle.rng_normals(77, u * m).reshape(u, m)
+    cfg = oracle.Gbm(n_paths=n, n_steps=m, barrier=extra.get("barrier", 80.0))
+    p_ref, se_ref = oracle.gbm_tech_from_normals(cfg, z, payoff, technique)
+    req = make_request(payoff, n_paths=n, n_steps=m, precision=precision, technique=technique,
+                       **extra)
+    res = engine.price_from_normals(req, z.T)
+    tp, ts = _tol(precision, payoff)
+    assert _close(res.price, p_ref, tp), (res.price, p_ref)
+    assert _close(res.stderr, se_ref, ts), (res.stderr, se_ref)
+    assert res.n_paths == n
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("technique", TECHS)
+def test_terminal_techniques_from_reference_normals(engine, oracle, technique, precision):
+    n = 10001
+    u = _units(n, technique)
+    z = oracle.rng_normals(78, u)
+    p_ref, se_ref = oracle.terminal_tech_from_normals(oracle.Gbm(n_paths=n, n_steps=1), z, technique)
+    req = make_request("european_call", n_paths=n, n_steps=64, method="terminal",
+                       precision=precision, technique=technique)
+    res = engine.price_from_normals(req, z.reshape(1, u))
+    tp, ts = _tol(precision)
+    assert _close(res.price, p_ref, tp) and _close(res.stderr, se_ref, ts)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("technique", TECHS)
+@pytest.mark.parametrize("scheme", ["reference_floor", "full_truncation"])
+def test_heston_techniques_from_reference_normals(engine, oracle, scheme, technique, precision):
+    n, m = 4001, 32
+    u = _units(n, technique)
+    z = oracle.rng_normals(79, u * m * 2).reshape(u, m, 2)
+    cfg = oracle.Heston(n_paths=n, n_steps=m, kappa=2.0, xi=0.5, rho=-0.7)
+    sc = 0 if scheme == "reference_floor" else 1
+    p_ref, se_ref = oracle.heston_tech_from_normals(cfg, z, sc, technique)
+    req = make_request("heston_european_call", n_paths=n, n_steps=m, precision=precision,
+                       kappa=2.0, theta=0.04, v0=0.04, xi=0.5, rho=-0.7, heston_scheme=scheme,
+                       technique=technique)
+    res = engine.price_from_normals(req, np.ascontiguousarray(z.transpose(1, 2, 0)))
+    # FP32 Heston paths near zero variance carry up to ~1e-4 relative error
+    tp, ts = (1e-12, 1e-10) if precision == 64 else (2e-4, 2e-3)
+    assert _close(res.price, p_ref, tp), (res.price, p_ref)
+    assert _close(res.stderr, se_ref, ts), (res.stderr, se_ref)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("technique", TECHS)
+def test_basket_techniques_from_reference_normals(engine, oracle, technique, precision):
+    n, m, na = 3001, 8, 5
+    corr = np.full((na, na), 0.5) + 0.5 * np.eye(na)
+    s0 = [100.0, 95.0, 105.0, 90.0, 110.0]
+    sig = [0.2, 0.25, 0.3, 0.15, 0.35]
+    w = [0.2] * 5
+    u = _units(n, technique)
+    z = oracle.rng_normals(80, u * m * na).reshape(u, m, na)
+    p_ref, se_ref = oracle.basket_tech_from_normals(s0, sig, w, corr, 0.03, 1.0, 100.0, m, u,
+                                                    technique, z)
+    req = make_request("basket_call", n_paths=n, n_steps=m, precision=precision, asset_s0=s0,
+                       asset_sigma=sig, asset_weight=w, correlation=corr, k=100.0,
+                       technique=technique)
+    res = engine.price_from_normals(req, np.ascontiguousarray(z.transpose(1, 2, 0)))
+    tp, ts = _tol(precision)
+    assert _close(res.price, p_ref, tp), (res.price, p_ref)
+    assert _close(res.stderr, se_ref, ts), (res.stderr, se_ref)
+
+
+@pytest.mark.parametrize("technique", TECHS)
+@pytest.mark.parametrize("bridge", [False, True])
+def test_qmc_techniques_fp64_match_oracle(engine, oracle, technique, bridge):
+    # pair k (antithetic) or path k (control variate) takes Sobol point k, as
+    # the reference's QMC technique pricers do (cpu.rs:7113-7197)
+    n, m, seed = 3001, 16, 42
+    u = _units(n, technique)
+    pts = np.stack([oracle.sobol_point_normals(seed, p, m) for p in range(u)])
+    if bridge:
+        z = np.stack([oracle.bb_increments(m, 1.0, pts[p]) for p in range(u)])
+    else:
+        z = pts
+    p_ref, se_ref = oracle.gbm_tech_from_normals(oracle.Gbm(n_paths=n, n_steps=m), z,
+                                                 "european_call", technique)
+    sampling = "scrambled_sobol_brownian_bridge" if bridge else "scrambled_sobol"
+    res = engine.price(make_request("european_call", n_paths=n, n_steps=m, seed=seed, precision=64,
+                                    sampling=sampling, technique=technique))
+    assert _close(res.price, p_ref, 1e-12), (res.price, p_ref)
+    assert _close(res.stderr, se_ref, 1e-10), (res.stderr, se_ref)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_philox_techniques_price_black_scholes_and_reduce_variance(engine, precision):
+    kw = dict(n_paths=1 << 20, n_steps=64, seed=31, precision=precision)
+    std = engine.price(make_request("european_call", **kw))
+    anti = engine.price(make_request("european_call", technique="antithetic", **kw))
+    cv = engine.price(make_request("european_call", technique="control_variate", **kw))
+    for r in (std, anti, cv):
+        assert abs(r.price - BS_ATM) <= 4.0 * r.stderr, (r.price, r.stderr)
+    # the reference's published ratios are 0.747 (antithetic) and 0.411 (CV)
+    assert 0.70 <= anti.stderr / std.stderr <= 0.80
+    assert 0.38 <= cv.stderr / std.stderr <= 0.44
+    assert anti.n_paths == 1 << 20
+
+
+def test_techniques_identical_alone_and_in_a_batch(engine):
+    reqs = [
+        make_request("european_call", n_paths=20011, n_steps=40, seed=3, contract_id=1),
+        make_request("european_call", n_paths=20011, n_steps=40, seed=3, contract_id=1,
+                     technique="antithetic"),
+        make_request("arithmetic_asian_call", n_paths=30011, n_steps=24, seed=3, contract_id=2,
+                     technique="control_variate"),
+        make_request("heston_european_call", n_paths=15011, n_steps=32, seed=3, contract_id=3,
+                     kappa=2.0, xi=0.5, rho=-0.7, technique="antithetic"),
+        make_request("down_and_out_call", n_paths=9011, n_steps=50, seed=3, contract_id=4,
+                     barrier=85.0, technique="control_variate"),
+    ]
+    alone = [engine.price(r) for r in reqs]
+    batch = engine.price_batch(reqs)
+    for a, b in zip(alone, batch):
+        assert a.price == b.price and a.stderr == b.stderr
+    assert alone[0].price != alone[1].price
