@@ -125,6 +125,11 @@ def lib() -> C.CDLL:
         "mpo_gbm_paths_from_normals": (None, [C.POINTER(GbmCfg), C.c_int, _D, _D, _D]),
         "mpo_heston_price": (Result, [C.POINTER(HestonCfg), C.c_int]),
         "mpo_heston_tech_price": (Result, [C.POINTER(HestonCfg), C.c_int, C.c_int]),
+        "mpo_structured_normals": (None, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                          C.c_uint64, _D]),
+        "mpo_gbm_structured_price": (Result, [C.POINTER(GbmCfg), C.c_int, C.c_int, C.c_int, _D]),
+        "mpo_terminal_structured_price": (Result, [C.POINTER(GbmCfg), C.c_int, C.c_int]),
+        "mpo_basket2_structured_price": (Result, [C.c_void_p, C.c_int, C.c_int]),
         "mpo_heston_tech_from_normals": (Result, [C.POINTER(HestonCfg), C.c_int, C.c_int, _D]),
         "mpo_basket2_tech_price": (Result, [C.c_void_p, C.c_int]),
         "mpo_basket_tech_from_normals": (Result, [C.c_uint64, _D, _D, _D, _D, C.c_double, C.c_double,
@@ -645,6 +650,50 @@ def gbm_qmc_price(cfg: Gbm, payoff: str = "european_call", bridge: bool = True):
     c = cfg.c()
     r = lib().mpo_gbm_qmc_price(C.byref(c), PAYOFF_CODES[payoff], int(bridge), _pd(term))
     return r.price, r.stderr, term
+
+
+SAMPLING_CODES = {"pseudorandom": 0, "randomized_halton": 1, "latin_hypercube": 2,
+                  "scrambled_sobol": 3, "scrambled_sobol_brownian_bridge": 4}
+
+
+def structured_normals(sampling: str, seed: int, n_paths: int, dims: int, point0: int,
+                       n_points: int) -> np.ndarray:
+    """StructuredNormalSampler::standard_normal (cpu.rs:5790-5840) for points
+    [point0, point0 + n_points), all dims: randomized Halton, Latin hypercube
+    (built over n_paths points) or scrambled Sobol."""
+    out = np.empty((n_points, dims))
+    lib().mpo_structured_normals(SAMPLING_CODES[sampling], seed, n_paths, dims, point0, n_points,
+                                 _pd(out))
+    return out
+
+
+def gbm_structured_price(cfg: Gbm, payoff: str, sampling: str, technique: str = "standard"):
+    """QMC GBM pricers (cpu.rs:7113-7501) with any structured sampler and
+    technique; returns (price, stderr, S_T per unit)."""
+    units = (cfg.n_paths + 1) // 2 if technique == "antithetic" else cfg.n_paths
+    term = np.empty(units)
+    c = cfg.c()
+    r = lib().mpo_gbm_structured_price(C.byref(c), PAYOFF_CODES[payoff], SAMPLING_CODES[sampling],
+                                       TECHNIQUE_CODES[technique], _pd(term))
+    return r.price, r.stderr, term
+
+
+def terminal_structured_price(cfg: Gbm, sampling: str, technique: str = "standard"):
+    """simulate_terminal_qmc (cpu.rs:7064-7111)."""
+    c = cfg.c()
+    r = lib().mpo_terminal_structured_price(C.byref(c), SAMPLING_CODES[sampling],
+                                            TECHNIQUE_CODES[technique])
+    return r.price, r.stderr
+
+
+def basket2_structured_price(sampling: str, technique: str = "standard", s01=100.0, s02=100.0,
+                             k=100.0, r=0.03, sigma1=0.2, sigma2=0.25, rho=0.5, weight1=0.5,
+                             weight2=0.5, t=1.0, n_paths=100_000, seed=42):
+    """simulate_basket_qmc (cpu.rs:7555-7597)."""
+    c = Basket2Cfg(s01, s02, k, r, sigma1, sigma2, rho, weight1, weight2, t, n_paths, seed)
+    res = lib().mpo_basket2_structured_price(C.byref(c), SAMPLING_CODES[sampling],
+                                             TECHNIQUE_CODES[technique])
+    return res.price, res.stderr
 
 
 def heston_qmc_paths(cfg: Heston, scheme: int, path0: int = 0) -> np.ndarray:

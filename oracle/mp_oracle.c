@@ -1779,6 +1779,273 @@ MPO_EXPORT mpo_result mpo_gbm_qmc_price(const mpo_gbm_cfg* cfg, int payoff, int 
     return mpo_summarize(cfg->n_paths, sum, sq);
 }
 
+/* ------------------------------------------------------------------ */
+/* Structured normal samplers (StructuredNormalSampler cpu.rs:5790-5840)*/
+/* RandomizedHalton cpu.rs:5686-5721 (halton_with_shift :6063-6077,    */
+/* first_n_primes :6079-6102); LatinHypercube cpu.rs:5723-5788          */
+/* (coprime_multiplier :5983-6010, open01_from_u64 :6104-6107); scrambled*/
+/* Sobol as mpo_sobol_point_normals.                                    */
+/* ------------------------------------------------------------------ */
+enum { MPO_PSEUDORANDOM = 0, MPO_HALTON = 1, MPO_LHS = 2, MPO_SOBOL = 3, MPO_SOBOL_BB = 4 };
+
+static double open01_from_u64(uint64_t v) {
+    double x = (double)(v >> 11) * (1.0 / 9007199254740992.0);
+    return x > DBL_MIN ? x : DBL_MIN;
+}
+
+static uint64_t gcd_u64(uint64_t a, uint64_t b) {
+    while (b != 0) {
+        uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+static uint64_t coprime_multiplier(uint64_t seed, uint64_t modulus) {
+    if (modulus <= 1) return 1;
+    uint64_t c = seed % modulus;
+    if (c < 1) c = 1;
+    if (c % 2 == 0) c = c == UINT64_MAX ? c : c + 1;
+    while (gcd_u64(c, modulus) != 1) {
+        c = c > UINT64_MAX - 2 ? UINT64_MAX : c + 2;
+        if (c >= modulus) c = 1;
+    }
+    return c;
+}
+
+static void first_n_primes(uint64_t n, uint32_t* out) {
+    uint64_t count = 0;
+    uint32_t cand = 2;
+    while (count < n) {
+        uint32_t limit = (uint32_t)sqrt((double)cand);
+        int prime = 1;
+        for (uint64_t i = 0; i < count; ++i) {
+            if (out[i] > limit) break;
+            if (cand % out[i] == 0) {
+                prime = 0;
+                break;
+            }
+        }
+        if (prime) out[count++] = cand;
+        cand = cand == UINT32_MAX ? cand : cand + 1;
+    }
+}
+
+static double halton(uint64_t index, uint32_t base) {
+    double f = 1.0 / (double)base, value = 0.0;
+    while (index > 0) {
+        value += f * (double)(index % (uint64_t)base);
+        index /= (uint64_t)base;
+        f /= (double)base;
+    }
+    return value > DBL_MIN ? value : DBL_MIN;
+}
+
+typedef struct {
+    int kind;
+    uint64_t seed, n_paths, dims;
+    uint32_t* primes;
+    double* shifts;
+    uint64_t *mult, *off, *jit;
+} mpo_sampler;
+
+static void sampler_init(mpo_sampler* s, int kind, uint64_t seed, uint64_t n_paths, uint64_t dims) {
+    memset(s, 0, sizeof(*s));
+    s->kind = kind;
+    s->seed = seed;
+    s->n_paths = n_paths;
+    s->dims = dims;
+    if (kind == MPO_HALTON) {
+        uint64_t nd = dims > 2 ? dims : 2;
+        s->primes = (uint32_t*)malloc(sizeof(uint32_t) * nd);
+        s->shifts = (double*)malloc(sizeof(double) * nd);
+        first_n_primes(nd, s->primes);
+        for (uint64_t d = 0; d < nd; ++d)
+            s->shifts[d] = open01_from_u64(mpo_splitmix64(seed + d * 0x9E3779B97F4A7C15ull));
+    } else if (kind == MPO_LHS) {
+        uint64_t nd = dims > 1 ? dims : 1;
+        s->mult = (uint64_t*)malloc(sizeof(uint64_t) * nd);
+        s->off = (uint64_t*)malloc(sizeof(uint64_t) * nd);
+        s->jit = (uint64_t*)malloc(sizeof(uint64_t) * nd);
+        for (uint64_t d = 0; d < nd; ++d) {
+            uint64_t mixed = mpo_splitmix64(seed ^ (d * 0x9E3779B97F4A7C15ull) ^ 0xD1B54A32D192ED03ull);
+            s->mult[d] = coprime_multiplier(mixed, n_paths > 1 ? n_paths : 1);
+            s->off[d] = n_paths == 0 ? 0 : mpo_splitmix64(mixed ^ 0xA24BAED4963EE407ull) % n_paths;
+            s->jit[d] = mpo_splitmix64(mixed ^ 0x9FB21C651E98DF25ull);
+        }
+    }
+}
+
+static void sampler_free(mpo_sampler* s) {
+    free(s->primes);
+    free(s->shifts);
+    free(s->mult);
+    free(s->off);
+    free(s->jit);
+}
+
+static double sampler_normal(const mpo_sampler* s, uint64_t point, uint64_t dim) {
+    if (s->kind == MPO_HALTON) {
+        double shifted = halton(point + 1, s->primes[dim]) + s->shifts[dim];
+        return mpo_inverse_normal(shifted - floor(shifted));
+    }
+    if (s->kind == MPO_LHS) {
+        uint64_t stratum = s->n_paths == 0 ? 0 : (s->mult[dim] * point + s->off[dim]) % s->n_paths;
+        double jitter = open01_from_u64(mpo_splitmix64(s->jit[dim] ^ (point * 0xBF58476D1CE4E5B9ull)));
+        double u = ((double)stratum + jitter) / (double)s->n_paths;
+        if (u < DBL_MIN) u = DBL_MIN;
+        if (u > 1.0 - DBL_EPSILON) u = 1.0 - DBL_EPSILON;
+        return mpo_inverse_normal(u);
+    }
+    /* scrambled Sobol, fill_scrambled_sobol_point_standard_normals */
+    uint64_t block = point >> 16;
+    uint32_t local = (uint32_t)(point & 0xFFFFu);
+    uint32_t block_seed = (uint32_t)s->seed + (uint32_t)block * 0x9E3779B9u;
+    uint32_t off = (uint32_t)(dim / MPO_SOBOL_DIMS);
+    return mpo_inverse_normal(mpo_sobol_sample(local, (uint32_t)(dim % MPO_SOBOL_DIMS), block_seed + off));
+}
+
+/* Normals of one point in step order: the sampler's dims 0..m-1, or for
+ * Sobol with a Brownian bridge the bridge's step normals (cpu.rs:5963-5980). */
+static void point_step_normals(const mpo_sampler* s, uint64_t point, uint64_t m, double t, double* in,
+                               double* z) {
+    for (uint64_t d = 0; d < m; ++d) in[d] = sampler_normal(s, point, d);
+    if (s->kind == MPO_SOBOL_BB)
+        mpo_bb_increments(m, t, in, z);
+    else
+        memcpy(z, in, sizeof(double) * m);
+}
+
+MPO_EXPORT void mpo_structured_normals(int kind, uint64_t seed, uint64_t n_paths, uint64_t dims,
+                                       uint64_t point0, uint64_t n_points, double* out) {
+    mpo_sampler s;
+    sampler_init(&s, kind, seed, n_paths, dims);
+    for (uint64_t p = 0; p < n_points; ++p)
+        for (uint64_t d = 0; d < dims; ++d) out[p * dims + d] = sampler_normal(&s, point0 + p, d);
+    sampler_free(&s);
+}
+
+/* QMC GBM family with any technique: simulate_stepwise_qmc cpu.rs:7113-7197,
+ * Asian :7199-7290, down-and-out :7292-7407, lookback :7409-7501; one point
+ * per path (or pair), dims = n_steps, the sampler built over n_paths. */
+MPO_EXPORT mpo_result mpo_gbm_structured_price(const mpo_gbm_cfg* cfg, int payoff, int sampling,
+                                               int technique, double* terminal_out) {
+    const uint64_t m = cfg->n_steps;
+    const double dt = cfg->t / (double)m;
+    const double drift_dt = (cfg->r - 0.5 * cfg->sigma * cfg->sigma) * dt;
+    const double vol_dt = cfg->sigma * sqrt(dt);
+    const double discount = exp(-cfg->r * cfg->t);
+    const double log_s0 = log(cfg->s0);
+    mpo_sampler smp;
+    sampler_init(&smp, sampling, cfg->seed, cfg->n_paths, m);
+    const uint64_t units = technique == MPO_ANTITHETIC ? (cfg->n_paths + 1) / 2 : cfg->n_paths;
+    double* in = (double*)malloc(sizeof(double) * m);
+    double* z = (double*)malloc(sizeof(double) * m);
+    double sum = 0.0, sq = 0.0;
+    mpo_cv_moments mo;
+    memset(&mo, 0, sizeof(mo));
+    for (uint64_t u = 0; u < units; ++u) {
+        point_step_normals(&smp, u, m, cfg->t, in, z);
+        gbm_state a, b;
+        gbm_state_init(&a, log_s0, cfg->s0, cfg->barrier);
+        gbm_state_init(&b, log_s0, cfg->s0, cfg->barrier);
+        for (uint64_t st = 0; st < m; ++st) {
+            gbm_state_step(&a, payoff, drift_dt + vol_dt * z[st], cfg->barrier);
+            if (technique == MPO_ANTITHETIC) gbm_state_step(&b, payoff, drift_dt - vol_dt * z[st], cfg->barrier);
+        }
+        double pa = gbm_state_payoff(&a, payoff, cfg->k, m, discount);
+        if (terminal_out) terminal_out[u] = exp(a.log_s);
+        if (technique == MPO_ANTITHETIC) {
+            double block = 0.5 * (pa + gbm_state_payoff(&b, payoff, cfg->k, m, discount));
+            sum += block;
+            sq += block * block;
+        } else if (technique == MPO_CONTROL_VARIATE) {
+            cv_record(&mo, pa, discount * exp(a.log_s));
+        } else {
+            sum += pa;
+            sq += pa * pa;
+        }
+    }
+    free(in);
+    free(z);
+    sampler_free(&smp);
+    if (technique == MPO_CONTROL_VARIATE) return mpo_summarize_cv(&mo, cfg->s0);
+    return mpo_summarize(units, sum, sq);
+}
+
+/* QMC terminal European: simulate_terminal_qmc cpu.rs:7064-7111 (one dim). */
+MPO_EXPORT mpo_result mpo_terminal_structured_price(const mpo_gbm_cfg* cfg, int sampling, int technique) {
+    const double drift_t = (cfg->r - 0.5 * cfg->sigma * cfg->sigma) * cfg->t;
+    const double vol_t = cfg->sigma * sqrt(cfg->t);
+    const double discount = exp(-cfg->r * cfg->t);
+    mpo_sampler smp;
+    sampler_init(&smp, sampling == MPO_SOBOL_BB ? MPO_SOBOL : sampling, cfg->seed, cfg->n_paths, 1);
+    const uint64_t units = technique == MPO_ANTITHETIC ? (cfg->n_paths + 1) / 2 : cfg->n_paths;
+    double* z = (double*)malloc(sizeof(double) * (units ? units : 1));
+    for (uint64_t u = 0; u < units; ++u) z[u] = sampler_normal(&smp, u, 0);
+    sampler_free(&smp);
+    mpo_result r;
+    if (technique == MPO_STANDARD) {
+        double sum = 0.0, sq = 0.0;
+        for (uint64_t u = 0; u < units; ++u) {
+            double pay = fmax_ref(cfg->s0 * exp(drift_t + vol_t * z[u]) - cfg->k, 0.0) * discount;
+            sum += pay;
+            sq += pay * pay;
+        }
+        r = mpo_summarize(units, sum, sq);
+    } else {
+        r = term_tech(cfg, technique, z);
+    }
+    free(z);
+    return r;
+}
+
+/* QMC two-asset basket: simulate_basket_qmc cpu.rs:7555-7597 (dims 2). */
+MPO_EXPORT mpo_result mpo_basket2_structured_price(const mpo_basket2_cfg* cfg, int sampling,
+                                                   int technique) {
+    double drift1 = (cfg->r - 0.5 * cfg->sigma1 * cfg->sigma1) * cfg->t;
+    double drift2 = (cfg->r - 0.5 * cfg->sigma2 * cfg->sigma2) * cfg->t;
+    double vol1 = cfg->sigma1 * sqrt(cfg->t);
+    double vol2 = cfg->sigma2 * sqrt(cfg->t);
+    double rp = 1.0 - cfg->rho * cfg->rho;
+    double ortho = sqrt(rp > 0.0 ? rp : 0.0);
+    double discount = exp(-cfg->r * cfg->t);
+    mpo_sampler smp;
+    sampler_init(&smp, sampling == MPO_SOBOL_BB ? MPO_SOBOL : sampling, cfg->seed, cfg->n_paths, 2);
+    const uint64_t units = technique == MPO_ANTITHETIC ? (cfg->n_paths + 1) / 2 : cfg->n_paths;
+    double sum = 0.0, sq = 0.0;
+    mpo_cv_moments mo;
+    memset(&mo, 0, sizeof(mo));
+    for (uint64_t u = 0; u < units; ++u) {
+        double z1 = sampler_normal(&smp, u, 0), z2i = sampler_normal(&smp, u, 1);
+        double pay[2], ctrl[2];
+        for (int side = 0; side < (technique == MPO_ANTITHETIC ? 2 : 1); ++side) {
+            double a = side ? -z1 : z1, b = side ? -z2i : z2i;
+            double z2 = cfg->rho * a + ortho * b;
+            double s1 = cfg->s01 * exp(drift1 + vol1 * a);
+            double s2 = cfg->s02 * exp(drift2 + vol2 * z2);
+            double basket = cfg->weight1 * s1 + cfg->weight2 * s2;
+            pay[side] = fmax_ref(basket - cfg->k, 0.0) * discount;
+            ctrl[side] = basket * discount;
+        }
+        if (technique == MPO_ANTITHETIC) {
+            double block = 0.5 * (pay[0] + pay[1]);
+            sum += block;
+            sq += block * block;
+        } else if (technique == MPO_CONTROL_VARIATE) {
+            cv_record(&mo, pay[0], ctrl[0]);
+        } else {
+            sum += pay[0];
+            sq += pay[0] * pay[0];
+        }
+    }
+    sampler_free(&smp);
+    if (technique == MPO_CONTROL_VARIATE)
+        return mpo_summarize_cv(&mo, cfg->weight1 * cfg->s01 + cfg->weight2 * cfg->s02);
+    return mpo_summarize(units, sum, sq);
+}
+
 /* Two-factor Sobol + Brownian-bridge Heston (extension; the reference has
  * no QMC Heston).  One Sobol point of 2*n_steps dims per path; bridge input
  * k of factor f (0 = variance shock, 1 = independent shock) is dimension

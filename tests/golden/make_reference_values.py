@@ -52,6 +52,27 @@ ROWS = {
                          "technique": "control_variate"}),  # :3563
     "mc_cpu_lookback_call_rust_control_variate": (
         "gbm_stepwise", {"payoff": "lookback_call", "technique": "control_variate"}),  # :3686
+    # randomized Halton and Latin hypercube QMC (StructuredNormalSampler)
+    "mc_cpu_european_call_rust_randomized_halton": (
+        "gbm_structured", {"payoff": "european_call", "sampling": "randomized_halton"}),  # :3003
+    "mc_cpu_european_call_rust_latin_hypercube": (
+        "gbm_structured", {"payoff": "european_call", "sampling": "latin_hypercube"}),  # :3273
+    "mc_cpu_arithmetic_asian_call_rust_randomized_halton": (
+        "gbm_structured", {"payoff": "arithmetic_asian_call", "sampling": "randomized_halton"}),
+    "mc_cpu_arithmetic_asian_call_rust_latin_hypercube": (
+        "gbm_structured", {"payoff": "arithmetic_asian_call", "sampling": "latin_hypercube"}),
+    "mc_cpu_down_and_out_call_rust_randomized_halton": (
+        "gbm_structured", {"payoff": "down_and_out_call", "barrier": 80.0,
+                           "sampling": "randomized_halton"}),
+    "mc_cpu_down_and_out_call_rust_latin_hypercube": (
+        "gbm_structured", {"payoff": "down_and_out_call", "barrier": 80.0,
+                           "sampling": "latin_hypercube"}),
+    "mc_cpu_basket_call_rust_randomized_halton": (
+        "basket2_structured", {"s01": 100.0, "s02": 95.0, "sigma1": 0.2, "sigma2": 0.25, "rho": 0.35,
+                               "weight1": 0.5, "weight2": 0.5, "sampling": "randomized_halton"}),
+    "mc_cpu_basket_call_rust_latin_hypercube": (
+        "basket2_structured", {"s01": 100.0, "s02": 95.0, "sigma1": 0.2, "sigma2": 0.25, "rho": 0.35,
+                               "weight1": 0.5, "weight2": 0.5, "sampling": "latin_hypercube"}),
     # scrambled-Sobol rows: sobol_burley 0.5 bits, not reproducible without the crate
     "mc_cpu_european_call_rust_scrambled_sobol": ("gbm_qmc", {"bridge": False}),  # :3356
     "mc_cpu_european_call_rust_scrambled_sobol_brownian_bridge": ("gbm_qmc", {"bridge": True}),

@@ -50,6 +50,16 @@ def _row_price(O, name):
             prices.append(O.basket2_price(s01=c["s01"], s02=c["s02"], sigma1=c["sigma1"],
                                           sigma2=c["sigma2"], rho=c["rho"], weight1=c["weight1"],
                                           weight2=c["weight2"], n_paths=c["n_paths"], seed=seed)[0])
+        elif kind == "gbm_structured":
+            g = O.Gbm(s0=c["s0"], k=c["k"], r=c["r"], sigma=c["sigma"], t=c["t"],
+                      n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed,
+                      barrier=c.get("barrier", 80.0))
+            prices.append(O.gbm_structured_price(g, c["payoff"], c["sampling"])[0])
+        elif kind == "basket2_structured":
+            prices.append(O.basket2_structured_price(
+                c["sampling"], s01=c["s01"], s02=c["s02"], sigma1=c["sigma1"], sigma2=c["sigma2"],
+                rho=c["rho"], weight1=c["weight1"], weight2=c["weight2"], n_paths=c["n_paths"],
+                seed=seed)[0])
         elif kind in ("american_lsm", "bermudan_lsm"):
             cfg = O.Lsm(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=seed,
                         basis_degree=c["basis_degree"],
@@ -152,6 +162,26 @@ def test_basket_techniques_pinned(oracle, technique):
                                              [[1.0, 0.35], [0.35, 1.0]], 0.03, 1.0, 100.0, 1,
                                              units, technique, z2)
     assert abs(p2 - pn) <= 1e-12 * p2 and abs(s2 - sn) <= 1e-9 * s2
+
+
+def test_structured_samplers_stratify(oracle):
+    # Latin hypercube: in every dimension each of the n strata [i/n, (i+1)/n)
+    # holds exactly one point; randomized Halton in base 2 (dimension 0)
+    # puts one point in each half, quarter, ... of [0, 1) up to the shift
+    from scipy.special import ndtr
+
+    n, dims = 1024, 6
+    z = oracle.structured_normals("latin_hypercube", 11, n, dims, 0, n)
+    strata = np.floor(ndtr(z) * n).astype(int)
+    for d in range(dims):
+        assert sorted(strata[:, d]) == list(range(n))
+    h = oracle.structured_normals("randomized_halton", 11, n, 2, 0, n)
+    u = ndtr(h[:, 0])
+    # points 0..n-1 take Halton indices 1..n: base-2 radical inverses k/n
+    # (k = 1..n-1) and 1/(2n) for index n, all moved by one shift mod 1
+    w = np.mod(u - (u[n - 1] - 0.5 / n), 1.0)
+    cells = np.floor(w * n + 1e-6).astype(int) % n
+    assert np.bincount(cells, minlength=n).max() == 1
 
 
 def test_black_scholes_fixture(oracle):
