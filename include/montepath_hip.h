@@ -64,10 +64,15 @@ typedef enum mp_workload {
     MP_BERMUDAN_PUT = 9
 } mp_workload;
 
-/* Sampling (reference SamplingMethod, runtime/cpu.rs).  Halton and Latin
- * hypercube are not offered by this engine (MP_E_UNSUPPORTED). */
+/* Sampling (reference SamplingMethod, runtime/cpu.rs:5796-5830).  Randomized
+ * Halton and Latin hypercube draw one point per path (per antithetic pair)
+ * over at most 65536 dimensions: the GBM family, Heston (extension: factor f
+ * of step s is dimension 2s + f) and the basket (asset a of step s is
+ * dimension s * n_assets + a).  Merton and LSM take pseudorandom draws only. */
 typedef enum mp_sampling {
     MP_PSEUDORANDOM = 0,
+    MP_RANDOMIZED_HALTON = 1,
+    MP_LATIN_HYPERCUBE = 2,
     MP_SCRAMBLED_SOBOL = 3,
     MP_SCRAMBLED_SOBOL_BRIDGE = 4
 } mp_sampling;

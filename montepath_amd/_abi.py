@@ -35,6 +35,8 @@ AMERICAN_PUT = 8
 BERMUDAN_PUT = 9
 
 PSEUDORANDOM = 0
+RANDOMIZED_HALTON = 1
+LATIN_HYPERCUBE = 2
 SCRAMBLED_SOBOL = 3
 SCRAMBLED_SOBOL_BRIDGE = 4
 

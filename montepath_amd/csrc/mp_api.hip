@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -113,6 +114,7 @@ struct prepared {
     int degree;
     uint64_t n_steps_cfg;  // steps as configured (terminal method walks 1)
     uint64_t n_paths_req;  // paths as requested (antithetic units are pairs)
+    std::vector<mp_sampler_dim> sampler;  // Halton / Latin hypercube dimensions
 };
 
 }  // namespace
@@ -138,6 +140,7 @@ struct mp_ctx {
     dbuf lsm_paths, lsm_cash, lsm_ex, lsm_coeff, lsm_valid, lsm_gram, lsm_par, lsm_cnt,
         lsm_part, lsm_out, plain_pay;
     dbuf draws;
+    dbuf sampler;
 };
 
 namespace {
@@ -228,6 +231,89 @@ uint32_t tile_paths_for(uint64_t n_paths, uint64_t n_steps) {
     return (uint32_t)t;
 }
 
+// ---------------------------------------------------------------------------
+// Structured-sampler tables, built on the host once per request.
+// ---------------------------------------------------------------------------
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double open01(uint64_t v) {
+    const double x = (double)(v >> 11) * (1.0 / 9007199254740992.0);
+    return x > DBL_MIN ? x : DBL_MIN;
+}
+
+uint64_t gcd64(uint64_t a, uint64_t b) {
+    while (b != 0) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// Smallest odd candidate at or after seed mod n (stepping by 2, wrapping to 1)
+// that is coprime to n (coprime_multiplier, cpu.rs:5983-6001).
+uint64_t coprime_multiplier(uint64_t seed, uint64_t n) {
+    if (n <= 1) return 1;
+    uint64_t c = std::max<uint64_t>(seed % n, 1);
+    if (c % 2 == 0) c += 1;
+    while (gcd64(c, n) != 1) {
+        c += 2;
+        if (c >= n) c = 1;
+    }
+    return c;
+}
+
+// The first n primes by a sieve of Eratosthenes over a bound from Rosser's
+// theorem (p_n < n (ln n + ln ln n) for n >= 6).
+void first_primes(uint64_t n, std::vector<uint32_t>& out) {
+    out.clear();
+    const double nd = (double)std::max<uint64_t>(n, 6);
+    const uint64_t lim = (uint64_t)(nd * (std::log(nd) + std::log(std::log(nd)))) + 16;
+    std::vector<uint8_t> comp(lim + 1, 0);
+    for (uint64_t i = 2; i <= lim && out.size() < n; ++i) {
+        if (comp[i]) continue;
+        out.push_back((uint32_t)i);
+        for (uint64_t j = i * i; j <= lim; j += i) comp[j] = 1;
+    }
+}
+
+// Randomized Halton (RandomizedHalton::new, cpu.rs:5692-5704): dimension d
+// takes the d-th prime and the shift open01(splitmix64(seed + d * golden)),
+// over max(dims, 2) dimensions.  Latin hypercube (LatinHypercube::new,
+// cpu.rs:5737-5762): per dimension a seed mixed from (seed, d), a stride
+// multiplier coprime to n_paths, an offset and a jitter seed.
+void build_sampler(int kind, uint64_t seed, uint64_t n_paths, uint64_t dims,
+                   std::vector<mp_sampler_dim>& out) {
+    out.clear();
+    if (kind == MP_SAMPLER_HALTON) {
+        const uint64_t nd = std::max<uint64_t>(dims, 2);
+        std::vector<uint32_t> primes;
+        first_primes(nd, primes);
+        out.resize(nd);
+        for (uint64_t d = 0; d < nd; ++d) {
+            out[d].a = primes[d];
+            out[d].b = out[d].c = 0;
+            out[d].shift = open01(splitmix64(seed + d * 0x9E3779B97F4A7C15ull));
+        }
+    } else {
+        const uint64_t nd = std::max<uint64_t>(dims, 1);
+        out.resize(nd);
+        for (uint64_t d = 0; d < nd; ++d) {
+            const uint64_t mixed = splitmix64(seed ^ (d * 0x9E3779B97F4A7C15ull) ^ 0xD1B54A32D192ED03ull);
+            out[d].a = coprime_multiplier(mixed, std::max<uint64_t>(n_paths, 1));
+            out[d].b = n_paths == 0 ? 0 : splitmix64(mixed ^ 0xA24BAED4963EE407ull) % n_paths;
+            out[d].c = splitmix64(mixed ^ 0x9FB21C651E98DF25ull);
+            out[d].shift = 0.0;
+        }
+    }
+}
+
 int prepare_request(const mp_request& r, prepared& P) {
     std::memset(&P.d, 0, sizeof(P.d));
     P.schedule.clear();
@@ -236,9 +322,10 @@ int prepare_request(const mp_request& r, prepared& P) {
         return fail(MP_E_INVALID_CONFIG, "precision must be 32 or 64");
     if (r.technique < MP_TECH_STANDARD || r.technique > MP_TECH_CONTROL_VARIATE)
         return fail(MP_E_INVALID_CONFIG, "technique must be 0 (standard), 1 (antithetic) or 2 (control variate)");
-    if (r.sampling != MP_PSEUDORANDOM && r.sampling != MP_SCRAMBLED_SOBOL &&
+    if (r.sampling != MP_PSEUDORANDOM && r.sampling != MP_RANDOMIZED_HALTON &&
+        r.sampling != MP_LATIN_HYPERCUBE && r.sampling != MP_SCRAMBLED_SOBOL &&
         r.sampling != MP_SCRAMBLED_SOBOL_BRIDGE)
-        return fail(MP_E_UNSUPPORTED, "sampling method not offered on rocm_hip");
+        return fail(MP_E_INVALID_CONFIG, "unknown sampling method " + std::to_string(r.sampling));
     if (r.n_paths == 0) return fail(MP_E_INVALID_CONFIG, "n_paths must be > 0");
     if (r.n_paths > 0xFFFFFFFFull)
         return fail(MP_E_UNSUPPORTED, "n_paths above 2^32 per request is not supported");
@@ -272,7 +359,9 @@ int prepare_request(const mp_request& r, prepared& P) {
     P.factors = 1;
     P.bridge = r.sampling == MP_SCRAMBLED_SOBOL_BRIDGE;
     P.n_steps_cfg = r.n_steps;
-    const bool qmc = r.sampling != MP_PSEUDORANDOM;
+    P.sampler.clear();
+    const bool qmc = r.sampling == MP_SCRAMBLED_SOBOL || r.sampling == MP_SCRAMBLED_SOBOL_BRIDGE;
+    const bool structured = r.sampling == MP_RANDOMIZED_HALTON || r.sampling == MP_LATIN_HYPERCUBE;
 
     switch (r.workload) {
         case MP_EUROPEAN_CALL:
@@ -365,7 +454,8 @@ int prepare_request(const mp_request& r, prepared& P) {
             if (!(r.jump_volatility >= 0.0))
                 return fail(MP_E_INVALID_CONFIG, "jump_volatility must be >= 0");
             if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
-            if (qmc) return fail(MP_E_UNSUPPORTED, "Merton is priced with pseudorandom draws only");
+            if (qmc || structured)
+                return fail(MP_E_UNSUPPORTED, "Merton is priced with pseudorandom draws only");
             const uint64_t m = r.n_steps == 0 ? 1 : r.n_steps;  // reference Merton is terminal
             if (m > 0x7FFFFFFFull) return fail(MP_E_UNSUPPORTED, "n_steps too large");
             d.model = MP_MODEL_MERTON;
@@ -423,7 +513,8 @@ int prepare_request(const mp_request& r, prepared& P) {
             if (!(sig >= 0.0)) return fail(MP_E_INVALID_CONFIG, "sigma must be >= 0");
             if (!(t > 0.0)) return fail(MP_E_INVALID_CONFIG, "t must be > 0");
             if (r.basis_degree <= 0) return fail(MP_E_INVALID_CONFIG, "basis_degree must be > 0");
-            if (qmc) return fail(MP_E_UNSUPPORTED, "LSM is priced with pseudorandom draws only");
+            if (qmc || structured)
+                return fail(MP_E_UNSUPPORTED, "LSM is priced with pseudorandom draws only");
             P.kind = KIND_LSM;
             P.degree = std::min(std::max(r.basis_degree, 1), 3);
             if (r.workload == MP_AMERICAN_PUT) {
@@ -451,6 +542,20 @@ int prepare_request(const mp_request& r, prepared& P) {
             return fail(MP_E_UNSUPPORTED, "unknown workload " + std::to_string(r.workload));
     }
     P.n_paths_req = r.n_paths;
+    if (structured) {
+        // one point per path (or antithetic pair); dimensions: step s of the
+        // GBM family, factor f of Heston step s at 2s + f, asset a of basket
+        // step s at s * n_assets + a
+        const uint64_t dims = d.model == MP_MODEL_HESTON ? 2ull * (uint64_t)d.n_steps
+                              : P.kind == KIND_BASKET  ? (uint64_t)d.n_steps * (uint64_t)P.n_assets
+                                                       : (uint64_t)d.n_steps;
+        if (dims > MP_SAMPLER_MAX_DIMS)
+            return fail(MP_E_UNSUPPORTED, "Halton and Latin hypercube sampling take at most " +
+                                              std::to_string(MP_SAMPLER_MAX_DIMS) + " dimensions");
+        d.sampling = r.sampling == MP_RANDOMIZED_HALTON ? MP_SAMPLER_HALTON : MP_SAMPLER_LHS;
+        d.sampler_n = r.n_paths;
+        build_sampler(d.sampling, r.seed, r.n_paths, dims, P.sampler);
+    }
     if (r.technique != MP_TECH_STANDARD) {
         // the reference offers techniques for the GBM family, Heston and the
         // basket (MonteCarloTechnique, cpu.rs:24-28); Merton and LSM have none
@@ -533,7 +638,7 @@ struct mp_batch {
     std::vector<size_t> req_index;  // position in the caller's array
     uint32_t n_tiles = 0;
     int grid = 1, block = 256, ppl = 1;
-    dbuf d_reqs, d_tile_req, d_order, d_partial, d_out, d_counter;
+    dbuf d_reqs, d_tile_req, d_order, d_partial, d_out, d_counter, d_sampler;
     std::vector<double> h_out;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     hipGraph_t graph = nullptr;
@@ -555,9 +660,14 @@ int batch_build(mp_batch* b) {
     const size_t n = b->reqs.size();
     std::vector<mp_dreq> dreqs(n);
     uint32_t tiles = 0;
+    std::vector<mp_sampler_dim> sampler;  // every structured request's dimensions
     for (size_t i = 0; i < n; ++i) {
         dreqs[i] = b->reqs[i].d;
         dreqs[i].tile0 = tiles;
+        if (dreqs[i].sampling != MP_SAMPLER_NONE) {
+            dreqs[i].sampler_off = (uint32_t)sampler.size();
+            sampler.insert(sampler.end(), b->reqs[i].sampler.begin(), b->reqs[i].sampler.end());
+        }
         tiles += dreqs[i].n_tiles;
     }
     b->n_tiles = tiles;
@@ -580,6 +690,11 @@ int batch_build(mp_batch* b) {
     HIPCHK(hipMemcpyAsync(b->d_reqs.p, dreqs.data(), n * sizeof(mp_dreq), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_tile_req.p, tile_req.data(), (size_t)tiles * 4, hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_order.p, order.data(), (size_t)tiles * 4, hipMemcpyHostToDevice, b->stream));
+    if (!sampler.empty()) {
+        ENSURE(b->d_sampler, sampler.size() * sizeof(mp_sampler_dim));
+        HIPCHK(hipMemcpyAsync(b->d_sampler.p, sampler.data(), sampler.size() * sizeof(mp_sampler_dim),
+                              hipMemcpyHostToDevice, b->stream));
+    }
     HIPCHK(hipStreamSynchronize(b->stream));
     // launch shape: tuned for the dominant request class
     mp_tuning tu = tuning_for(b->ctx, dreqs.empty() ? mp_dreq{} : dreqs[0], b->precision);
@@ -603,6 +718,7 @@ int batch_enqueue(mp_batch* b) {
     L.n_tiles = b->n_tiles;
     L.counter = b->d_counter.as<unsigned int>();
     L.partial = b->d_partial.as<double>();
+    L.sampler = b->d_sampler.as<mp_sampler_dim>();
     L.grid = b->grid;
     L.block = b->block;
     L.ppl = b->ppl;
@@ -626,6 +742,7 @@ int batch_capture(mp_batch* b) {
     L.n_tiles = b->n_tiles;
     L.counter = b->d_counter.as<unsigned int>();
     L.partial = b->d_partial.as<double>();
+    L.sampler = b->d_sampler.as<mp_sampler_dim>();
     L.grid = b->grid;
     L.block = b->block;
     L.ppl = b->ppl;
@@ -803,12 +920,19 @@ int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const
     if (normals_host) ENSURE(c->normals, nn * 8);
     if (term_host) ENSURE(c->term, d.n_paths * 8);
     if (pay_host) ENSURE(c->pay, d.n_paths * 8);
+    if (normals_host) d.sampling = MP_SAMPLER_NONE;  // caller buffers replace every draw
+    if (d.sampling != MP_SAMPLER_NONE) {
+        d.sampler_off = 0;
+        ENSURE(c->sampler, P.sampler.size() * sizeof(mp_sampler_dim));
+        HIPCHK(hipMemcpyAsync(c->sampler.p, P.sampler.data(), P.sampler.size() * sizeof(mp_sampler_dim),
+                              hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->basket_par.p, par.data(), par.size() * 8, hipMemcpyHostToDevice, c->stream));
     if (normals_host) HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, nn * 8, hipMemcpyHostToDevice, c->stream));
     const double* bp = c->basket_par.as<double>();
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(mp::launch_basket(c->reqs.as<mp_dreq>(), na, bp, bp + na * na, bp + na * na + na,
+    HIPCHK(mp::launch_basket(c->reqs.as<mp_dreq>(), c->sampler.as<mp_sampler_dim>(), na, bp, bp + na * na, bp + na * na + na,
                              bp + na * na + 2 * na, bp + na * na + 3 * na,
                              normals_host ? c->normals.as<double>() : nullptr, c->partial.as<double>(),
                              term_host ? c->term.as<double>() : nullptr,
@@ -980,6 +1104,13 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
         }
     }
     std::vector<uint32_t> tile_req(d.n_tiles, 0u);
+    if (!buffer && !P.sampler.empty()) {
+        d.sampler_off = 0;
+        ENSURE(c->sampler, P.sampler.size() * sizeof(mp_sampler_dim));
+        HIPCHK(hipMemcpyAsync(c->sampler.p, P.sampler.data(), P.sampler.size() * sizeof(mp_sampler_dim),
+                              hipMemcpyHostToDevice, c->stream));
+    }
+    if (buffer) d.sampling = MP_SAMPLER_NONE;  // caller buffers replace every draw
     HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->tile_req.p, tile_req.data(), (size_t)d.n_tiles * 4, hipMemcpyHostToDevice, c->stream));
     if (buffer) {
@@ -1000,6 +1131,7 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     L.normals = buffer ? c->normals.as<double>() : nullptr;
     L.uniforms = buffer && d.model == MP_MODEL_MERTON ? c->uniforms.as<double>() : nullptr;
     L.jump_normals = buffer && d.model == MP_MODEL_MERTON ? c->jumps.as<double>() : nullptr;
+    L.sampler = c->sampler.as<mp_sampler_dim>();
     L.terminal_out = want_paths ? c->term.as<double>() : nullptr;
     L.payoff_out = want_paths ? c->pay.as<double>() : nullptr;
     L.block = tu.block_size > 0 ? tu.block_size : 256;
@@ -1267,8 +1399,8 @@ MP_EXPORT int mp_hip_price_plain_fp64(mp_ctx* c, const mp_request* req, mp_resul
     if (rc != MP_OK) return rc;
     if (P.kind != KIND_TILE || P.d.model != MP_MODEL_GBM ||
         (P.d.payoff != MP_PAY_EUROPEAN && P.d.payoff != MP_PAY_ASIAN) || P.d.n_strikes != 1 ||
-        P.d.technique != MP_TECH_STANDARD)
-        return fail(MP_E_UNSUPPORTED, "the plain FP64 path prices single-strike GBM European/Asian calls (standard technique)");
+        P.d.technique != MP_TECH_STANDARD || P.d.sampling != MP_SAMPLER_NONE)
+        return fail(MP_E_UNSUPPORTED, "the plain FP64 path prices single-strike GBM European/Asian calls (standard technique, pseudorandom draws)");
     mp_dreq d = P.d;
     d.discount = std::exp(-req->r * req->t);
     const uint64_t n = d.n_paths;

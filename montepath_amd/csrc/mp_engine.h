@@ -10,6 +10,19 @@
 // Control-variate tiles use the first five slots of a partial row for the
 // discounted moments sum P, sum P^2, sum C, sum C^2, sum P*C (one strike).
 enum mp_technique { MP_TECH_STANDARD = 0, MP_TECH_ANTITHETIC = 1, MP_TECH_CONTROL_VARIATE = 2 };
+// Structured samplers drawn per (point, dimension) in the work-queue and
+// basket kernels (the ABI's sampling codes).
+enum mp_structured { MP_SAMPLER_NONE = 0, MP_SAMPLER_HALTON = 1, MP_SAMPLER_LHS = 2 };
+
+// One dimension of a structured sampler.  Randomized Halton: a = prime base,
+// shift = Cranley-Patterson shift.  Latin hypercube: a = stride multiplier,
+// b = offset, c = jitter seed.
+struct mp_sampler_dim {
+    uint64_t a, b, c;
+    double shift;
+};
+// Largest structured-sampler dimension count per request (32 B per dimension).
+#define MP_SAMPLER_MAX_DIMS 65536
 // LSM kernels run at most this many blocks: the last block reads every
 // block's partial sums in one round trip (two rows per thread).
 #define MP_LSM_MAX_BLOCKS 512
@@ -40,6 +53,9 @@ struct mp_dreq {
     double strikes[MP_NS_MAX];
     double discount;
     double cv_expect;         // E[discounted control] (control variate)
+    int32_t sampling;         // mp_structured: 0 = Philox (or caller buffers)
+    uint32_t sampler_off;     // first mp_sampler_dim of this request
+    uint64_t sampler_n;       // points the sampler is built over (requested n_paths)
     double v0, kappa, theta, xi, rho, rho_perp, rate, dt, sqrt_dt;
     double mu_j, sigma_j, jump_mean_dt;
     double cum[MP_POISSON_TERMS];
@@ -94,6 +110,7 @@ struct tile_launch {
     const double* jump_normals;
     double* terminal_out;
     double* payoff_out;
+    const mp_sampler_dim* sampler;  // structured-sampler tables of every request
     int grid;
     int block;
     int ppl;                    // paths interleaved per lane
@@ -109,7 +126,7 @@ hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, i
                       double* terminal_out, double* payoff_out, int precision, int grid, int block,
                       bool nib_lds, hipStream_t s);
 size_t qmc_lds_bytes(int precision, int factors, int n_slots, int block, bool nib_lds);
-hipError_t launch_basket(const mp_dreq* req, int n_assets, const double* chol,
+hipError_t launch_basket(const mp_dreq* req, const mp_sampler_dim* sampler, int n_assets, const double* chol,
                          const double* drift, const double* vol, const double* log_s0,
                          const double* weight, const double* normals, double* partial,
                          double* terminal_out, double* payoff_out, int precision, int grid,

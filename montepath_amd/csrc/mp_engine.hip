@@ -197,6 +197,50 @@ __device__ __forceinline__ unsigned int next_queue_position(unsigned int* counte
     return __builtin_amdgcn_readfirstlane(*slot);
 }
 
+// Standard normal of a structured sampler at (point, dimension), in FP64 for
+// both path precisions.  Randomized Halton (RandomizedHalton::standard_normal,
+// cpu.rs:5706-5713 with halton_with_shift :6063-6077): the radical inverse of
+// point + 1 in the dimension's prime base plus the dimension's shift, mod 1.
+// Latin hypercube (cpu.rs:5764-5778): stratum (multiplier * point + offset)
+// mod n, jitter from the dimension's seed hashed with the point, clamped to
+// [f64::MIN_POSITIVE, 1 - f64::EPSILON].  Then Acklam's inverse normal.  The
+// Halton digits are accumulated without multiply-add fusion, as the
+// reference rounds them.
+__device__ __noinline__ double structured_normal(const mp_sampler_dim* tab, const mp_dreq& q,
+                                                 uint64_t point, uint32_t dim) {
+#pragma clang fp contract(off)
+    const mp_sampler_dim sd = tab[q.sampler_off + dim];
+    double u;
+    if (q.sampling == MP_SAMPLER_HALTON) {
+        // point + 1 < 2^32 (requests hold at most 2^32 - 1 paths): 32-bit digits
+        const uint32_t base = (uint32_t)sd.a;
+        uint32_t index = (uint32_t)point + 1u;
+        double f = 1.0 / (double)base, value = 0.0;
+        while (index > 0) {
+            const uint32_t digit = index % base;
+            value = value + f * (double)digit;
+            index /= base;
+            f = f / (double)base;
+        }
+        value = value > 2.2250738585072014e-308 ? value : 2.2250738585072014e-308;
+        const double shifted = value + sd.shift;
+        u = shifted - floor(shifted);
+    } else {
+        const uint64_t n = q.sampler_n;
+        const uint64_t stratum = n == 0 ? 0 : (sd.a * point + sd.b) % n;
+        uint64_t h = (sd.c ^ (point * 0xBF58476D1CE4E5B9ull)) + 0x9E3779B97F4A7C15ull;
+        h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+        h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+        h ^= h >> 31;  // splitmix64
+        double jitter = (double)(h >> 11) * 0x1p-53;
+        jitter = jitter > 2.2250738585072014e-308 ? jitter : 2.2250738585072014e-308;
+        u = ((double)stratum + jitter) / (double)n;
+        u = u < 2.2250738585072014e-308 ? 2.2250738585072014e-308 : u;
+        u = u > 1.0 - 2.220446049250313e-16 ? 1.0 - 2.220446049250313e-16 : u;
+    }
+    return mp_acklam<double>(u);
+}
+
 // Adds one unit's contribution to the FP64 sums.  Standard: the payoff per
 // strike.  Antithetic: the pair estimate (payoff(+z) + payoff(-z)) / 2 per
 // strike (the reference's block estimate, cpu.rs:7807-7835).  Control
@@ -275,6 +319,10 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
                     R z[4];
                     if constexpr (SRC == 0) {
                         mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, z);
+                    } else if constexpr (SRC == 2) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            z[e] = e < nsub ? R(structured_normal(L.sampler, q, pr, (uint32_t)(s0i + e))) : R(0);
                     } else {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
@@ -292,6 +340,14 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
                     if constexpr (SRC == 0) {
                         mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, zv);
                         mp_normals4<R>((uint32_t)p, blk, 1u, q.seed_hi, q.key0, q.key1, zi);
+                    } else if constexpr (SRC == 2) {
+                        // factor f of step s takes dimension 2s + f
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const uint32_t d = (uint32_t)(2 * (s0i + e));
+                            zv[e] = e < nsub ? R(structured_normal(L.sampler, q, pr, d)) : R(0);
+                            zi[e] = e < nsub ? R(structured_normal(L.sampler, q, pr, d + 1)) : R(0);
+                        }
                     } else {
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
@@ -310,7 +366,7 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
                 } else {  // Merton (standard technique only)
                     R z[4];
                     double u[4];
-                    if constexpr (SRC == 0) {
+                    if constexpr (SRC != 1) {  // structured samplers are not offered for Merton
                         mp_normals4<R>((uint32_t)p, blk, 0u, q.seed_hi, q.key0, q.key1, z);
                         mp_uniforms4<double>((uint32_t)p, blk, 1u, q.seed_hi, q.key0, q.key1, u);
                     } else {
@@ -327,7 +383,7 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
                             const int nj = poisson_count(u[e], q);
                             R zj = R(0);
                             if (nj != 0) {
-                                if constexpr (SRC == 0) {
+                                if constexpr (SRC != 1) {
                                     R zz[4];
                                     mp_normals4<R>((uint32_t)p, blk, 2u, q.seed_hi, q.key0, q.key1, zz);
                                     zj = zz[e];
@@ -380,7 +436,10 @@ __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
         const uint32_t tile = L.order ? L.order[qpos] : qpos;
         const uint32_t ri = L.tile_req[tile];
         const mp_dreq& q = L.reqs[ri];
-        if constexpr (sizeof(R) == 4 && SRC == 0) {
+        if (SRC == 0 && q.sampling != MP_SAMPLER_NONE) {
+            // randomized Halton / Latin hypercube requests
+            if constexpr (SRC == 0) tile_body<R, 2, PPL, -1, -1, -1>(L, tile, q, lane);
+        } else if constexpr (sizeof(R) == 4 && SRC == 0) {
             // FP32 Philox tiles (batches, serving): a body per model and payoff
             const int mdl = q.model, pay = q.payoff;
             if (q.technique != MP_TECH_STANDARD) {
@@ -796,6 +855,7 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
                                                     const double* __restrict__ logs0,
                                                     const double* __restrict__ weight,
                                                     const double* __restrict__ normals,
+                                                    const mp_sampler_dim* __restrict__ sampler,
                                                     double* partial, double* terminal_out,
                                                     double* payoff_out) {
     const mp_dreq& q = *reqp;
@@ -826,8 +886,15 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 if constexpr (SRC == 0) {
-                    mp_normals4<R>((uint32_t)p, (uint32_t)(s0i >> 2), (uint32_t)a, q.seed_hi,
-                                   q.key0, q.key1, z[a]);
+                    if (q.sampling != MP_SAMPLER_NONE) {
+                        // structured sampler: asset a of step s is dimension s * NA + a
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            z[a][e] = e < nsub ? R(structured_normal(sampler, q, pr, (uint32_t)((s0i + e) * NA + a))) : R(0);
+                    } else {
+                        mp_normals4<R>((uint32_t)p, (uint32_t)(s0i >> 2), (uint32_t)a, q.seed_hi,
+                                       q.key0, q.key1, z[a]);
+                    }
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
@@ -1389,26 +1456,26 @@ hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, i
 template <class R, int NA>
 static hipError_t basket_na(const mp_dreq* req, const double* chol, const double* drift,
                             const double* vol, const double* logs0, const double* weight,
-                            const double* normals, double* partial, double* term, double* pay,
-                            int grid, int block, hipStream_t s) {
+                            const double* normals, const mp_sampler_dim* sampler, double* partial,
+                            double* term, double* pay, int grid, int block, hipStream_t s) {
     if (normals)
         hipLaunchKernelGGL((basket_kernel<R, NA, 1>), dim3(grid), dim3(block), 0, s, req, chol,
-                           drift, vol, logs0, weight, normals, partial, term, pay);
+                           drift, vol, logs0, weight, normals, sampler, partial, term, pay);
     else
         hipLaunchKernelGGL((basket_kernel<R, NA, 0>), dim3(grid), dim3(block), 0, s, req, chol,
-                           drift, vol, logs0, weight, normals, partial, term, pay);
+                           drift, vol, logs0, weight, normals, sampler, partial, term, pay);
     return hipGetLastError();
 }
 
 template <class R>
 static hipError_t basket_r(int na, const mp_dreq* req, const double* chol, const double* drift,
                            const double* vol, const double* logs0, const double* weight,
-                           const double* normals, double* partial, double* term, double* pay,
-                           int grid, int block, hipStream_t s) {
-#define MP_BASKET_CASE(N)                                                                         \
-    case N:                                                                                       \
-        return basket_na<R, N>(req, chol, drift, vol, logs0, weight, normals, partial, term, pay, \
-                               grid, block, s);
+                           const double* normals, const mp_sampler_dim* sampler, double* partial,
+                           double* term, double* pay, int grid, int block, hipStream_t s) {
+#define MP_BASKET_CASE(N)                                                                       \
+    case N:                                                                                     \
+        return basket_na<R, N>(req, chol, drift, vol, logs0, weight, normals, sampler, partial, \
+                               term, pay, grid, block, s);
     switch (na) {
         MP_BASKET_CASE(1)
         MP_BASKET_CASE(2)
@@ -1424,16 +1491,16 @@ static hipError_t basket_r(int na, const mp_dreq* req, const double* chol, const
 #undef MP_BASKET_CASE
 }
 
-hipError_t launch_basket(const mp_dreq* req, int n_assets, const double* chol,
-                         const double* drift, const double* vol, const double* log_s0,
-                         const double* weight, const double* normals, double* partial,
-                         double* terminal_out, double* payoff_out, int precision, int grid,
-                         int block, hipStream_t s) {
+hipError_t launch_basket(const mp_dreq* req, const mp_sampler_dim* sampler, int n_assets,
+                         const double* chol, const double* drift, const double* vol,
+                         const double* log_s0, const double* weight, const double* normals,
+                         double* partial, double* terminal_out, double* payoff_out, int precision,
+                         int grid, int block, hipStream_t s) {
     if (precision == 64)
-        return basket_r<double>(n_assets, req, chol, drift, vol, log_s0, weight, normals, partial,
-                                terminal_out, payoff_out, grid, block, s);
-    return basket_r<float>(n_assets, req, chol, drift, vol, log_s0, weight, normals, partial,
-                           terminal_out, payoff_out, grid, block, s);
+        return basket_r<double>(n_assets, req, chol, drift, vol, log_s0, weight, normals, sampler,
+                                partial, terminal_out, payoff_out, grid, block, s);
+    return basket_r<float>(n_assets, req, chol, drift, vol, log_s0, weight, normals, sampler,
+                           partial, terminal_out, payoff_out, grid, block, s);
 }
 
 hipError_t launch_lsm_fill(unsigned int* counters, int n_counters, int* valid, double* coeff,

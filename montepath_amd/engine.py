@@ -27,6 +27,8 @@ WORKLOADS = {
 }
 SAMPLING = {
     "pseudorandom": A.PSEUDORANDOM,
+    "randomized_halton": A.RANDOMIZED_HALTON,
+    "latin_hypercube": A.LATIN_HYPERCUBE,
     "scrambled_sobol": A.SCRAMBLED_SOBOL,
     "scrambled_sobol_brownian_bridge": A.SCRAMBLED_SOBOL_BRIDGE,
 }

@@ -131,12 +131,6 @@ def _engine_keys(cfg: Mapping) -> dict:
     return out
 
 
-def _engine_sampling(s: str) -> str:
-    if s in ("randomized_halton", "latin_hypercube"):
-        raise ValueError(f"rocm_hip does not offer sampling '{s}'; use pseudorandom or scrambled Sobol")
-    return s
-
-
 
 
 # ---------------------------------------------------------------- response
@@ -191,7 +185,7 @@ def _gbm_family(workload: str, function: str, cfg: Mapping, barrier: bool = Fals
     if barrier:
         c["barrier"] = _f(cfg, ["barrier"], 80.0)
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], s0=c["s0"], k=c["k"],
-              r=c["r"], sigma=c["sigma"], t=c["t"], sampling=_engine_sampling(c["sampling"]),
+              r=c["r"], sigma=c["sigma"], t=c["t"], sampling=c["sampling"],
               technique=c["technique"])
     if barrier:
         kw["barrier"] = c["barrier"]
@@ -247,7 +241,7 @@ def price_heston_european_call_rocm(config: Mapping) -> dict:
     if scheme not in ("reference_floor", "full_truncation"):
         raise ValueError(f"unsupported Heston scheme '{scheme}'")
     c["scheme"] = scheme
-    sampling = _engine_sampling(_sampling(config))
+    sampling = _sampling(config)
     c["sampling"] = sampling
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], s0=c["s0"], k=c["k"],
               r=c["r"], t=c["t"], v0=c["v0"], kappa=c["kappa"], theta=c["theta"],
@@ -287,9 +281,10 @@ def price_basket_call_rocm(config: Mapping) -> dict:
     asset_weight / correlation lists.  n_steps (default 1) walks the assets
     stepwise; one step is the reference's terminal basket."""
     technique = _technique(config)
-    sampling = _engine_sampling(_sampling(config))
-    if sampling != "pseudorandom":
-        raise ValueError("rocm_hip prices baskets with pseudorandom draws only")
+    sampling = _sampling(config)
+    if sampling.startswith("scrambled_sobol"):
+        raise ValueError("rocm_hip prices baskets with pseudorandom, randomized Halton or Latin "
+                         "hypercube draws")
     if config.get("asset_s0") is not None:
         s0 = [float(x) for x in config.get("asset_s0")]
         sig = [float(x) for x in config.get("asset_sigma")]
@@ -316,7 +311,7 @@ def price_basket_call_rocm(config: Mapping) -> dict:
     }
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], k=c["k"], r=c["r"],
               t=c["t"], asset_s0=s0, asset_sigma=sig, asset_weight=w, correlation=corr,
-              technique=technique)
+              technique=technique, sampling=sampling)
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)

@@ -555,7 +555,9 @@ static mpo_result gbm_tech(const mpo_gbm_cfg* cfg, int payoff, int technique,
         mpo_cv_moments m;
         memset(&m, 0, sizeof(m));
         for (uint64_t i = 0; i < count; ++i) cv_merge(&m, &ch[i].m);
-        r = mpo_summarize_cv(&m, cfg->s0);
+        /* standard (from caller normals only): the discounted payoff moments */
+        r = technique == MPO_STANDARD ? mpo_summarize(units, m.payoff_sum, m.payoff_sq_sum)
+                                      : mpo_summarize_cv(&m, cfg->s0);
     }
     free(ch);
     return r;
@@ -644,7 +646,9 @@ static mpo_result term_tech(const mpo_gbm_cfg* cfg, int technique, const double*
         mpo_cv_moments m;
         memset(&m, 0, sizeof(m));
         for (uint64_t i = 0; i < count; ++i) cv_merge(&m, &ch[i].m);
-        r = mpo_summarize_cv(&m, cfg->s0);
+        /* standard (from caller normals only): the discounted payoff moments */
+        r = technique == MPO_STANDARD ? mpo_summarize(units, m.payoff_sum, m.payoff_sq_sum)
+                                      : mpo_summarize_cv(&m, cfg->s0);
     }
     free(ch);
     return r;
@@ -868,7 +872,9 @@ static mpo_result heston_tech(const mpo_heston_cfg* cfg, int scheme, int techniq
         mpo_cv_moments m;
         memset(&m, 0, sizeof(m));
         for (uint64_t i = 0; i < count; ++i) cv_merge(&m, &ch[i].m);
-        r = mpo_summarize_cv(&m, cfg->s0);
+        /* standard (from caller normals only): the discounted payoff moments */
+        r = technique == MPO_STANDARD ? mpo_summarize(units, m.payoff_sum, m.payoff_sq_sum)
+                                      : mpo_summarize_cv(&m, cfg->s0);
     }
     free(ch);
     return r;

@@ -83,7 +83,11 @@ def test_valid_requests_report_footprints():
     ("heston_european_call", {"technique": "control_variate", "strikes": [80.0, 100.0, 120.0]},
      A.MP_E_INVALID_CONFIG,
      "one strike per request"),
-    ("european_call", {"sampling": 1}, A.MP_E_UNSUPPORTED, "sampling"),
+    ("european_call", {"sampling": 7}, A.MP_E_INVALID_CONFIG, "unknown sampling method"),
+    ("merton_jump_diffusion_call", {"sampling": "latin_hypercube"}, A.MP_E_UNSUPPORTED, "pseudorandom"),
+    ("american_put", {"sampling": "randomized_halton"}, A.MP_E_UNSUPPORTED, "pseudorandom"),
+    ("heston_european_call", {"sampling": "randomized_halton", "n_steps": 40000}, A.MP_E_UNSUPPORTED,
+     "at most 65536 dimensions"),
     ("european_call", {"precision": 16}, A.MP_E_INVALID_CONFIG, "precision"),
     ("european_call", {"n_paths": 1 << 33}, A.MP_E_UNSUPPORTED, "2^32"),
     ("down_and_out_call", {"barrier": 0.0}, A.MP_E_INVALID_CONFIG, "barrier must be > 0"),
@@ -112,6 +116,22 @@ def test_techniques_are_accepted_where_the_reference_offers_them():
             kw = basket if workload == "basket_call" else {}
             rc, b, err = footprint(make_request(workload, technique=tech, **kw))
             assert rc == A.MP_OK, (workload, tech, err)
+
+
+def test_structured_samplers_are_accepted_where_offered():
+    # the GBM family, terminal European and the basket as in the reference;
+    # Heston as an extension
+    basket = dict(asset_s0=[100.0, 95.0], asset_sigma=[0.2, 0.25], asset_weight=[0.5, 0.5],
+                  correlation=[[1.0, 0.35], [0.35, 1.0]])
+    for workload in ("european_call", "arithmetic_asian_call", "down_and_out_call", "lookback_call",
+                     "heston_european_call", "basket_call"):
+        for sampling in ("randomized_halton", "latin_hypercube"):
+            for tech in ("standard", "antithetic", "control_variate"):
+                kw = basket if workload == "basket_call" else {}
+                rc, _, err = footprint(make_request(workload, sampling=sampling, technique=tech, **kw))
+                assert rc == A.MP_OK, (workload, sampling, tech, err)
+    rc, _, err = footprint(make_request("european_call", method="terminal", sampling="latin_hypercube"))
+    assert rc == A.MP_OK, err
 
 
 def test_basket_validation():

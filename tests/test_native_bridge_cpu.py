@@ -100,7 +100,7 @@ def test_merton_defaults_are_terminal(fake):
 
 @pytest.mark.parametrize("fn,cfg,msg", [
     (native.price_european_call_rocm, {"technique": "bogus"}, "unsupported Monte Carlo technique"),
-    (native.price_european_call_rocm, {"sampling": "latin-hypercube"}, "does not offer sampling"),
+    (native.price_basket_call_rocm, {"sampling": "scrambled_sobol"}, "randomized Halton or Latin"),
     (native.price_european_call_rocm, {"sampling": "bogus"}, "unsupported sampling method"),
     (native.price_european_call_rocm, {"method": "bogus"}, "unsupported European call method"),
     (native.price_european_call_rocm, {"n_paths": -5}, "non-negative integer"),
@@ -118,6 +118,15 @@ def test_technique_reaches_the_request(fake):
     assert fake.requests[-1].technique == 2
     native.price_basket_call_rocm({"technique": "antithetic"})
     assert fake.requests[-1].technique == 1
+
+
+def test_sampling_reaches_the_request(fake):
+    native.price_european_call_rocm({"sampling": "latin-hypercube"})
+    assert fake.requests[-1].sampling == A.LATIN_HYPERCUBE
+    native.price_heston_european_call_rocm({"sampling": "randomized_halton"})
+    assert fake.requests[-1].sampling == A.RANDOMIZED_HALTON
+    native.price_basket_call_rocm({"sampling": "randomized_halton", "technique": "antithetic"})
+    assert fake.requests[-1].sampling == A.RANDOMIZED_HALTON and fake.requests[-1].technique == 1
 
 
 def test_engine_errors_become_value_errors():

@@ -121,6 +121,31 @@ def test_control_variate_summary_special_cases(oracle):
     assert oracle.summarize_cv(m2, 7.0) == oracle.summarize(len(xs), xs.sum(), (xs * xs).sum())
 
 
+def test_standard_from_normals_matches_path_pricers(oracle):
+    # the technique pricers fed caller normals with the standard technique
+    # summarize (summarize_payoffs, pinned by the published rows) the
+    # discounted payoffs of the per-path pricers on those normals
+    n, m = 3001, 12
+    disc = np.exp(-0.03)
+
+    def check(got, pay):
+        want = oracle.summarize(len(pay), pay.sum(), (pay * pay).sum())
+        assert abs(got[0] - want[0]) <= 1e-12 * want[0]
+        assert abs(got[1] - want[1]) <= 1e-9 * want[1]
+
+    g = oracle.Gbm(n_paths=n, n_steps=m)
+    z = oracle.rng_normals(5, n * m).reshape(n, m)
+    check(oracle.gbm_tech_from_normals(g, z, "european_call", "standard"),
+          oracle.gbm_paths_from_normals(g, z)[1])
+    zt = oracle.rng_normals(6, n)
+    check(oracle.terminal_tech_from_normals(oracle.Gbm(n_paths=n, n_steps=1), zt, "standard"),
+          np.maximum(100.0 * np.exp(0.03 - 0.02 + 0.2 * zt) - 100.0, 0.0) * disc)
+    h = oracle.Heston(n_paths=n, n_steps=m)
+    zh = oracle.rng_normals(7, n * m * 2).reshape(n, m, 2)
+    sh, _ = oracle.heston_paths_from_normals(h, zh, 1)
+    check(oracle.heston_tech_from_normals(h, zh, 1, "standard"), np.maximum(sh - 100.0, 0.0) * disc)
+
+
 @pytest.mark.parametrize("technique", ["antithetic", "control_variate"])
 @pytest.mark.parametrize("rho,factor", [(1.0, 0), (0.0, 1)])
 def test_heston_techniques_reduce_to_gbm_techniques(oracle, technique, rho, factor):
