@@ -8,6 +8,7 @@ file:line citations into ``crates/mc-core/src/runtime/cpu.rs``, live in
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass
 
@@ -55,6 +56,11 @@ class MertonCfg(C.Structure):
         ("lam", C.c_double), ("mu_j", C.c_double), ("sigma_j", C.c_double), ("t", C.c_double),
         ("n_paths", C.c_uint64), ("seed", C.c_uint64), ("n_threads", C.c_uint64),
     ]
+
+
+class BsGreeks(C.Structure):
+    _fields_ = [("price", C.c_double), ("delta", C.c_double), ("vega", C.c_double),
+                ("rho", C.c_double), ("theta", C.c_double)]
 
 
 class Basket2Cfg(C.Structure):
@@ -170,6 +176,8 @@ def lib() -> C.CDLL:
         "mpo_sobol_point_normals": (None, [C.c_uint32, C.c_uint64, C.c_uint64, _D]),
         "mpo_gbm_qmc_price": (Result, [C.POINTER(GbmCfg), C.c_int, C.c_int, _D]),
         "mpo_heston_qmc_paths": (None, [C.POINTER(HestonCfg), C.c_int, C.c_uint64, _D]),
+        "mpo_bs_greeks": (BsGreeks, [C.c_double] * 5),
+        "mpo_terminal_greeks": (None, [C.POINTER(GbmCfg), C.c_int, _D, _D]),
         "mpo_philox4x32_10": (None, [_U32, _U32, _U32]),
         "mpo_philox_normals": (
             None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, _D]),
@@ -723,3 +731,108 @@ def philox_uniforms(seed: int, contract: int, stream: int, path0: int, n_paths: 
     out = np.empty((n_paths, n))
     lib().mpo_philox_uniforms(seed, contract, stream, path0, n_paths, n, _pd(out))
     return out
+
+
+# ------------------------------------------------------------------ Greeks
+GREEK_NAMES = ("delta", "delta2", "vega", "vega2", "rho", "theta")
+_DBL_MIN = 2.2250738585072014e-308
+
+
+def bs_greeks(s0: float, k: float, r: float, sigma: float, t: float) -> dict:
+    """black_scholes_european_call_greeks (cpu.rs:866-892), A&S normal CDF."""
+    g = lib().mpo_bs_greeks(s0, k, r, sigma, t)
+    return {"price": g.price, "delta": g.delta, "vega": g.vega, "rho": g.rho, "theta": g.theta}
+
+
+def terminal_greeks(cfg: Gbm, estimator: str, normals: np.ndarray | None = None) -> dict:
+    """Pathwise (Delta, Vega, Rho, Theta) or likelihood-ratio (Delta, Vega,
+    Rho) terminal-GBM Greeks (cpu.rs:4670-4857): {greek: (value, stderr)}.
+    Draws: the single reference stream of cfg.seed, or normals[path]."""
+    code = {"pathwise": 1, "likelihood_ratio": 2}[estimator]
+    out = np.zeros(8)
+    buf = None
+    if normals is not None:
+        buf = np.ascontiguousarray(normals, dtype=np.float64)
+        assert buf.size == cfg.n_paths
+    c = cfg.c()
+    lib().mpo_terminal_greeks(C.byref(c), code, _pd(buf) if buf is not None else None, _pd(out))
+    names = ("delta", "vega", "rho", "theta") if code == 1 else ("delta", "vega", "rho")
+    return {n: (out[2 * i], out[2 * i + 1]) for i, n in enumerate(names)}
+
+
+def _bump_spot(spot):
+    return max(abs(spot) * 0.01, 0.01)
+
+
+def _bump_volatility(vol):
+    return max(abs(vol) * 0.05, 0.005)
+
+
+def _bump_maturity(t):
+    return min(1.0 / 365.0, t * 0.25)
+
+
+def _bump_estimate(greek, base_price, center, bump, price_at, reverse=False):
+    """push_bump_estimate (cpu.rs:4938-4992): central common-random-number
+    difference, down leg floored (Rho at -1, the others at f64::MIN_POSITIVE),
+    independent-leg standard error."""
+    floor = -1.0 if greek == "rho" else _DBL_MIN
+    lo = max(center - bump, floor)
+    up = price_at(center + bump)
+    down = price_at(lo)
+    eff = ((center + bump) - lo) * 0.5
+    raw = (up[0] - down[0]) / (2.0 * eff) if eff > 0.0 else 0.0
+    se = math.sqrt(up[1] * up[1] + down[1] * down[1]) / (2.0 * eff) if eff > 0.0 else 0.0
+    return {"greek": greek, "value": -raw if reverse else raw, "stderr": se, "bump_size": eff,
+            "base_price": base_price, "bumped_up_price": up[0], "bumped_down_price": down[0]}
+
+
+def bump_greeks(workload: str, cfg, scheme: int = HESTON_REFERENCE_FLOOR) -> list:
+    """Bump-and-revalue Greeks (european_call_bump_greeks_cpu cpu.rs:4409-4478,
+    the Asian / down-and-out / lookback arms :4047-4324, basket :4484-4590,
+    Heston :4592-4668) on the oracle pricers.  cfg: Gbm (GBM family), Heston,
+    or a dict of basket2_price keywords."""
+    from dataclasses import replace
+
+    if workload == "basket_call":
+        price = lambda **kw: basket2_price(**{**cfg, **kw})
+        base = price()
+        b = cfg
+        return [
+            _bump_estimate("delta", base[0], b["s01"], _bump_spot(b["s01"]), lambda v: price(s01=v)),
+            _bump_estimate("delta2", base[0], b["s02"], _bump_spot(b["s02"]), lambda v: price(s02=v)),
+            _bump_estimate("vega", base[0], b["sigma1"], _bump_volatility(b["sigma1"]),
+                           lambda v: price(sigma1=v)),
+            _bump_estimate("vega2", base[0], b["sigma2"], _bump_volatility(b["sigma2"]),
+                           lambda v: price(sigma2=v)),
+            _bump_estimate("rho", base[0], b["r"], 0.001, lambda v: price(r=v)),
+            _bump_estimate("theta", base[0], b["t"], _bump_maturity(b["t"]), lambda v: price(t=v),
+                           reverse=True),
+        ]
+    if workload == "heston_european_call":
+        price = lambda c: heston_price(c, scheme)
+        base = price(cfg)
+        vol = math.sqrt(max(cfg.v0, 0.0))
+        return [
+            _bump_estimate("delta", base[0], cfg.s0, _bump_spot(cfg.s0),
+                           lambda v: price(replace(cfg, s0=v))),
+            _bump_estimate("vega", base[0], vol, _bump_volatility(vol),
+                           lambda v: price(replace(cfg, v0=max(v, 0.0) * max(v, 0.0)))),
+            _bump_estimate("rho", base[0], cfg.r, 0.001, lambda v: price(replace(cfg, r=v))),
+            _bump_estimate("theta", base[0], cfg.t, _bump_maturity(cfg.t),
+                           lambda v: price(replace(cfg, t=v)), reverse=True),
+        ]
+    if workload == "european_call":
+        price = european_terminal_price
+    else:
+        price = lambda c: gbm_price(c, workload)
+    base = price(cfg)
+    return [
+        _bump_estimate("delta", base[0], cfg.s0, _bump_spot(cfg.s0),
+                       lambda v: price(replace(cfg, s0=v))),
+        _bump_estimate("vega", base[0], cfg.sigma, _bump_volatility(cfg.sigma),
+                       lambda v: price(replace(cfg, sigma=v))),
+        _bump_estimate("rho", base[0], cfg.r, 0.001, lambda v: price(replace(cfg, r=v))),
+        _bump_estimate("theta", base[0], cfg.t, _bump_maturity(cfg.t),
+                       lambda v: price(replace(cfg, t=v)), reverse=True),
+    ]

@@ -2161,3 +2161,98 @@ MPO_EXPORT void mpo_philox_uniforms(uint64_t seed, uint32_t contract, uint32_t s
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Greeks: black_scholes_european_call_greeks cpu.rs:866-896; pathwise */
+/* and likelihood-ratio single-pass estimators cpu.rs:4670-4892 with   */
+/* MomentAccumulator (sample variance n-1).  Bump-and-revalue is the   */
+/* composition of the pricers above (oracle/cpu.py).                   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double price, delta, vega, rho, theta;
+} mpo_bs_greeks_t;
+
+MPO_EXPORT mpo_bs_greeks_t mpo_bs_greeks(double s0, double k, double r, double sigma, double t) {
+    double sqrt_t = sqrt(t);
+    double d1 = (log(s0 / k) + (r + 0.5 * sigma * sigma) * t) / (sigma * sqrt_t);
+    double d2 = d1 - sigma * sqrt_t;
+    double discount = exp(-r * t);
+    double pdf_d1 = exp(-0.5 * d1 * d1) / sqrt(2.0 * M_PI);
+    mpo_bs_greeks_t g;
+    g.price = s0 * mpo_norm_cdf_as(d1) - k * discount * mpo_norm_cdf_as(d2);
+    g.delta = mpo_norm_cdf_as(d1);
+    g.vega = s0 * pdf_d1 * sqrt_t;
+    g.rho = k * t * discount * mpo_norm_cdf_as(d2);
+    g.theta = -(s0 * pdf_d1 * sigma) / (2.0 * sqrt_t) - r * k * discount * mpo_norm_cdf_as(d2);
+    return g;
+}
+
+/* MomentAccumulator::mean_stderr cpu.rs:4861-4876 */
+static void moment_mean_stderr(uint64_t count, double sum, double sq_sum, double* mean,
+                               double* stderr_out) {
+    if (count == 0) {
+        *mean = 0.0;
+        *stderr_out = 0.0;
+        return;
+    }
+    double n = (double)count;
+    *mean = sum / n;
+    double variance = 0.0;
+    if (count > 1) {
+        variance = (sq_sum - sum * sum / n) / (n - 1.0);
+        if (!(variance > 0.0)) variance = 0.0;
+    }
+    *stderr_out = sqrt(variance / n);
+}
+
+/* Single-pass terminal-GBM Greeks.  estimator 1 = pathwise (Delta, Vega,
+ * Rho, Theta), 2 = likelihood ratio (Delta, Vega, Rho; slot 3 zero).  The
+ * draws are the reference stream MonteCarloRng::new(seed) (one stream for
+ * every path, whatever n_threads), or the caller's normals[path] when
+ * normals is not NULL.  out[2g] = value, out[2g+1] = stderr. */
+MPO_EXPORT void mpo_terminal_greeks(const mpo_gbm_cfg* cfg, int estimator, const double* normals,
+                                    double* out) {
+    mpo_rng g;
+    rng_init(&g, cfg->seed);
+    double sqrt_t = sqrt(cfg->t);
+    double drift_t = (cfg->r - 0.5 * cfg->sigma * cfg->sigma) * cfg->t;
+    double vol_t = cfg->sigma * sqrt_t;
+    double discount = exp(-cfg->r * cfg->t);
+    double sum[4] = {0, 0, 0, 0}, sq[4] = {0, 0, 0, 0};
+    for (uint64_t p = 0; p < cfg->n_paths; ++p) {
+        double z = normals ? normals[p] : rng_normal(&g);
+        double st = cfg->s0 * exp(drift_t + vol_t * z);
+        double v[4] = {0, 0, 0, 0};
+        if (estimator == 1) {
+            double intrinsic = st - cfg->k;
+            int active = intrinsic > 0.0;
+            double payoff = fmax_ref(intrinsic, 0.0);
+            double discounted = discount * payoff;
+            v[0] = active ? discount * st / cfg->s0 : 0.0;
+            v[1] = active ? discount * st * (-cfg->sigma * cfg->t + sqrt_t * z) : 0.0;
+            v[2] = active ? discount * (cfg->t * st - cfg->t * payoff) : -cfg->t * discounted;
+            double dst_dt =
+                st * (cfg->r - 0.5 * cfg->sigma * cfg->sigma + cfg->sigma * z / (2.0 * sqrt_t));
+            double dp = active ? discount * (dst_dt - cfg->r * payoff) : -cfg->r * discounted;
+            v[3] = -dp;
+        } else {
+            double discounted = discount * fmax_ref(st - cfg->k, 0.0);
+            v[0] = discounted * z / (cfg->s0 * cfg->sigma * sqrt_t);
+            double vega_score = (z * z - 1.0) / cfg->sigma - sqrt_t * z;
+            v[1] = discounted * vega_score;
+            v[2] = discounted * (z / (cfg->sigma * sqrt_t) - cfg->t);
+        }
+        int n_g = estimator == 1 ? 4 : 3;
+        for (int i = 0; i < n_g; ++i) {
+            sum[i] += v[i];
+            sq[i] += v[i] * v[i];
+        }
+    }
+    int n_g = estimator == 1 ? 4 : 3;
+    for (int i = 0; i < 4; ++i) {
+        if (i < n_g)
+            moment_mean_stderr(cfg->n_paths, sum[i], sq[i], &out[2 * i], &out[2 * i + 1]);
+        else
+            out[2 * i] = out[2 * i + 1] = 0.0;
+    }
+}

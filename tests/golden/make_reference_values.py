@@ -96,6 +96,23 @@ QUALITY_ROWS = {
                          "technique": "control_variate", "seed": 7311}),
 }
 
+# |Delta - Black-Scholes Delta| (A&S CDF) at GREEK_PATHS = 50_000, GREEK_STEPS = 64
+# (harness.rs:45-46, :4080-4184); the count row is 6 workloads' estimates
+GREEK_ROWS = {
+    "mc_cpu_european_call_greeks_bump_rust": (
+        "european_greeks", {"estimator": "bump_and_revalue", "seed": 9401}),
+    "mc_cpu_european_call_greeks_pathwise_rust": (
+        "european_greeks", {"estimator": "pathwise", "seed": 9401}),
+    "mc_cpu_european_call_greeks_likelihood_ratio_rust": (
+        "european_greeks", {"estimator": "likelihood_ratio", "seed": 9401}),
+    "mc_cpu_heston_greeks_black_scholes_limit_delta_quality": (
+        "heston_greeks", {"estimator": "bump_and_revalue", "seed": 9402, "kappa": 1.5,
+                          "theta": 0.04, "v0": 0.04, "xi": 0.0, "rho": 0.0}),
+    "mc_cpu_all_workload_greeks_bump_rust": (
+        "all_workload_greeks", {"estimator": "bump_and_revalue", "seed": 9403,
+                                "n_paths": 25_000}),
+}
+
 COMMON = {"s0": 100.0, "k": 100.0, "r": 0.03, "sigma": 0.2, "t": 1.0, "n_paths": 100_000,
           "n_steps": 64, "seed": 42, "n_threads": 8, "repeats": 3}
 
@@ -128,7 +145,18 @@ def main() -> None:
             full["repeats"] = 1
             quality[name] = {"pricer": kind, "config": full, "metric_name": r["metric_name"],
                              "published_value": r["metric_value"]}
+    greeks = {}
+    for r in release["results"]:
+        name = r["benchmark_name"]
+        if name in GREEK_ROWS:
+            kind, cfg = GREEK_ROWS[name]
+            full = dict(COMMON)
+            full.update({"n_paths": 50_000, "repeats": 1})
+            full.update(cfg)
+            greeks[name] = {"pricer": kind, "config": full, "metric_name": r["metric_name"],
+                            "published_value": r["metric_value"]}
     out = {
+        "greek_rows": greeks,
         "source": "montepath benchmarks/reference-fixtures.json and benchmarks/release-results.json",
         "quality_rows": quality,
         "note": "published rows are means over seeds seed..seed+2 with n_threads=8",
@@ -137,7 +165,8 @@ def main() -> None:
     }
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
-    print(OUT, len(fx), "fixtures", len(rows), "rows", len(quality), "quality rows")
+    print(OUT, len(fx), "fixtures", len(rows), "rows", len(quality), "quality rows", len(greeks),
+          "Greek rows")
 
 
 if __name__ == "__main__":

@@ -104,6 +104,89 @@ def test_published_technique_stderr_ratios(oracle, name):
     assert abs(ratio - want) <= 1e-12 * want, (name, ratio, want)
 
 
+GREEK_ROWS = GOLD["greek_rows"]
+
+
+def _all_workload_bump_greeks(oracle, n_paths, n_steps, seed, n_threads):
+    # price_all_current_greeks_bump_and_revalue_cpu (cpu.rs:4358-4407): the
+    # default configs of six workloads, seeds seed..seed+5
+    g = lambda i, **kw: oracle.Gbm(n_paths=n_paths, n_steps=n_steps, seed=seed + i,
+                                   n_threads=n_threads, **kw)
+    basket = {"s01": 100.0, "s02": 95.0, "k": 100.0, "r": 0.03, "sigma1": 0.2, "sigma2": 0.25,
+              "rho": 0.35, "weight1": 0.5, "weight2": 0.5, "t": 1.0, "n_paths": n_paths,
+              "seed": seed + 4}
+    heston = oracle.Heston(n_paths=n_paths, n_steps=n_steps, seed=seed + 5, n_threads=n_threads,
+                           kappa=1.5, theta=0.04, v0=0.04, xi=0.3, rho=-0.6)
+    return [
+        oracle.bump_greeks("european_call", g(0)),
+        oracle.bump_greeks("arithmetic_asian_call", g(1)),
+        oracle.bump_greeks("down_and_out_call", g(2, barrier=80.0)),
+        oracle.bump_greeks("lookback_call", g(3)),
+        oracle.bump_greeks("basket_call", basket),
+        oracle.bump_greeks("heston_european_call", heston),
+    ]
+
+
+@pytest.mark.parametrize("name", sorted(GREEK_ROWS))
+def test_published_greek_rows(oracle, name):
+    # |Delta - Black-Scholes Delta| of the bump-and-revalue, pathwise and
+    # likelihood-ratio estimators and the Heston Black-Scholes limit, bit for
+    # bit; the all-workload row counts the estimates (26) and needs them finite
+    row = GREEK_ROWS[name]
+    c = row["config"]
+    want = row["published_value"]
+    bs_delta = oracle.bs_greeks(100.0, 100.0, 0.03, 0.2, 1.0)["delta"]
+    if row["pricer"] == "european_greeks":
+        g = oracle.Gbm(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"],
+                       n_threads=c["n_threads"])
+        if c["estimator"] == "bump_and_revalue":
+            delta = oracle.bump_greeks("european_call", g)[0]["value"]
+        else:
+            delta = oracle.terminal_greeks(g, c["estimator"])["delta"][0]
+        assert abs(delta - bs_delta) == want, (name, abs(delta - bs_delta), want)
+    elif row["pricer"] == "heston_greeks":
+        h = oracle.Heston(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"],
+                          n_threads=c["n_threads"], kappa=c["kappa"], theta=c["theta"], v0=c["v0"],
+                          xi=c["xi"], rho=c["rho"])
+        delta = oracle.bump_greeks("heston_european_call", h)[0]["value"]
+        assert abs(delta - bs_delta) == want, (name, abs(delta - bs_delta), want)
+    else:
+        reports = _all_workload_bump_greeks(oracle, c["n_paths"], c["n_steps"], c["seed"],
+                                            c["n_threads"])
+        assert sum(len(r) for r in reports) == want
+        assert all(np.isfinite(e["value"]) for r in reports for e in r)
+
+
+def test_greek_estimators_against_black_scholes(oracle):
+    # the reference's own test tolerances (cpu_runtime.rs:1568-1636) and the
+    # likelihood-ratio / pathwise estimators on one shared normal buffer: the
+    # pathwise Delta of a call is E[disc S_T/S0 1{S_T>K}], the LR Delta
+    # E[disc payoff z / (S0 sigma sqrt T)]; both agree with Black-Scholes
+    bs = oracle.bs_greeks(100.0, 100.0, 0.03, 0.2, 1.0)
+    g = oracle.Gbm(n_paths=200_000, n_steps=64, seed=14_001, n_threads=8)
+    est = {e["greek"]: e for e in oracle.bump_greeks("european_call", g)}
+    assert abs(est["delta"]["value"] - bs["delta"]) < 0.03
+    assert abs(est["vega"]["value"] - bs["vega"]) < 2.0
+    assert abs(est["rho"]["value"] - bs["rho"]) < 3.0
+    assert abs(est["theta"]["value"] - bs["theta"]) < 2.0
+    z = oracle.rng_normals(3, 400_000)
+    g2 = oracle.Gbm(n_paths=400_000, n_steps=1)
+    pw = oracle.terminal_greeks(g2, "pathwise", z)
+    lr = oracle.terminal_greeks(g2, "likelihood_ratio", z)
+    for name in ("delta", "vega", "rho"):
+        for e in (pw, lr):
+            assert abs(e[name][0] - bs[name]) <= 4.0 * e[name][1], (name, e[name], bs[name])
+    assert abs(pw["theta"][0] - bs["theta"]) <= 4.0 * pw["theta"][1]
+    # the pathwise estimator's per-path Delta is the bump estimator's limit:
+    # a tiny CRN bump on the same draws equals it to the bump's order
+    st = 100.0 * np.exp(0.03 - 0.02 + 0.2 * z)
+    disc = np.exp(-0.03)
+    h = 1e-4
+    fd = (np.maximum(st * (1 + h) - 100.0, 0.0) - np.maximum(st * (1 - h) - 100.0, 0.0)) \
+        * disc / (2.0 * 100.0 * h)
+    assert abs(fd.mean() - pw["delta"][0]) < 1e-4
+
+
 def test_control_variate_summary_special_cases(oracle):
     # a control that is an exact linear function of the payoff removes all
     # variance; a constant control falls back to the plain summary
