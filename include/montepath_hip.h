@@ -153,6 +153,35 @@ typedef struct mp_result {
     uint64_t regression_steps;
 } mp_result;
 
+/* Greeks (reference Greek / GreekEstimator / GreekEstimate, runtime/cpu.rs:
+ * 140-170).  Up to MP_MAX_GREEKS estimates per request. */
+#define MP_MAX_GREEKS 6
+typedef enum mp_greek {
+    MP_GREEK_DELTA = 0,
+    MP_GREEK_DELTA2 = 1,     /* basket: second asset's spot */
+    MP_GREEK_VEGA = 2,
+    MP_GREEK_VEGA2 = 3,      /* basket: second asset's volatility */
+    MP_GREEK_RHO = 4,
+    MP_GREEK_THETA = 5       /* calendar convention: -dV/dT */
+} mp_greek;
+
+typedef enum mp_greek_estimator {
+    MP_GREEKS_BUMP_AND_REVALUE = 0,
+    MP_GREEKS_PATHWISE = 1,
+    MP_GREEKS_LIKELIHOOD_RATIO = 2
+} mp_greek_estimator;
+
+typedef struct mp_greek_estimate {
+    int32_t greek;             /* mp_greek */
+    int32_t estimator;         /* mp_greek_estimator */
+    double value;
+    double std_error;          /* bump: independent-leg propagation; single pass: sample SE */
+    double bump_size;          /* effective half-width of the central difference; NaN single pass */
+    double base_price;
+    double bumped_up_price;    /* NaN for single-pass estimators */
+    double bumped_down_price;
+} mp_greek_estimate;
+
 /* Device description (reference DeviceInfo, backend/mod.rs:32-43). */
 typedef struct mp_device_info {
     char device_id[32];      /* "rocm:<ordinal>" */
@@ -241,6 +270,34 @@ int mp_hip_price_from_normals(mp_ctx* ctx, const mp_request* req, const double* 
 int mp_hip_paths_from_normals(mp_ctx* ctx, const mp_request* req, const double* normals,
                               const double* uniforms, const double* jump_normals,
                               double* terminal_out, double* payoff_out);
+
+/* Greeks of n requests.  out holds MP_MAX_GREEKS estimates per request
+ * (row i starts at out + i * MP_MAX_GREEKS), n_estimates[i] how many were
+ * written, base[i] the request priced as given.
+ *
+ * Bump-and-revalue (european_call_bump_greeks_cpu and the Asian, barrier,
+ * lookback, basket and Heston arms, runtime/cpu.rs:4036-4668, with
+ * push_bump_estimate :4938-4992 and bump_spot/volatility/rate/maturity
+ * :4994-5008): every request's up and down legs are priced with its own seed
+ * and contract id -- common random numbers -- and all legs of all requests
+ * run in one persistent work-queue launch (baskets in their own kernel).
+ * GBM family: Delta, Vega, Rho, Theta; Heston: the same with Vega on
+ * sqrt(v0); basket: Delta, Delta2, Vega, Vega2 (assets 0 and 1), Rho, Theta.
+ *
+ * Pathwise / likelihood ratio (european_call_pathwise_greeks_cpu,
+ * european_call_likelihood_ratio_greeks_cpu, :4670-4857): European calls
+ * only, from the terminal draws of the request's Philox stream whatever its
+ * method, technique or sampling (the reference likewise uses plain terminal
+ * shocks); pathwise gives Delta, Vega, Rho, Theta, likelihood ratio Delta,
+ * Vega, Rho.  Other workloads: MP_E_UNSUPPORTED (as the reference reports). */
+int mp_hip_greeks(mp_ctx* ctx, const mp_request* reqs, size_t n, int32_t estimator,
+                  mp_result* base, mp_greek_estimate* out, int32_t* n_estimates);
+
+/* Parity hook: the pathwise / likelihood-ratio Greeks of one European call
+ * from caller normals (one terminal draw per path, n_paths doubles). */
+int mp_hip_greeks_from_normals(mp_ctx* ctx, const mp_request* req, int32_t estimator,
+                               const double* normals, mp_result* base, mp_greek_estimate* out,
+                               int32_t* n_estimates);
 
 /* Price one request and also return, per path, the terminal value (basket
  * value for a basket) and the undiscounted payoff for strike 0, from the

@@ -48,6 +48,11 @@ HESTON_FULL_TRUNCATION = 1
 
 MAX_STRIKES = 4
 MAX_ASSETS = 8
+MAX_GREEKS = 6
+
+GREEK_DELTA, GREEK_DELTA2, GREEK_VEGA, GREEK_VEGA2, GREEK_RHO, GREEK_THETA = range(6)
+GREEK_NAMES = ("delta", "delta2", "vega", "vega2", "rho", "theta")
+GREEKS_BUMP_AND_REVALUE, GREEKS_PATHWISE, GREEKS_LIKELIHOOD_RATIO = 0, 1, 2
 
 
 class Request(C.Structure):
@@ -81,6 +86,14 @@ class Result(C.Structure):
         ("n_paths", C.c_uint64), ("n_steps", C.c_uint64),
         ("early_exercise_count", C.c_uint64), ("maturity_exercise_count", C.c_uint64),
         ("regression_steps", C.c_uint64),
+    ]
+
+
+class GreekEstimate(C.Structure):
+    _fields_ = [
+        ("greek", C.c_int32), ("estimator", C.c_int32), ("value", C.c_double),
+        ("std_error", C.c_double), ("bump_size", C.c_double), ("base_price", C.c_double),
+        ("bumped_up_price", C.c_double), ("bumped_down_price", C.c_double),
     ]
 
 
@@ -141,6 +154,11 @@ SIGNATURES = {
     "mp_hip_price_plain_fp64": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Result)]),
     "mp_hip_paths_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, _D, _D, _D, _D]),
     "mp_hip_price_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, _D, _D, C.POINTER(Result)]),
+    "mp_hip_greeks": (C.c_int, [_P, C.POINTER(Request), C.c_size_t, C.c_int32, C.POINTER(Result),
+                                C.POINTER(GreekEstimate), C.POINTER(C.c_int32)]),
+    "mp_hip_greeks_from_normals": (C.c_int, [_P, C.POINTER(Request), C.c_int32, _D,
+                                             C.POINTER(Result), C.POINTER(GreekEstimate),
+                                             C.POINTER(C.c_int32)]),
     "mp_hip_price_with_paths": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Result), _D, _D]),
     "mp_hip_lsm_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, C.POINTER(Result), _D,
                                           _U32]),

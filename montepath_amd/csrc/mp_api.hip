@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -404,6 +405,7 @@ int prepare_request(const mp_request& r, prepared& P) {
             d.sqrt_dt = std::sqrt(d.dt);
             d.drift_dt = (rr - 0.5 * sig * sig) * d.dt;
             d.vol_dt = sig * std::sqrt(d.dt);
+            d.sigma = sig;
             if (qmc) {
                 P.kind = KIND_QMC;
                 P.factors = 1;
@@ -1498,6 +1500,248 @@ MP_EXPORT int mp_hip_price_with_paths(mp_ctx* c, const mp_request* req, mp_resul
             return run_tiles_single(c, P, P.precision, out, nullptr, nullptr, nullptr, terminal_out,
                                     payoff_out);
     }
+}
+
+// ===========================================================================
+// Greeks
+// ===========================================================================
+namespace {
+
+// bump_spot / bump_volatility / bump_rate / bump_maturity, cpu.rs:4994-5008
+double bump_spot(double s) { return std::max(std::fabs(s) * 0.01, 0.01); }
+double bump_volatility(double v) { return std::max(std::fabs(v) * 0.05, 0.005); }
+double bump_maturity(double t) { return std::min(1.0 / 365.0, t * 0.25); }
+
+enum greek_param { GP_S0, GP_SIGMA, GP_RATE, GP_MATURITY, GP_HESTON_VOL, GP_ASSET_S0, GP_ASSET_SIGMA };
+
+struct greek_spec {
+    int greek;
+    int param;
+    int asset;
+    double center, bump;
+    bool reverse;  // Theta: the calendar convention negates dV/dT
+};
+
+void set_param(mp_request& r, const greek_spec& g, double v) {
+    switch (g.param) {
+        case GP_S0: r.s0 = v; break;
+        case GP_SIGMA: r.sigma = v; break;
+        case GP_RATE: r.r = v; break;
+        case GP_MATURITY: r.t = v; break;
+        case GP_HESTON_VOL: {
+            const double x = std::max(v, 0.0);  // Vega in volatility units: v0 = vol^2
+            r.v0 = x * x;
+            break;
+        }
+        case GP_ASSET_S0: r.asset_s0[g.asset] = v; break;
+        default: r.asset_sigma[g.asset] = v; break;
+    }
+}
+
+// parameter_floor, cpu.rs:4985-4992: Rho may go to -1, the rest stay positive
+double param_floor(int greek) { return greek == MP_GREEK_RHO ? -1.0 : DBL_MIN; }
+
+// The bumped parameters of one request, in the reference's order.
+int bump_specs(const mp_request& r, std::vector<greek_spec>& out) {
+    out.clear();
+    auto add = [&](int greek, int param, int asset, double center, double bump, bool rev) {
+        out.push_back(greek_spec{greek, param, asset, center, bump, rev});
+    };
+    switch (r.workload) {
+        case MP_EUROPEAN_CALL:
+        case MP_ARITHMETIC_ASIAN_CALL:
+        case MP_DOWN_AND_OUT_CALL:
+        case MP_LOOKBACK_CALL:
+            add(MP_GREEK_DELTA, GP_S0, 0, r.s0, bump_spot(r.s0), false);
+            add(MP_GREEK_VEGA, GP_SIGMA, 0, r.sigma, bump_volatility(r.sigma), false);
+            add(MP_GREEK_RHO, GP_RATE, 0, r.r, 0.001, false);
+            add(MP_GREEK_THETA, GP_MATURITY, 0, r.t, bump_maturity(r.t), true);
+            return MP_OK;
+        case MP_HESTON_EUROPEAN_CALL: {  // heston_call_bump_greeks_cpu cpu.rs:4592-4668
+            const double vol = std::sqrt(std::max(r.v0, 0.0));
+            add(MP_GREEK_DELTA, GP_S0, 0, r.s0, bump_spot(r.s0), false);
+            add(MP_GREEK_VEGA, GP_HESTON_VOL, 0, vol, bump_volatility(vol), false);
+            add(MP_GREEK_RHO, GP_RATE, 0, r.r, 0.001, false);
+            add(MP_GREEK_THETA, GP_MATURITY, 0, r.t, bump_maturity(r.t), true);
+            return MP_OK;
+        }
+        case MP_BASKET_CALL:  // basket_call_bump_greeks_cpu cpu.rs:4484-4590
+            if (r.n_assets < 1 || r.n_assets > MP_MAX_ASSETS)
+                return fail(MP_E_INVALID_CONFIG, "n_assets must be in 1..8");
+            add(MP_GREEK_DELTA, GP_ASSET_S0, 0, r.asset_s0[0], bump_spot(r.asset_s0[0]), false);
+            if (r.n_assets > 1)
+                add(MP_GREEK_DELTA2, GP_ASSET_S0, 1, r.asset_s0[1], bump_spot(r.asset_s0[1]), false);
+            add(MP_GREEK_VEGA, GP_ASSET_SIGMA, 0, r.asset_sigma[0], bump_volatility(r.asset_sigma[0]), false);
+            if (r.n_assets > 1)
+                add(MP_GREEK_VEGA2, GP_ASSET_SIGMA, 1, r.asset_sigma[1],
+                    bump_volatility(r.asset_sigma[1]), false);
+            add(MP_GREEK_RHO, GP_RATE, 0, r.r, 0.001, false);
+            add(MP_GREEK_THETA, GP_MATURITY, 0, r.t, bump_maturity(r.t), true);
+            return MP_OK;
+        default:
+            return fail(MP_E_UNSUPPORTED,
+                        "bump-and-revalue Greeks cover the GBM family, Heston and the basket");
+    }
+}
+
+void nan_estimate(mp_greek_estimate& e) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    e.value = e.std_error = e.bump_size = e.base_price = e.bumped_up_price = e.bumped_down_price = nan;
+}
+
+// Every request's base and bumped legs in one batch (push_bump_estimate,
+// cpu.rs:4938-4983: central difference over the effective half-width, the
+// down leg floored; standard error from the two legs as if independent).
+int greeks_bump(mp_ctx* c, const mp_request* reqs, size_t n, mp_result* base, mp_greek_estimate* out,
+                int32_t* n_est) {
+    std::vector<std::vector<greek_spec>> specs(n);
+    std::vector<mp_request> legs;
+    std::vector<size_t> first(n);
+    for (size_t i = 0; i < n; ++i) {
+        int rc = bump_specs(reqs[i], specs[i]);
+        if (rc != MP_OK) return fail(rc, "request " + std::to_string(i) + ": " + g_err);
+        first[i] = legs.size();
+        legs.push_back(reqs[i]);
+        for (const greek_spec& g : specs[i]) {
+            mp_request up = reqs[i], down = reqs[i];
+            set_param(up, g, g.center + g.bump);
+            set_param(down, g, std::max(g.center - g.bump, param_floor(g.greek)));
+            legs.push_back(up);
+            legs.push_back(down);
+        }
+    }
+    std::vector<mp_result> res(legs.size());
+    int rc = mp_hip_price_batch(c, legs.data(), legs.size(), res.data());
+    if (rc != MP_OK) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        const mp_result& b = res[first[i]];
+        base[i] = b;
+        n_est[i] = (int32_t)specs[i].size();
+        for (size_t j = 0; j < specs[i].size(); ++j) {
+            const greek_spec& g = specs[i][j];
+            const mp_result& up = res[first[i] + 1 + 2 * j];
+            const mp_result& down = res[first[i] + 2 + 2 * j];
+            const double lo = std::max(g.center - g.bump, param_floor(g.greek));
+            const double eff = ((g.center + g.bump) - lo) * 0.5;
+            const double raw = eff > 0.0 ? (up.price - down.price) / (2.0 * eff) : 0.0;
+            mp_greek_estimate& e = out[i * MP_MAX_GREEKS + j];
+            e.greek = g.greek;
+            e.estimator = MP_GREEKS_BUMP_AND_REVALUE;
+            e.value = g.reverse ? -raw : raw;
+            e.std_error = eff > 0.0 ? std::sqrt(up.std_error * up.std_error + down.std_error * down.std_error) /
+                                          (2.0 * eff)
+                                    : 0.0;
+            e.bump_size = eff;
+            e.base_price = b.price;
+            e.bumped_up_price = up.price;
+            e.bumped_down_price = down.price;
+        }
+    }
+    return MP_OK;
+}
+
+// Pathwise / likelihood-ratio Greeks of one European call: the base price as
+// requested (or from the caller's normals, terminal method), then one pass
+// of greeks_kernel over the terminal draws and MomentAccumulator summaries
+// (sample variance, cpu.rs:4859-4876).
+int greeks_single(mp_ctx* c, const mp_request& r, int estimator, const double* normals,
+                  mp_result* base, mp_greek_estimate* out, int32_t* n_est) {
+    if (r.workload != MP_EUROPEAN_CALL)
+        return fail(MP_E_UNSUPPORTED,
+                    "pathwise and likelihood-ratio Greeks are offered for European calls only; use "
+                    "bump-and-revalue");
+    if (!(r.sigma > 0.0)) return fail(MP_E_INVALID_CONFIG, "sigma must be > 0 for single-pass Greeks");
+    mp_request rt = r;  // terminal draws, plain Philox stream, strike k
+    rt.method = MP_METHOD_TERMINAL;
+    rt.technique = MP_TECH_STANDARD;
+    rt.sampling = MP_PSEUDORANDOM;
+    rt.n_strikes = 0;
+    rt.fp64_check = 0;
+    prepared P;
+    int rc = prepare_request(rt, P);
+    if (rc != MP_OK) return rc;
+    const int precision = P.precision;
+    if (normals) {
+        rc = run_tiles_single(c, P, precision, base, normals, nullptr, nullptr, nullptr, nullptr);
+    } else {
+        rc = price_one(c, r, r.precision == 64 ? 64 : 32, base);
+    }
+    if (rc != MP_OK) return rc;
+    mp_dreq d = P.d;
+    const int block = 256;
+    const uint64_t want = (d.n_paths + block - 1) / block;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)c->n_cu * 8));
+    d.tile0 = 0;
+    d.n_tiles = (uint32_t)grid;
+    d.technique = MP_TECH_GREEK_MOMENTS;
+    ENSURE(c->reqs, sizeof(mp_dreq));
+    ENSURE(c->partial, (size_t)grid * MP_PARTIAL_STRIDE * 8);
+    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
+    if (normals) ENSURE(c->normals, d.n_paths * 8);
+    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    if (normals) HIPCHK(hipMemcpyAsync(c->normals.p, normals, d.n_paths * 8, hipMemcpyHostToDevice, c->stream));
+    const int est = estimator == MP_GREEKS_PATHWISE ? MP_GREEK_EST_PATHWISE : MP_GREEK_EST_LIKELIHOOD_RATIO;
+    HIPCHK(mp::launch_greeks(c->reqs.as<mp_dreq>(), est, normals ? c->normals.as<double>() : nullptr,
+                             c->partial.as<double>(), precision, grid, block, c->stream));
+    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->out.as<double>(),
+                               c->stream));
+    double tot[MP_PARTIAL_STRIDE];
+    HIPCHK(hipMemcpyAsync(tot, c->out.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    static const int pathwise[4] = {MP_GREEK_DELTA, MP_GREEK_VEGA, MP_GREEK_RHO, MP_GREEK_THETA};
+    const int count = est == MP_GREEK_EST_PATHWISE ? 4 : 3;
+    const double nn = (double)d.n_paths;
+    for (int j = 0; j < count; ++j) {
+        const double sum = tot[2 * j], sq = tot[2 * j + 1];
+        const double mean = sum / nn;
+        double var = d.n_paths > 1 ? (sq - sum * sum / nn) / (nn - 1.0) : 0.0;
+        if (!(var > 0.0)) var = 0.0;
+        mp_greek_estimate& e = out[j];
+        nan_estimate(e);
+        e.greek = pathwise[j];
+        e.estimator = estimator;
+        e.value = mean;
+        e.std_error = std::sqrt(var / nn);
+        e.base_price = base->price;
+    }
+    *n_est = count;
+    return MP_OK;
+}
+
+}  // namespace
+
+MP_EXPORT int mp_hip_greeks(mp_ctx* c, const mp_request* reqs, size_t n, int32_t estimator,
+                            mp_result* base, mp_greek_estimate* out, int32_t* n_estimates) {
+    if (!c || (!reqs && n) || ((!base || !out || !n_estimates) && n))
+        return fail(MP_E_INVALID_CONFIG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    for (size_t i = 0; i < n; ++i) {
+        n_estimates[i] = 0;
+        for (int j = 0; j < MP_MAX_GREEKS; ++j) nan_estimate(out[i * MP_MAX_GREEKS + j]);
+    }
+    if (estimator == MP_GREEKS_BUMP_AND_REVALUE) return greeks_bump(c, reqs, n, base, out, n_estimates);
+    if (estimator != MP_GREEKS_PATHWISE && estimator != MP_GREEKS_LIKELIHOOD_RATIO)
+        return fail(MP_E_INVALID_CONFIG, "estimator must be 0 (bump and revalue), 1 (pathwise) or 2 "
+                                         "(likelihood ratio)");
+    for (size_t i = 0; i < n; ++i) {
+        int rc = greeks_single(c, reqs[i], estimator, nullptr, &base[i], out + i * MP_MAX_GREEKS,
+                               &n_estimates[i]);
+        if (rc != MP_OK) return rc;
+    }
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_greeks_from_normals(mp_ctx* c, const mp_request* req, int32_t estimator,
+                                         const double* normals, mp_result* base,
+                                         mp_greek_estimate* out, int32_t* n_estimates) {
+    if (!c || !req || !normals || !base || !out || !n_estimates)
+        return fail(MP_E_INVALID_CONFIG, "null argument");
+    if (estimator != MP_GREEKS_PATHWISE && estimator != MP_GREEKS_LIKELIHOOD_RATIO)
+        return fail(MP_E_INVALID_CONFIG, "caller normals drive the single-pass estimators only");
+    HIPCHK(hipSetDevice(c->device));
+    *n_estimates = 0;
+    for (int j = 0; j < MP_MAX_GREEKS; ++j) nan_estimate(out[j]);
+    return greeks_single(c, *req, estimator, normals, base, out, n_estimates);
 }
 
 MP_EXPORT int mp_hip_lsm_from_normals(mp_ctx* c, const mp_request* req, const double* normals,

@@ -10,6 +10,11 @@
 // Control-variate tiles use the first five slots of a partial row for the
 // discounted moments sum P, sum P^2, sum C, sum C^2, sum P*C (one strike).
 enum mp_technique { MP_TECH_STANDARD = 0, MP_TECH_ANTITHETIC = 1, MP_TECH_CONTROL_VARIATE = 2 };
+// Finalize mode for single-pass Greek estimators (never a request's
+// technique): the eight columns of a partial row are (sum, sum of squares)
+// of four per-path estimates, written out raw.
+#define MP_TECH_GREEK_MOMENTS 3
+enum mp_greek_est { MP_GREEK_EST_PATHWISE = 1, MP_GREEK_EST_LIKELIHOOD_RATIO = 2 };
 // Structured samplers drawn per (point, dimension) in the work-queue and
 // basket kernels (the ABI's sampling codes).
 enum mp_structured { MP_SAMPLER_NONE = 0, MP_SAMPLER_HALTON = 1, MP_SAMPLER_LHS = 2 };
@@ -57,6 +62,7 @@ struct mp_dreq {
     uint32_t sampler_off;     // first mp_sampler_dim of this request
     uint64_t sampler_n;       // points the sampler is built over (requested n_paths)
     double v0, kappa, theta, xi, rho, rho_perp, rate, dt, sqrt_dt;
+    double sigma;             // GBM volatility (single-pass Greeks)
     double mu_j, sigma_j, jump_mean_dt;
     double cum[MP_POISSON_TERMS];
 };
@@ -131,6 +137,10 @@ hipError_t launch_basket(const mp_dreq* req, const mp_sampler_dim* sampler, int 
                          const double* weight, const double* normals, double* partial,
                          double* terminal_out, double* payoff_out, int precision, int grid,
                          int block, hipStream_t s);
+// Single-pass terminal-GBM Greeks (estimator mp_greek_est): per-block
+// partial rows of four estimates' (sum, sum of squares).
+hipError_t launch_greeks(const mp_dreq* req, int estimator, const double* normals, double* partial,
+                         int precision, int grid, int block, hipStream_t s);
 hipError_t launch_lsm_fill(unsigned int* counters, int n_counters, int* valid, double* coeff,
                            int n_dates, hipStream_t s);
 hipError_t launch_lsm_paths(const mp_lsm_params* p, const double* normals, void* paths,

@@ -519,7 +519,8 @@ __global__ __launch_bounds__(64 * MP_NS_MAX) void finalize_kernel(const mp_dreq*
     const int w = threadIdx.x >> 6;
     const double n = (double)q.n_paths;
     const bool cv = q.technique == MP_TECH_CONTROL_VARIATE;
-    const int n_cols = cv ? 5 : 2 * q.n_strikes;
+    const bool moments = q.technique == MP_TECH_GREEK_MOMENTS;
+    const int n_cols = moments ? MP_PARTIAL_STRIDE : cv ? 5 : 2 * q.n_strikes;
     // Lane l adds tiles l, l + 64, ... in that order; sixteen tiles' column
     // pairs are loaded before any is added.
     mp_neumaier a, b;
@@ -563,7 +564,9 @@ __global__ __launch_bounds__(64 * MP_NS_MAX) void finalize_kernel(const mp_dreq*
     if (threadIdx.x == 0) {
         double* o = out + (uint64_t)blockIdx.x * MP_PARTIAL_STRIDE;
         for (int j = 0; j < MP_PARTIAL_STRIDE; ++j) o[j] = 0.0;
-        if (cv) {
+        if (moments) {
+            for (int j = 0; j < MP_PARTIAL_STRIDE; ++j) o[j] = tot[j];
+        } else if (cv) {
             control_variate_summary(tot, n, q.cv_expect, &o[0], &o[1]);
         } else {
             for (int j = 0; j < q.n_strikes; ++j) {
@@ -951,6 +954,66 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
             block_sum_store<2>(vv, partial + (uint64_t)tile * MP_PARTIAL_STRIDE);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass terminal-GBM Greeks (european_call_pathwise_greeks_cpu and
+// european_call_likelihood_ratio_greeks_cpu, cpu.rs:4670-4857).  Path p takes
+// the terminal draw of the engine's terminal-method price (Philox block 0,
+// stream 0, first normal) or normals[p].  EST 1 (pathwise): Delta, Vega, Rho,
+// Theta; EST 2 (likelihood ratio): Delta, Vega, Rho.  Each block writes the
+// FP64 (sum, sum of squares) of the four estimates to partial[block].
+// ---------------------------------------------------------------------------
+template <class R, int SRC, int EST>
+__global__ __launch_bounds__(256) void greeks_kernel(const mp_dreq* __restrict__ reqp,
+                                                    const double* __restrict__ normals,
+                                                    double* partial) {
+    const mp_dreq& q = *reqp;
+    const R s0 = R(q.s0), k = R(q.strikes[0]), r = R(q.rate), t = R(q.dt), sqrt_t = R(q.sqrt_dt);
+    const R sigma = R(q.sigma), drift_t = R(q.drift_dt), vol_t = R(q.vol_dt), disc = R(q.discount);
+    const R inv_s0 = R(1) / s0;
+    const R lr_delta = R(1) / (s0 * sigma * sqrt_t);
+    const R inv_sig_sqrt_t = R(1) / (sigma * sqrt_t);
+    const R inv_sigma = R(1) / sigma;
+    const R drift_rate = r - R(0.5) * sigma * sigma;
+    const R half_sig_rsqrt_t = sigma / (R(2) * sqrt_t);
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < q.n_paths; p += stride) {
+        R z;
+        if constexpr (SRC == 0) {
+            R zz[4];
+            mp_normals4<R>((uint32_t)p, 0u, 0u, q.seed_hi, q.key0, q.key1, zz);
+            z = zz[0];
+        } else {
+            z = R(normals[p]);
+        }
+        const R st = s0 * mp_exp(drift_t + vol_t * z);
+        const R intrinsic = st - k;
+        const bool active = intrinsic > R(0);
+        const R payoff = active ? intrinsic : R(0);
+        const R dpay = disc * payoff;
+        R g[4];
+        if constexpr (EST == MP_GREEK_EST_PATHWISE) {
+            g[0] = active ? disc * st * inv_s0 : R(0);
+            g[1] = active ? disc * st * (-sigma * t + sqrt_t * z) : R(0);
+            g[2] = active ? disc * (t * st - t * payoff) : -t * dpay;
+            const R dst_dt = st * (drift_rate + z * half_sig_rsqrt_t);
+            g[3] = -(active ? disc * (dst_dt - r * payoff) : -r * dpay);
+        } else {
+            g[0] = dpay * z * lr_delta;
+            g[1] = dpay * ((z * z - R(1)) * inv_sigma - sqrt_t * z);
+            g[2] = dpay * (z * inv_sig_sqrt_t - t);
+            g[3] = R(0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double gi = (double)g[i];
+            v[2 * i] += gi;
+            v[2 * i + 1] += gi * gi;
+        }
+    }
+    block_sum_store<8>(v, partial + (uint64_t)blockIdx.x * MP_PARTIAL_STRIDE);
 }
 
 // ---------------------------------------------------------------------------
@@ -1501,6 +1564,36 @@ hipError_t launch_basket(const mp_dreq* req, const mp_sampler_dim* sampler, int 
                                 partial, terminal_out, payoff_out, grid, block, s);
     return basket_r<float>(n_assets, req, chol, drift, vol, log_s0, weight, normals, sampler,
                            partial, terminal_out, payoff_out, grid, block, s);
+}
+
+template <class R>
+static hipError_t greeks_r(const mp_dreq* req, int est, const double* normals, double* partial,
+                           int grid, int block, hipStream_t s) {
+    if (est == MP_GREEK_EST_PATHWISE) {
+        if (normals)
+            hipLaunchKernelGGL((greeks_kernel<R, 1, MP_GREEK_EST_PATHWISE>), dim3(grid), dim3(block), 0, s,
+                               req, normals, partial);
+        else
+            hipLaunchKernelGGL((greeks_kernel<R, 0, MP_GREEK_EST_PATHWISE>), dim3(grid), dim3(block), 0, s,
+                               req, normals, partial);
+    } else {
+        if (normals)
+            hipLaunchKernelGGL((greeks_kernel<R, 1, MP_GREEK_EST_LIKELIHOOD_RATIO>), dim3(grid), dim3(block),
+                               0, s, req, normals, partial);
+        else
+            hipLaunchKernelGGL((greeks_kernel<R, 0, MP_GREEK_EST_LIKELIHOOD_RATIO>), dim3(grid), dim3(block),
+                               0, s, req, normals, partial);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_greeks(const mp_dreq* req, int estimator, const double* normals, double* partial,
+                         int precision, int grid, int block, hipStream_t s) {
+    if (block != 256 || grid < 1) return hipErrorInvalidValue;
+    if (estimator != MP_GREEK_EST_PATHWISE && estimator != MP_GREEK_EST_LIKELIHOOD_RATIO)
+        return hipErrorInvalidValue;
+    if (precision == 64) return greeks_r<double>(req, estimator, normals, partial, grid, block, s);
+    return greeks_r<float>(req, estimator, normals, partial, grid, block, s);
 }
 
 hipError_t launch_lsm_fill(unsigned int* counters, int n_counters, int* valid, double* coeff,

@@ -139,6 +139,29 @@ class PriceResult:
         )
 
 
+GREEK_ESTIMATORS = {
+    "bump_and_revalue": A.GREEKS_BUMP_AND_REVALUE,
+    "pathwise": A.GREEKS_PATHWISE,
+    "likelihood_ratio": A.GREEKS_LIKELIHOOD_RATIO,
+}
+_ESTIMATOR_NAMES = {v: k for k, v in GREEK_ESTIMATORS.items()}
+
+
+def _greek_dict(e: A.GreekEstimate) -> dict:
+    """GreekEstimate (cpu.rs:157-169) as a dict; None where the estimator has no value."""
+    opt = lambda x: None if x != x else x
+    return {
+        "greek": A.GREEK_NAMES[e.greek], "estimator": _ESTIMATOR_NAMES[e.estimator],
+        "value": e.value, "stderr": e.std_error, "bump_size": opt(e.bump_size),
+        "base_price": e.base_price, "bumped_up_price": opt(e.bumped_up_price),
+        "bumped_down_price": opt(e.bumped_down_price),
+    }
+
+
+def _estimator_code(estimator) -> int:
+    return GREEK_ESTIMATORS[estimator] if isinstance(estimator, str) else int(estimator)
+
+
 def _pd(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -233,6 +256,36 @@ class Engine:
             self._ctx, C.byref(req), _pd(normals), None if u is None else _pd(u),
             None if j is None else _pd(j), C.byref(res)))
         return PriceResult.of(res)
+
+    def greeks(self, reqs, estimator="bump_and_revalue"):
+        """Greeks of one request or a list: [(base PriceResult, [estimate dict])].
+        Bump-and-revalue legs of every request run as one batch."""
+        single = isinstance(reqs, A.Request)
+        reqs = [reqs] if single else list(reqs)
+        n = len(reqs)
+        arr = (A.Request * n)(*reqs)
+        base = (A.Result * n)()
+        est = (A.GreekEstimate * (n * A.MAX_GREEKS))()
+        counts = (C.c_int32 * n)()
+        A.check(self.lib.mp_hip_greeks(self._ctx, arr, n, _estimator_code(estimator), base, est,
+                                       counts))
+        out = [(PriceResult.of(base[i]),
+                [_greek_dict(est[i * A.MAX_GREEKS + j]) for j in range(counts[i])])
+               for i in range(n)]
+        return out[0] if single else out
+
+    def greeks_from_normals(self, req: A.Request, estimator, normals):
+        """Pathwise / likelihood-ratio Greeks of a European call from one
+        terminal normal per path; returns (base PriceResult, [estimate dict])."""
+        z = np.ascontiguousarray(normals, dtype=np.float64)
+        assert z.size == int(req.n_paths)
+        base = A.Result()
+        est = (A.GreekEstimate * A.MAX_GREEKS)()
+        count = C.c_int32()
+        A.check(self.lib.mp_hip_greeks_from_normals(self._ctx, C.byref(req),
+                                                    _estimator_code(estimator), _pd(z),
+                                                    C.byref(base), est, C.byref(count)))
+        return PriceResult.of(base), [_greek_dict(est[j]) for j in range(count.value)]
 
     def price_with_paths(self, req: A.Request):
         """Price and return (result, terminal values, undiscounted payoffs)."""

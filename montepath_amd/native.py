@@ -31,6 +31,8 @@ KNOWN_ROCM_FUNCTIONS = (
     "price_basket_call_rocm",
     "price_american_put_rocm",
     "price_bermudan_put_rocm",
+    "greeks_rocm",
+    "price_all_current_greeks_bump_and_revalue_rocm",
 )
 
 NATIVE_MODULE = "montepath_amd._lib.libmontepath_hip"
@@ -196,7 +198,7 @@ def _gbm_family(workload: str, function: str, cfg: Mapping, barrier: bool = Fals
 
 
 # ------------------------------------------------------------- functions
-def price_european_call_rocm(config: Mapping) -> dict:
+def _european_call_parts(config: Mapping):
     """European call.  The reference's bridge prices by the terminal
     distribution (Auto, runtime/cpu.rs:3020-3023); ``method`` may ask for
     ``stepwise_paths`` (the default here, the path engine's workload)."""
@@ -207,25 +209,25 @@ def price_european_call_rocm(config: Mapping) -> dict:
         raise ValueError(f"unsupported European call method '{m}'")
     c["method"] = method
     kw["method"] = "terminal" if method in ("auto", "terminal_distribution") else "stepwise"
-    return _run("european_call", "price_european_call_rocm", c, kw)
+    return c, kw
 
 
-def price_arithmetic_asian_call_rocm(config: Mapping) -> dict:
+def _arithmetic_asian_call_parts(config: Mapping):
     c, kw = _gbm_family("arithmetic_asian_call", "price_arithmetic_asian_call_rocm", config)
-    return _run("arithmetic_asian_call", "price_arithmetic_asian_call_rocm", c, kw)
+    return c, kw
 
 
-def price_down_and_out_call_rocm(config: Mapping) -> dict:
+def _down_and_out_call_parts(config: Mapping):
     c, kw = _gbm_family("down_and_out_call", "price_down_and_out_call_rocm", config, barrier=True)
-    return _run("down_and_out_call", "price_down_and_out_call_rocm", c, kw)
+    return c, kw
 
 
-def price_lookback_call_rocm(config: Mapping) -> dict:
+def _lookback_call_parts(config: Mapping):
     c, kw = _gbm_family("lookback_call", "price_lookback_call_rocm", config)
-    return _run("lookback_call", "price_lookback_call_rocm", c, kw)
+    return c, kw
 
 
-def price_heston_european_call_rocm(config: Mapping) -> dict:
+def _heston_european_call_parts(config: Mapping):
     c = {
         "s0": _f(config, ["s0", "spot"], 100.0), "k": _f(config, ["k", "strike"], 100.0),
         "r": _f(config, ["r", "rate"], 0.03), "v0": _f(config, ["v0", "initial_variance"], 0.04),
@@ -250,10 +252,10 @@ def price_heston_european_call_rocm(config: Mapping) -> dict:
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)
-    return _run("heston_european_call", "price_heston_european_call_rocm", c, kw)
+    return c, kw
 
 
-def price_merton_jump_diffusion_call_rocm(config: Mapping) -> dict:
+def _merton_jump_diffusion_call_parts(config: Mapping):
     c = {
         "s0": _f(config, ["s0", "spot"], 100.0), "k": _f(config, ["k", "strike"], 100.0),
         "r": _f(config, ["r", "rate"], 0.03), "sigma": _f(config, ["sigma", "volatility"], 0.2),
@@ -272,10 +274,10 @@ def price_merton_jump_diffusion_call_rocm(config: Mapping) -> dict:
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)
-    return _run("merton_jump_diffusion_call", "price_merton_jump_diffusion_call_rocm", c, kw)
+    return c, kw
 
 
-def price_basket_call_rocm(config: Mapping) -> dict:
+def _basket_call_parts(config: Mapping):
     """Two-asset basket with the reference's keys (s01, s02, sigma1, sigma2,
     rho, weight1, weight2), or n assets via asset_s0 / asset_sigma /
     asset_weight / correlation lists.  n_steps (default 1) walks the assets
@@ -315,13 +317,13 @@ def price_basket_call_rocm(config: Mapping) -> dict:
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)
-    return _run("basket_call", "price_basket_call_rocm", c, kw)
+    return c, kw
 
 
 _LSM_DETAILS = ["early_exercise_count", "maturity_exercise_count", "regression_steps"]
 
 
-def price_american_put_rocm(config: Mapping) -> dict:
+def _american_put_parts(config: Mapping):
     c = _gbm(config, steps_default=64)
     c["basis_degree"] = _u(config, ["basis_degree"], 2)
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], s0=c["s0"], k=c["k"],
@@ -329,10 +331,10 @@ def price_american_put_rocm(config: Mapping) -> dict:
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)
-    return _run("american_put", "price_american_put_rocm", c, kw, extra=_LSM_DETAILS)
+    return c, kw
 
 
-def price_bermudan_put_rocm(config: Mapping) -> dict:
+def _bermudan_put_parts(config: Mapping):
     c = _gbm(config, steps_default=64)
     c["basis_degree"] = _u(config, ["basis_degree"], 2)
     sched = config.get("exercise_steps")
@@ -343,4 +345,109 @@ def price_bermudan_put_rocm(config: Mapping) -> dict:
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)
-    return _run("bermudan_put", "price_bermudan_put_rocm", c, kw, extra=_LSM_DETAILS)
+    return c, kw
+
+
+# ------------------------------------------------- the price_*_rocm surface
+def _priced(workload: str, parts, extra=None):
+    fn = f"price_{workload}_rocm"
+
+    def price(config: Mapping) -> dict:
+        c, kw = parts(config)
+        return _run(workload, fn, c, kw, extra=extra)
+
+    price.__name__ = fn
+    price.__qualname__ = fn
+    price.__doc__ = parts.__doc__
+    return price
+
+
+price_european_call_rocm = _priced("european_call", _european_call_parts)
+price_arithmetic_asian_call_rocm = _priced("arithmetic_asian_call", _arithmetic_asian_call_parts)
+price_down_and_out_call_rocm = _priced("down_and_out_call", _down_and_out_call_parts)
+price_lookback_call_rocm = _priced("lookback_call", _lookback_call_parts)
+price_heston_european_call_rocm = _priced("heston_european_call", _heston_european_call_parts)
+price_merton_jump_diffusion_call_rocm = _priced("merton_jump_diffusion_call",
+                                                _merton_jump_diffusion_call_parts)
+price_basket_call_rocm = _priced("basket_call", _basket_call_parts)
+price_american_put_rocm = _priced("american_put", _american_put_parts, extra=_LSM_DETAILS)
+price_bermudan_put_rocm = _priced("bermudan_put", _bermudan_put_parts, extra=_LSM_DETAILS)
+
+
+# ------------------------------------------------------------------- Greeks
+_GREEK_PARTS = {
+    "european_call": _european_call_parts,
+    "arithmetic_asian_call": _arithmetic_asian_call_parts,
+    "down_and_out_call": _down_and_out_call_parts,
+    "lookback_call": _lookback_call_parts,
+    "basket_call": _basket_call_parts,
+    "heston_european_call": _heston_european_call_parts,
+}
+_SINGLE_PASS_NOTE = (
+    "single-pass Greeks use the terminal draws of the request's pseudorandom stream; the "
+    "configured technique and sampling apply to the base price only")
+
+
+def _greek_request(workload: str, config: Mapping):
+    parts = _GREEK_PARTS.get(workload)
+    if parts is None:
+        raise ValueError(f"Greeks are offered for {', '.join(_GREEK_PARTS)}; not '{workload}'")
+    cfg = dict(config)
+    if workload == "european_call" and cfg.get("method") is None:
+        cfg["method"] = "auto"  # european_call_price_mc_cpu's default (terminal distribution)
+    c, kw = parts(cfg)
+    return c, make_request(workload, **kw)
+
+
+def _greek_report(workload: str, estimator: str, c: dict, base, estimates) -> dict:
+    """GreekReport (runtime/cpu.rs:171-183) with backend rocm_hip."""
+    warnings = []
+    if estimator != "bump_and_revalue":
+        warnings.append(_SINGLE_PASS_NOTE)
+    return {
+        "workload": workload, "estimator": estimator, "backend": "rocm_hip",
+        "paths": c["n_paths"], "steps": c["n_steps"], "seed": c["seed"],
+        "base_price": base.price, "base_stderr": base.stderr, "estimated_runtime_ms": None,
+        "estimates": estimates, "warnings": warnings,
+    }
+
+
+def greeks_rocm(workload: str, config: Mapping, estimator: str = "bump_and_revalue") -> dict:
+    """Monte Carlo Greeks of one workload on the GPU (european_call_greeks_cpu,
+    arithmetic_asian_call_greeks_cpu, down_and_out_call_greeks_cpu,
+    lookback_call_greeks_cpu, basket_call_greeks_cpu,
+    heston_european_call_greeks_cpu; runtime/cpu.rs:4036-4356).  Bump and
+    revalue prices every leg with common random numbers in one batch;
+    pathwise and likelihood ratio are European-call single-pass estimators.
+    The European default method is the reference's (terminal distribution)."""
+    est = _normalize(str(estimator))
+    if est not in ("bump_and_revalue", "pathwise", "likelihood_ratio"):
+        raise ValueError(f"unsupported Greek estimator '{estimator}'")
+    c, req = _greek_request(workload, config)
+    try:
+        base, estimates = _engine().greeks(req, est)
+    except A.MontepathHipError as err:
+        raise ValueError(f"native ROCm execution failed: {err}") from err
+    return _greek_report(workload, est, c, base, estimates)
+
+
+def price_all_current_greeks_bump_and_revalue_rocm(n_paths: int, n_steps: int,
+                                                   seed: int) -> list:
+    """price_all_current_greeks_bump_and_revalue_cpu (runtime/cpu.rs:4358-4407):
+    default configs of six workloads with seeds seed..seed+5, every bumped leg
+    of all six in one GPU batch."""
+    cfgs = [
+        ("european_call", {"n_paths": n_paths, "n_steps": n_steps, "seed": seed}),
+        ("arithmetic_asian_call", {"n_paths": n_paths, "n_steps": n_steps, "seed": seed + 1}),
+        ("down_and_out_call", {"n_paths": n_paths, "n_steps": n_steps, "seed": seed + 2}),
+        ("lookback_call", {"n_paths": n_paths, "n_steps": n_steps, "seed": seed + 3}),
+        ("basket_call", {"n_paths": n_paths, "seed": seed + 4}),
+        ("heston_european_call", {"n_paths": n_paths, "n_steps": n_steps, "seed": seed + 5}),
+    ]
+    built = [(w,) + _greek_request(w, cfg) for w, cfg in cfgs]
+    try:
+        out = _engine().greeks([req for _, _, req in built], "bump_and_revalue")
+    except A.MontepathHipError as err:
+        raise ValueError(f"native ROCm execution failed: {err}") from err
+    return [_greek_report(w, "bump_and_revalue", c, base, est)
+            for (w, c, _), (base, est) in zip(built, out)]

@@ -29,6 +29,19 @@ class FakeEngine:
                            strike_prices=[1.25] * ns, strike_stderrs=[0.01] * ns,
                            n_paths=req.n_paths, n_steps=req.n_steps, regression_steps=7)
 
+    def greeks(self, reqs, estimator="bump_and_revalue"):
+        single = isinstance(reqs, A.Request)
+        reqs = [reqs] if single else list(reqs)
+        self.requests.extend(reqs)
+        self.greek_calls = getattr(self, "greek_calls", []) + [(len(reqs), estimator)]
+        out = []
+        for req in reqs:
+            base = self.price(req)
+            names = ["delta", "vega", "rho", "theta"]
+            out.append((base, [{"greek": g, "estimator": estimator, "value": 0.5, "stderr": 0.01}
+                               for g in names]))
+        return out[0] if single else out
+
 
 @pytest.fixture
 def fake():
@@ -141,3 +154,37 @@ def test_engine_errors_become_value_errors():
 def test_known_function_list_is_complete():
     for name in native.KNOWN_ROCM_FUNCTIONS:
         assert callable(getattr(native, name))
+
+
+def test_greeks_report_shape_and_european_default_method(fake):
+    rep = native.greeks_rocm("european_call", {"n_paths": 5000, "seed": 9})
+    r = fake.requests[-1]
+    # the reference's Greek functions price European calls by the terminal distribution
+    assert r.method == A.METHOD_TERMINAL and r.seed == 9 and r.n_paths == 5000
+    assert rep["workload"] == "european_call" and rep["estimator"] == "bump_and_revalue"
+    assert rep["backend"] == "rocm_hip" and rep["base_price"] == 1.25
+    assert [e["greek"] for e in rep["estimates"]] == ["delta", "vega", "rho", "theta"]
+    native.greeks_rocm("european_call", {"method": "stepwise_paths"}, "pathwise")
+    assert fake.requests[-1].method == A.METHOD_STEPWISE
+    assert fake.greek_calls[-1] == (1, "pathwise")
+
+
+def test_all_workload_greeks_run_as_one_batch(fake):
+    reps = native.price_all_current_greeks_bump_and_revalue_rocm(25_000, 64, 9403)
+    assert [r["workload"] for r in reps] == ["european_call", "arithmetic_asian_call",
+                                             "down_and_out_call", "lookback_call", "basket_call",
+                                             "heston_european_call"]
+    assert fake.greek_calls == [(6, "bump_and_revalue")]
+    seeds = [r.seed for r in fake.requests[:6]]
+    assert seeds == [9403 + i for i in range(6)]
+    heston = fake.requests[5]
+    assert (heston.kappa, heston.xi, heston.rho) == (1.5, 0.3, -0.6)
+
+
+@pytest.mark.parametrize("args,msg", [
+    (("merton_jump_diffusion_call", {}), "Greeks are offered for"),
+    (("european_call", {}, "adjoint"), "unsupported Greek estimator"),
+])
+def test_bad_greek_requests_raise_value_error(fake, args, msg):
+    with pytest.raises(ValueError, match=msg):
+        native.greeks_rocm(*args)
