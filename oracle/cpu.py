@@ -58,6 +58,12 @@ class MertonCfg(C.Structure):
     ]
 
 
+class MlmcCfg(C.Structure):
+    _fields_ = [("s0", C.c_double), ("k", C.c_double), ("r", C.c_double), ("sigma", C.c_double),
+                ("t", C.c_double), ("refinement", C.c_uint64), ("seed", C.c_uint64),
+                ("sampling", C.c_int64)]
+
+
 class BsGreeks(C.Structure):
     _fields_ = [("price", C.c_double), ("delta", C.c_double), ("vega", C.c_double),
                 ("rho", C.c_double), ("theta", C.c_double)]
@@ -178,6 +184,9 @@ def lib() -> C.CDLL:
         "mpo_heston_qmc_paths": (None, [C.POINTER(HestonCfg), C.c_int, C.c_uint64, _D]),
         "mpo_bs_greeks": (BsGreeks, [C.c_double] * 5),
         "mpo_terminal_greeks": (None, [C.POINTER(GbmCfg), C.c_int, _D, _D]),
+        "mpo_mlmc_level": (None, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, _D]),
+        "mpo_mlmc_level_from_normals": (None, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, _D,
+                                               _D]),
         "mpo_philox4x32_10": (None, [_U32, _U32, _U32]),
         "mpo_philox_normals": (
             None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, _D]),
@@ -836,3 +845,169 @@ def bump_greeks(workload: str, cfg, scheme: int = HESTON_REFERENCE_FLOOR) -> lis
         _bump_estimate("theta", base[0], cfg.t, _bump_maturity(cfg.t),
                        lambda v: price(replace(cfg, t=v)), reverse=True),
     ]
+
+
+# ---------------------------------------------------------------- MLMC Asian
+MLMC_DEFAULTS = {"s0": 100.0, "k": 100.0, "r": 0.03, "sigma": 0.2, "t": 1.0, "base_steps": 16,
+                 "levels": 4, "refinement_factor": 2,
+                 "paths_per_level": [50_000, 25_000, 12_500, 6_250], "seed": 42,
+                 "sampling": "pseudorandom", "scramble_replicates": 1}
+
+
+def mlmc_config(**kw) -> dict:
+    """ArithmeticAsianMlmcConfig with the reference defaults (cpu.rs:1536-1568)."""
+    c = dict(MLMC_DEFAULTS)
+    c.update(kw)
+    return c
+
+
+def mlmc_fine_steps(cfg: dict, level: int) -> int:
+    return cfg["base_steps"] * cfg["refinement_factor"] ** level
+
+
+def _mlmc_c(cfg: dict, seed: int) -> MlmcCfg:
+    return MlmcCfg(cfg["s0"], cfg["k"], cfg["r"], cfg["sigma"], cfg["t"], cfg["refinement_factor"],
+                   seed, SAMPLING_CODES[cfg["sampling"]])
+
+
+def mlmc_level_moments(cfg: dict, level: int, paths: int, normals=None) -> tuple[float, float]:
+    """simulate_asian_mlmc_level (cpu.rs:6639-6773): (sum, sum of squares)."""
+    out = np.zeros(2)
+    c = _mlmc_c(cfg, cfg["seed"])
+    fine = mlmc_fine_steps(cfg, level)
+    if normals is None:
+        lib().mpo_mlmc_level(C.byref(c), level, paths, fine, _pd(out))
+    else:
+        z = np.ascontiguousarray(normals, dtype=np.float64)
+        assert z.size == paths * fine
+        lib().mpo_mlmc_level_from_normals(C.byref(c), level, paths, fine, _pd(z), _pd(out))
+    return out[0], out[1]
+
+
+def _mlmc_single(cfg: dict) -> dict:
+    """arithmetic_asian_call_price_mlmc_single_cpu (cpu.rs:3612-3661)."""
+    price = 0.0
+    var_contrib = 0.0
+    total_paths = 0
+    total_updates = 0
+    levels = []
+    for level in range(cfg["levels"]):
+        paths = cfg["paths_per_level"][level]
+        fine = mlmc_fine_steps(cfg, level)
+        coarse = None if level == 0 else fine // cfg["refinement_factor"]
+        s, sq = mlmc_level_moments(cfg, level, paths)
+        n = float(paths)
+        mean = s / n
+        variance = max((sq / n) - mean * mean, 0.0)
+        stderr = math.sqrt(variance / n)
+        cost = paths * (fine + (coarse or 0))
+        price += mean
+        var_contrib += variance / n
+        total_paths += paths
+        total_updates += cost
+        levels.append({"level": level, "paths": paths, "fine_steps": fine, "coarse_steps": coarse,
+                       "mean": mean, "variance": variance, "stderr": stderr,
+                       "cost_step_updates": cost})
+    return {"price": price, "stderr": math.sqrt(var_contrib), "total_paths": total_paths,
+            "total_step_updates": total_updates, "scramble_replicates": 1,
+            "replicate_estimates": [price], "levels": levels}
+
+
+def mlmc_price(cfg: dict) -> dict:
+    """arithmetic_asian_call_price_mlmc_cpu (cpu.rs:3283-3293), with the
+    replicated structured estimator (cpu.rs:3663-3751)."""
+    if cfg["sampling"] == "pseudorandom" or cfg["scramble_replicates"] <= 1:
+        return _mlmc_single(cfg)
+    reps = []
+    for r in range(cfg["scramble_replicates"]):
+        rc = dict(cfg)
+        rc["seed"] = lib().mpo_derive_chunk_seed(cfg["seed"], r)
+        rc["scramble_replicates"] = 1
+        reps.append(_mlmc_single(rc))
+    R = len(reps)
+    est = [x["price"] for x in reps]
+    price = sum(est) / R
+    stderr = math.sqrt(sum((e - price) * (e - price) for e in est) / (R - 1) / R)
+    levels = []
+    for li in range(cfg["levels"]):
+        first = reps[0]["levels"][li]
+        mean = sum(x["levels"][li]["mean"] for x in reps) / R
+        variance = sum((x["levels"][li]["mean"] - mean) ** 2 for x in reps) / (R - 1)
+        levels.append({**first, "paths": first["paths"] * R, "mean": mean, "variance": variance,
+                       "stderr": math.sqrt(variance / R),
+                       "cost_step_updates": first["cost_step_updates"] * R})
+    return {"price": price, "stderr": stderr, "total_paths": sum(x["total_paths"] for x in reps),
+            "total_step_updates": sum(x["total_step_updates"] for x in reps),
+            "scramble_replicates": R, "replicate_estimates": est, "levels": levels}
+
+
+def _pilot_levels(cfg: dict, pilot_paths: int) -> list:
+    """measure_arithmetic_asian_mlmc_pilot_levels (cpu.rs:3567-3598)."""
+    out = []
+    for level in range(cfg["levels"]):
+        fine = mlmc_fine_steps(cfg, level)
+        coarse = None if level == 0 else fine // cfg["refinement_factor"]
+        s, sq = mlmc_level_moments(cfg, level, pilot_paths)
+        n = float(pilot_paths)
+        mean = s / n
+        out.append({"level": level, "fine_steps": fine, "coarse_steps": coarse,
+                    "pilot_paths": pilot_paths,
+                    "pilot_variance": max((sq / n) - mean * mean, 1.0e-18),
+                    "cost_per_path": fine + (coarse or 0), "recommended_paths": 1})
+    return out
+
+
+def _allocate(levels: list, numer: float, by_variance: bool) -> dict:
+    denom = sum(math.sqrt(l["pilot_variance"] * l["cost_per_path"]) for l in levels)
+    updates = 0
+    for l in levels:
+        w = math.sqrt(l["pilot_variance"] / l["cost_per_path"])
+        raw = math.ceil(denom * w / numer) if by_variance else math.ceil(numer * w / denom)
+        l["recommended_paths"] = max(int(raw), 2)
+        updates += l["recommended_paths"] * l["cost_per_path"]
+    return {"target_step_updates": updates if by_variance else int(numer),
+            "estimated_step_updates": updates,
+            "paths_per_level": [l["recommended_paths"] for l in levels], "levels": levels}
+
+
+def mlmc_tune_allocation(cfg: dict, target_step_updates: int, pilot_paths: int) -> dict:
+    """tune_arithmetic_asian_mlmc_allocation_cpu (cpu.rs:3295-3350)."""
+    return _allocate(_pilot_levels(cfg, pilot_paths), float(target_step_updates), False)
+
+
+def _estimate_stderr(plan: dict, reps: int) -> float:
+    var = sum(l["pilot_variance"] / l["recommended_paths"] for l in plan["levels"])
+    return math.sqrt(var / reps)
+
+
+def mlmc_solve_tolerance(cfg: dict, target_stderr: float, pilot_paths: int = 2_048,
+                         min_step_updates: int = 100_000,
+                         max_step_updates: int = 10_000_000) -> dict:
+    """solve_arithmetic_asian_mlmc_tolerance_cpu (cpu.rs:3352-3423)."""
+    reps = 1 if cfg["sampling"] == "pseudorandom" else max(cfg["scramble_replicates"], 1)
+    pilot = dict(cfg)
+    pilot["scramble_replicates"] = 1
+    plan = _allocate(_pilot_levels(pilot, pilot_paths), target_stderr * target_stderr * reps, True)
+    est = _estimate_stderr(plan, reps)
+    updates = plan["estimated_step_updates"] * reps
+    if updates < min_step_updates:
+        per = max(-(-min_step_updates // reps), 1)
+        plan = mlmc_tune_allocation(pilot, per, pilot_paths)
+        est = _estimate_stderr(plan, reps)
+        updates = plan["estimated_step_updates"] * reps
+    hit = False
+    if updates > max_step_updates:
+        hit = True
+        per = max(max_step_updates // reps, 1)
+        plan = mlmc_tune_allocation(pilot, per, pilot_paths)
+        est = _estimate_stderr(plan, reps)
+        updates = plan["estimated_step_updates"] * reps
+    rec = dict(cfg)
+    rec["paths_per_level"] = list(plan["paths_per_level"])
+    if rec["sampling"] == "pseudorandom":
+        rec["scramble_replicates"] = 1
+    return {"target_stderr": target_stderr, "estimated_stderr": est,
+            "target_met": est <= target_stderr and updates <= max_step_updates,
+            "max_step_updates_hit": hit, "scramble_replicates": reps,
+            "estimated_step_updates": updates, "paths_per_level": list(plan["paths_per_level"]),
+            "allocation": plan, "recommended_config": rec}

@@ -2256,3 +2256,111 @@ MPO_EXPORT void mpo_terminal_greeks(const mpo_gbm_cfg* cfg, int estimator, const
             out[2 * i] = out[2 * i + 1] = 0.0;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Multilevel Monte Carlo arithmetic Asian: simulate_asian_mlmc_level  */
+/* cpu.rs:6639-6773 with the level-0 payoff :6775-6795 and the coupled */
+/* fine/coarse difference :6812-6849 (fine increments summed in groups */
+/* of refinement_factor, divided by sqrt(refinement)).  Each level     */
+/* draws from MonteCarloRng(derive_chunk_seed(seed, level)) or from a  */
+/* structured sampler built with that seed over (paths, fine_steps).   */
+/* out[0] = sum, out[1] = sum of squares of the discounted level term. */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double s0, k, r, sigma, t;
+    uint64_t refinement, seed;
+    int64_t sampling;
+} mpo_mlmc_cfg;
+
+static double mlmc_term(const mpo_mlmc_cfg* c, uint64_t fine_steps, const double* z, double fine_drift,
+                        double fine_vol, double coarse_drift, double coarse_vol, double sqrt_ref,
+                        double discount, int level0) {
+    if (level0) {
+        double log_s = log(c->s0), sum = 0.0;
+        for (uint64_t i = 0; i < fine_steps; ++i) {
+            log_s += fine_drift + fine_vol * z[i];
+            sum += exp(log_s);
+        }
+        return fmax_ref(sum / (double)fine_steps - c->k, 0.0) * discount;
+    }
+    uint64_t coarse_steps = fine_steps / c->refinement;
+    double fine_log = log(c->s0), coarse_log = log(c->s0), fine_sum = 0.0, coarse_sum = 0.0;
+    uint64_t idx = 0;
+    for (uint64_t cs = 0; cs < coarse_steps; ++cs) {
+        double zsum = 0.0;
+        for (uint64_t j = 0; j < c->refinement; ++j) {
+            double zz = z[idx++];
+            zsum += zz;
+            fine_log += fine_drift + fine_vol * zz;
+            fine_sum += exp(fine_log);
+        }
+        double coarse_z = zsum / sqrt_ref;
+        coarse_log += coarse_drift + coarse_vol * coarse_z;
+        coarse_sum += exp(coarse_log);
+    }
+    double fine_pay = fmax_ref(fine_sum / (double)(coarse_steps * c->refinement) - c->k, 0.0) * discount;
+    double coarse_pay = fmax_ref(coarse_sum / (double)coarse_steps - c->k, 0.0) * discount;
+    return fine_pay - coarse_pay;
+}
+
+MPO_EXPORT void mpo_mlmc_level(const mpo_mlmc_cfg* c, uint64_t level, uint64_t paths,
+                               uint64_t fine_steps, double* out) {
+    const uint64_t lseed = mpo_derive_chunk_seed(c->seed, level);
+    const double discount = exp(-c->r * c->t);
+    const double fine_dt = c->t / (double)fine_steps;
+    const double fine_drift = (c->r - 0.5 * c->sigma * c->sigma) * fine_dt;
+    const double fine_vol = c->sigma * sqrt(fine_dt);
+    double coarse_drift = 0.0, coarse_vol = 0.0;
+    if (level > 0) {
+        const double coarse_dt = c->t / (double)(fine_steps / c->refinement);
+        coarse_drift = (c->r - 0.5 * c->sigma * c->sigma) * coarse_dt;
+        coarse_vol = c->sigma * sqrt(coarse_dt);
+    }
+    const double sqrt_ref = sqrt((double)c->refinement);
+    double* z = (double*)malloc(sizeof(double) * (fine_steps ? fine_steps : 1));
+    mpo_rng g;
+    rng_init(&g, lseed);
+    mpo_sampler smp;
+    const int structured = c->sampling != MPO_PSEUDORANDOM;
+    if (structured)
+        sampler_init(&smp, c->sampling == MPO_SOBOL_BB ? MPO_SOBOL : (int)c->sampling, lseed, paths,
+                     fine_steps);
+    double sum = 0.0, sq = 0.0;
+    for (uint64_t p = 0; p < paths; ++p) {
+        for (uint64_t i = 0; i < fine_steps; ++i)
+            z[i] = structured ? sampler_normal(&smp, p, i) : rng_normal(&g);
+        double v = mlmc_term(c, fine_steps, z, fine_drift, fine_vol, coarse_drift, coarse_vol, sqrt_ref,
+                             discount, level == 0);
+        sum += v;
+        sq += v * v;
+    }
+    if (structured) sampler_free(&smp);
+    free(z);
+    out[0] = sum;
+    out[1] = sq;
+}
+
+/* The same level term from caller normals [path][fine_steps] (parity). */
+MPO_EXPORT void mpo_mlmc_level_from_normals(const mpo_mlmc_cfg* c, uint64_t level, uint64_t paths,
+                                            uint64_t fine_steps, const double* normals, double* out) {
+    const double discount = exp(-c->r * c->t);
+    const double fine_dt = c->t / (double)fine_steps;
+    const double fine_drift = (c->r - 0.5 * c->sigma * c->sigma) * fine_dt;
+    const double fine_vol = c->sigma * sqrt(fine_dt);
+    double coarse_drift = 0.0, coarse_vol = 0.0;
+    if (level > 0) {
+        const double coarse_dt = c->t / (double)(fine_steps / c->refinement);
+        coarse_drift = (c->r - 0.5 * c->sigma * c->sigma) * coarse_dt;
+        coarse_vol = c->sigma * sqrt(coarse_dt);
+    }
+    const double sqrt_ref = sqrt((double)c->refinement);
+    double sum = 0.0, sq = 0.0;
+    for (uint64_t p = 0; p < paths; ++p) {
+        double v = mlmc_term(c, fine_steps, normals + p * fine_steps, fine_drift, fine_vol, coarse_drift,
+                             coarse_vol, sqrt_ref, discount, level == 0);
+        sum += v;
+        sq += v * v;
+    }
+    out[0] = sum;
+    out[1] = sq;
+}

@@ -113,6 +113,30 @@ GREEK_ROWS = {
                                 "n_paths": 25_000}),
 }
 
+# multilevel Asian (harness.rs:2613-2665, :2892-2980): the MLMC config is
+# solved by solve_arithmetic_asian_mlmc_tolerance_cpu from a pilot at the row's
+# seed (base 8 steps, 4 levels, refinement 2, 2048 pilot paths, target stderr
+# 0.05, 1e5..2e6 step updates), then priced (seeds seed..seed+repeats-1); the
+# quality and calibration rows compare with the standard Asian MC at
+# 100_000 x 64 (n_threads 8) on the same seed
+MLMC_ROWS = {
+    "mc_cpu_arithmetic_asian_call_rust_mlmc": (
+        "mlmc_price", {"sampling": "pseudorandom", "replicates": 1, "seed": 42, "repeats": 3}),
+    "mc_cpu_arithmetic_asian_call_rust_mlmc_quality": (
+        "mlmc_stderr_ratio", {"sampling": "pseudorandom", "replicates": 1, "seed": 7112}),
+    "mc_cpu_arithmetic_asian_call_rust_mlmc_reference_calibration": (
+        "mlmc_abs_error", {"sampling": "pseudorandom", "replicates": 1, "seed": 7114}),
+    # scrambled-Sobol multilevel rows: sobol_burley bits, statistical only
+    "mc_cpu_arithmetic_asian_call_rust_mlqmc": (
+        "mlmc_price", {"sampling": "scrambled_sobol", "replicates": 4, "seed": 42, "repeats": 3}),
+    "mc_cpu_arithmetic_asian_call_rust_mlqmc_quality": (
+        "mlmc_stderr_ratio", {"sampling": "scrambled_sobol", "replicates": 4, "seed": 7113}),
+    "mc_cpu_arithmetic_asian_call_rust_mlqmc_reference_calibration": (
+        "mlmc_abs_error", {"sampling": "scrambled_sobol", "replicates": 4, "seed": 7115}),
+}
+MLMC_PLAN = {"base_steps": 8, "levels": 4, "refinement_factor": 2, "pilot_paths": 2_048,
+             "target_stderr": 0.05, "min_step_updates": 100_000, "max_step_updates": 2_000_000}
+
 COMMON = {"s0": 100.0, "k": 100.0, "r": 0.03, "sigma": 0.2, "t": 1.0, "n_paths": 100_000,
           "n_steps": 64, "seed": 42, "n_threads": 8, "repeats": 3}
 
@@ -155,7 +179,19 @@ def main() -> None:
             full.update(cfg)
             greeks[name] = {"pricer": kind, "config": full, "metric_name": r["metric_name"],
                             "published_value": r["metric_value"]}
+    mlmc = {}
+    for r in release["results"]:
+        name = r["benchmark_name"]
+        if name in MLMC_ROWS:
+            kind, cfg = MLMC_ROWS[name]
+            full = dict(MLMC_PLAN)
+            full.update({"standard_paths": 100_000, "standard_steps": 64, "n_threads": 8,
+                         "repeats": 1})
+            full.update(cfg)
+            mlmc[name] = {"pricer": kind, "config": full, "metric_name": r["metric_name"],
+                          "published_value": r["metric_value"]}
     out = {
+        "mlmc_rows": mlmc,
         "greek_rows": greeks,
         "source": "montepath benchmarks/reference-fixtures.json and benchmarks/release-results.json",
         "quality_rows": quality,
@@ -166,7 +202,7 @@ def main() -> None:
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(OUT, len(fx), "fixtures", len(rows), "rows", len(quality), "quality rows", len(greeks),
-          "Greek rows")
+          "Greek rows", len(mlmc), "MLMC rows")
 
 
 if __name__ == "__main__":

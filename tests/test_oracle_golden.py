@@ -187,6 +187,88 @@ def test_greek_estimators_against_black_scholes(oracle):
     assert abs(fd.mean() - pw["delta"][0]) < 1e-4
 
 
+MLMC_ROWS = GOLD["mlmc_rows"]
+
+
+def _adaptive_mlmc(oracle, c, seed):
+    # adaptive_arithmetic_asian_mlmc_config (harness.rs:2635-2665)
+    cfg = oracle.mlmc_config(base_steps=c["base_steps"], levels=c["levels"],
+                             refinement_factor=c["refinement_factor"],
+                             paths_per_level=[c["pilot_paths"]] * c["levels"], seed=seed,
+                             sampling=c["sampling"], scramble_replicates=c["replicates"])
+    plan = oracle.mlmc_solve_tolerance(cfg, c["target_stderr"], c["pilot_paths"],
+                                       c["min_step_updates"], c["max_step_updates"])
+    cfg["paths_per_level"] = plan["paths_per_level"]
+    return cfg
+
+
+@pytest.mark.parametrize("name", sorted(MLMC_ROWS))
+def test_published_mlmc_rows(oracle, name):
+    # Pseudorandom rows: the adaptive plan, the coupled levels and the price
+    # reproduce bit for bit; the stderr-ratio and calibration rows also divide
+    # by / subtract the standard Asian MC, whose published value carries the
+    # host libm's last-ulp exp differences (the same 1e-12 bound as the
+    # technique ratios).  Scrambled-Sobol rows use sobol_burley's bits, which
+    # this oracle does not reproduce: their values are checked statistically.
+    row = MLMC_ROWS[name]
+    c = row["config"]
+    want = row["published_value"]
+    cfg = _adaptive_mlmc(oracle, c, c["seed"])
+    sobol = c["sampling"] != "pseudorandom"
+    if row["pricer"] == "mlmc_price":
+        prices = []
+        for i in range(c["repeats"]):
+            ci = dict(cfg)
+            ci["seed"] = c["seed"] + i
+            prices.append(oracle.mlmc_price(ci)["price"])
+        got = sum(prices) / len(prices)
+        if sobol:
+            assert abs(got - want) < 0.05, (got, want)
+        else:
+            assert got == want, (got, want)
+        return
+    std = oracle.gbm_price(oracle.Gbm(n_paths=c["standard_paths"], n_steps=c["standard_steps"],
+                                      seed=c["seed"], n_threads=c["n_threads"]),
+                           "arithmetic_asian_call")
+    res = oracle.mlmc_price(cfg)
+    if row["pricer"] == "mlmc_stderr_ratio":
+        got = res["stderr"] / std[1]
+        if sobol:
+            # a standard error from four replicate estimates has three degrees
+            # of freedom (relative spread ~40%): only its scale is checked
+            assert 0.2 * want < got < 3.0 * want, (got, want)
+        else:
+            assert abs(got - want) <= 1e-12 * want, (got, want)
+    else:
+        got = abs(res["price"] - std[0])
+        if sobol:
+            assert got < 0.1, (got, want)
+        else:
+            assert abs(got - want) <= 1e-13 * std[0], (got, want)
+
+
+def test_mlmc_telescoping_and_coupling(oracle):
+    # the level terms telescope: level-0 mean + sum of coupled differences on
+    # one normal buffer equals the finest level's plain payoff mean; coupled
+    # differences shrink with level (strong convergence of the coupling); and
+    # refinement 1 would make every difference zero (coarse = fine)
+    n = 4000
+    cfg = oracle.mlmc_config(base_steps=4, levels=3, refinement_factor=2, paths_per_level=[n] * 3)
+    z = oracle.rng_normals(11, n * 16).reshape(n, 16)
+    # coarse normals of level l from the level-(l+1) fine normals
+    z8 = (z[:, 0::2] + z[:, 1::2]) / np.sqrt(2.0)
+    z4 = (z8[:, 0::2] + z8[:, 1::2]) / np.sqrt(2.0)
+    m0 = oracle.mlmc_level_moments(cfg, 0, n, z4)[0] / n
+    d1 = oracle.mlmc_level_moments(cfg, 1, n, z8)[0] / n
+    d2 = oracle.mlmc_level_moments(cfg, 2, n, z)[0] / n
+    fine = oracle.gbm_paths_from_normals(
+        oracle.Gbm(n_paths=n, n_steps=16), z, "arithmetic_asian_call")[1].mean()
+    assert abs((m0 + d1 + d2) - fine) <= 1e-12 * fine
+    v1 = oracle.mlmc_level_moments(cfg, 1, n, z8)
+    v2 = oracle.mlmc_level_moments(cfg, 2, n, z)
+    assert v2[1] < v1[1]
+
+
 def test_control_variate_summary_special_cases(oracle):
     # a control that is an exact linear function of the payoff removes all
     # variance; a constant control falls back to the plain summary
