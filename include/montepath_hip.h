@@ -182,6 +182,75 @@ typedef struct mp_greek_estimate {
     double bumped_down_price;
 } mp_greek_estimate;
 
+/* Multilevel Monte Carlo arithmetic Asian call (ArithmeticAsianMlmcConfig,
+ * runtime/cpu.rs:1536-1568; defaults s0 = k = 100, r = 0.03, sigma = 0.2,
+ * t = 1, base_steps 16, 4 levels, refinement 2).  Level l walks
+ * base_steps * refinement^l fine steps; levels >= 1 price the coupled
+ * difference fine - coarse, the coarse path driven by sums of `refinement`
+ * fine normals / sqrt(refinement).  Level l draws from the seed
+ * derive_chunk_seed(seed, l) (replicate r of a structured run first takes
+ * derive_chunk_seed(seed, r)): Philox keyed by it, or a structured sampler
+ * built from it over (paths, fine steps). */
+#define MP_MLMC_MAX_LEVELS 16
+#define MP_MLMC_MAX_REPLICATES 64
+typedef struct mp_mlmc_config {
+    double s0, k, r, sigma, t;
+    uint32_t base_steps;
+    uint32_t levels;                /* 1..MP_MLMC_MAX_LEVELS */
+    uint32_t refinement_factor;     /* >= 2 */
+    uint32_t scramble_replicates;   /* structured sampling only; 1..MP_MLMC_MAX_REPLICATES */
+    int32_t sampling;               /* mp_sampling (both Sobol codes draw plain Sobol points) */
+    int32_t precision;              /* 32 or 64 */
+    uint64_t seed;
+    uint64_t contract_id;
+    uint64_t paths_per_level[MP_MLMC_MAX_LEVELS];
+} mp_mlmc_config;
+
+/* ArithmeticAsianMlmcLevelResult (cpu.rs:1570-1580); coarse_steps 0 = none. */
+typedef struct mp_mlmc_level {
+    uint64_t level, paths, fine_steps, coarse_steps;
+    double mean, variance, std_error;
+    uint64_t cost_step_updates;
+} mp_mlmc_level;
+
+/* ArithmeticAsianMlmcResult (cpu.rs:1582-1591). */
+typedef struct mp_mlmc_result {
+    double price, std_error;
+    uint64_t total_paths, total_step_updates;
+    uint32_t scramble_replicates, n_levels;
+    double runtime_ms;
+    mp_mlmc_level levels[MP_MLMC_MAX_LEVELS];
+    double replicate_estimates[MP_MLMC_MAX_REPLICATES];
+} mp_mlmc_result;
+
+/* ArithmeticAsianMlmcAllocationLevel / Plan (cpu.rs:1593-1610). */
+typedef struct mp_mlmc_allocation_level {
+    uint64_t level, fine_steps, coarse_steps, pilot_paths;
+    double pilot_variance;
+    uint64_t cost_per_path, recommended_paths;
+} mp_mlmc_allocation_level;
+
+typedef struct mp_mlmc_allocation {
+    uint64_t target_step_updates, estimated_step_updates;
+    uint32_t n_levels, reserved;
+    mp_mlmc_allocation_level levels[MP_MLMC_MAX_LEVELS];
+} mp_mlmc_allocation;
+
+/* ArithmeticAsianMlmcToleranceConfig / TolerancePlan (cpu.rs:1612-1642). */
+typedef struct mp_mlmc_tolerance {
+    double target_stderr;
+    uint64_t pilot_paths_per_level, min_step_updates, max_step_updates;
+} mp_mlmc_tolerance;
+
+typedef struct mp_mlmc_tolerance_plan {
+    double target_stderr, estimated_stderr;
+    int32_t target_met, max_step_updates_hit;
+    uint32_t scramble_replicates, reserved;
+    uint64_t estimated_step_updates;
+    mp_mlmc_allocation allocation;
+    mp_mlmc_config recommended;     /* the input with paths_per_level from the plan */
+} mp_mlmc_tolerance_plan;
+
 /* Device description (reference DeviceInfo, backend/mod.rs:32-43). */
 typedef struct mp_device_info {
     char device_id[32];      /* "rocm:<ordinal>" */
@@ -298,6 +367,28 @@ int mp_hip_greeks(mp_ctx* ctx, const mp_request* reqs, size_t n, int32_t estimat
 int mp_hip_greeks_from_normals(mp_ctx* ctx, const mp_request* req, int32_t estimator,
                                const double* normals, mp_result* base, mp_greek_estimate* out,
                                int32_t* n_estimates);
+
+/* Multilevel Asian price (arithmetic_asian_call_price_mlmc_cpu,
+ * runtime/cpu.rs:3283-3293, single estimator :3612-3661, replicated
+ * structured estimator :3663-3751).  Every (replicate, level) runs in one
+ * persistent work-queue launch; level terms are summed in FP64. */
+int mp_hip_mlmc_price(mp_ctx* ctx, const mp_mlmc_config* cfg, mp_mlmc_result* out);
+
+/* tune_arithmetic_asian_mlmc_allocation_cpu (cpu.rs:3295-3350): pilot every
+ * level with pilot_paths paths (one launch), then split target_step_updates
+ * in proportion to sqrt(variance / cost), at least two paths per level. */
+int mp_hip_mlmc_tune_allocation(mp_ctx* ctx, const mp_mlmc_config* cfg, uint64_t target_step_updates,
+                                uint64_t pilot_paths, mp_mlmc_allocation* out);
+
+/* solve_arithmetic_asian_mlmc_tolerance_cpu (cpu.rs:3352-3423): paths per
+ * level for a target standard error, clamped to [min, max] step updates. */
+int mp_hip_mlmc_solve_tolerance(mp_ctx* ctx, const mp_mlmc_config* cfg,
+                                const mp_mlmc_tolerance* tolerance, mp_mlmc_tolerance_plan* out);
+
+/* Parity hook: (sum, sum of squares) of one level's discounted terms from
+ * caller normals, step-major (fine_steps x paths). */
+int mp_hip_mlmc_level_from_normals(mp_ctx* ctx, const mp_mlmc_config* cfg, uint32_t level,
+                                   uint64_t paths, const double* normals, double* sum_sq);
 
 /* Price one request and also return, per path, the terminal value (basket
  * value for a basket) and the undiscounted payoff for strike 0, from the

@@ -97,6 +97,69 @@ class GreekEstimate(C.Structure):
     ]
 
 
+MLMC_MAX_LEVELS = 16
+MLMC_MAX_REPLICATES = 64
+
+
+class MlmcConfig(C.Structure):
+    _fields_ = [
+        ("s0", C.c_double), ("k", C.c_double), ("r", C.c_double), ("sigma", C.c_double),
+        ("t", C.c_double), ("base_steps", C.c_uint32), ("levels", C.c_uint32),
+        ("refinement_factor", C.c_uint32), ("scramble_replicates", C.c_uint32),
+        ("sampling", C.c_int32), ("precision", C.c_int32), ("seed", C.c_uint64),
+        ("contract_id", C.c_uint64), ("paths_per_level", C.c_uint64 * MLMC_MAX_LEVELS),
+    ]
+
+
+class MlmcLevel(C.Structure):
+    _fields_ = [
+        ("level", C.c_uint64), ("paths", C.c_uint64), ("fine_steps", C.c_uint64),
+        ("coarse_steps", C.c_uint64), ("mean", C.c_double), ("variance", C.c_double),
+        ("std_error", C.c_double), ("cost_step_updates", C.c_uint64),
+    ]
+
+
+class MlmcResult(C.Structure):
+    _fields_ = [
+        ("price", C.c_double), ("std_error", C.c_double), ("total_paths", C.c_uint64),
+        ("total_step_updates", C.c_uint64), ("scramble_replicates", C.c_uint32),
+        ("n_levels", C.c_uint32), ("runtime_ms", C.c_double),
+        ("levels", MlmcLevel * MLMC_MAX_LEVELS),
+        ("replicate_estimates", C.c_double * MLMC_MAX_REPLICATES),
+    ]
+
+
+class MlmcAllocationLevel(C.Structure):
+    _fields_ = [
+        ("level", C.c_uint64), ("fine_steps", C.c_uint64), ("coarse_steps", C.c_uint64),
+        ("pilot_paths", C.c_uint64), ("pilot_variance", C.c_double),
+        ("cost_per_path", C.c_uint64), ("recommended_paths", C.c_uint64),
+    ]
+
+
+class MlmcAllocation(C.Structure):
+    _fields_ = [
+        ("target_step_updates", C.c_uint64), ("estimated_step_updates", C.c_uint64),
+        ("n_levels", C.c_uint32), ("reserved", C.c_uint32),
+        ("levels", MlmcAllocationLevel * MLMC_MAX_LEVELS),
+    ]
+
+
+class MlmcTolerance(C.Structure):
+    _fields_ = [("target_stderr", C.c_double), ("pilot_paths_per_level", C.c_uint64),
+                ("min_step_updates", C.c_uint64), ("max_step_updates", C.c_uint64)]
+
+
+class MlmcTolerancePlan(C.Structure):
+    _fields_ = [
+        ("target_stderr", C.c_double), ("estimated_stderr", C.c_double),
+        ("target_met", C.c_int32), ("max_step_updates_hit", C.c_int32),
+        ("scramble_replicates", C.c_uint32), ("reserved", C.c_uint32),
+        ("estimated_step_updates", C.c_uint64), ("allocation", MlmcAllocation),
+        ("recommended", MlmcConfig),
+    ]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [
         ("device_id", C.c_char * 32), ("name", C.c_char * 96), ("arch", C.c_char * 32),
@@ -159,6 +222,13 @@ SIGNATURES = {
     "mp_hip_greeks_from_normals": (C.c_int, [_P, C.POINTER(Request), C.c_int32, _D,
                                              C.POINTER(Result), C.POINTER(GreekEstimate),
                                              C.POINTER(C.c_int32)]),
+    "mp_hip_mlmc_price": (C.c_int, [_P, C.POINTER(MlmcConfig), C.POINTER(MlmcResult)]),
+    "mp_hip_mlmc_tune_allocation": (C.c_int, [_P, C.POINTER(MlmcConfig), C.c_uint64, C.c_uint64,
+                                              C.POINTER(MlmcAllocation)]),
+    "mp_hip_mlmc_solve_tolerance": (C.c_int, [_P, C.POINTER(MlmcConfig), C.POINTER(MlmcTolerance),
+                                              C.POINTER(MlmcTolerancePlan)]),
+    "mp_hip_mlmc_level_from_normals": (C.c_int, [_P, C.POINTER(MlmcConfig), C.c_uint32, C.c_uint64,
+                                                 _D, _D]),
     "mp_hip_price_with_paths": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Result), _D, _D]),
     "mp_hip_lsm_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, C.POINTER(Result), _D,
                                           _U32]),

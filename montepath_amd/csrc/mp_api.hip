@@ -640,6 +640,7 @@ struct mp_batch {
     std::vector<size_t> req_index;  // position in the caller's array
     uint32_t n_tiles = 0;
     int grid = 1, block = 256, ppl = 1;
+    bool mlmc = false;
     dbuf d_reqs, d_tile_req, d_order, d_partial, d_out, d_counter, d_sampler;
     std::vector<double> h_out;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
@@ -702,6 +703,12 @@ int batch_build(mp_batch* b) {
     mp_tuning tu = tuning_for(b->ctx, dreqs.empty() ? mp_dreq{} : dreqs[0], b->precision);
     b->block = tu.block_size > 0 ? tu.block_size : 256;
     b->ppl = tu.paths_per_lane > 0 ? tu.paths_per_lane : 1;
+    size_t n_mlmc = 0;
+    for (const mp_dreq& d : dreqs) n_mlmc += d.payoff == MP_PAY_ASIAN_MLMC;
+    if (n_mlmc != 0 && n_mlmc != dreqs.size())
+        return fail(MP_E_INVALID_CONFIG, "multilevel levels run in batches of their own");
+    b->mlmc = n_mlmc != 0;
+    if (b->mlmc) b->ppl = 1;  // the multilevel kernel runs one path per lane
     b->grid = grid_for_tiles(b->ctx, tiles, b->block);
     b->h_out.assign(n * MP_PARTIAL_STRIDE, 0.0);
     return MP_OK;
@@ -721,6 +728,8 @@ int batch_enqueue(mp_batch* b) {
     L.counter = b->d_counter.as<unsigned int>();
     L.partial = b->d_partial.as<double>();
     L.sampler = b->d_sampler.as<mp_sampler_dim>();
+    L.nib = b->ctx->d_nib;
+    L.mlmc = b->mlmc ? 1 : 0;
     L.grid = b->grid;
     L.block = b->block;
     L.ppl = b->ppl;
@@ -745,6 +754,8 @@ int batch_capture(mp_batch* b) {
     L.counter = b->d_counter.as<unsigned int>();
     L.partial = b->d_partial.as<double>();
     L.sampler = b->d_sampler.as<mp_sampler_dim>();
+    L.nib = b->ctx->d_nib;
+    L.mlmc = b->mlmc ? 1 : 0;
     L.grid = b->grid;
     L.block = b->block;
     L.ppl = b->ppl;
@@ -1134,10 +1145,12 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     L.uniforms = buffer && d.model == MP_MODEL_MERTON ? c->uniforms.as<double>() : nullptr;
     L.jump_normals = buffer && d.model == MP_MODEL_MERTON ? c->jumps.as<double>() : nullptr;
     L.sampler = c->sampler.as<mp_sampler_dim>();
+    L.nib = c->d_nib;
+    L.mlmc = d.payoff == MP_PAY_ASIAN_MLMC ? 1 : 0;
     L.terminal_out = want_paths ? c->term.as<double>() : nullptr;
     L.payoff_out = want_paths ? c->pay.as<double>() : nullptr;
     L.block = tu.block_size > 0 ? tu.block_size : 256;
-    L.ppl = tu.paths_per_lane > 0 ? tu.paths_per_lane : 1;
+    L.ppl = tu.paths_per_lane > 0 && d.payoff != MP_PAY_ASIAN_MLMC ? tu.paths_per_lane : 1;
     L.grid = grid_for_tiles(c, d.n_tiles, L.block);
     HIPCHK(mp::launch_reset(L.counter, 1, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -1673,7 +1686,7 @@ int greeks_single(mp_ctx* c, const mp_request& r, int estimator, const double* n
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)c->n_cu * 8));
     d.tile0 = 0;
     d.n_tiles = (uint32_t)grid;
-    d.technique = MP_TECH_GREEK_MOMENTS;
+    d.technique = MP_TECH_RAW_MOMENTS;
     ENSURE(c->reqs, sizeof(mp_dreq));
     ENSURE(c->partial, (size_t)grid * MP_PARTIAL_STRIDE * 8);
     ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
@@ -1742,6 +1755,392 @@ MP_EXPORT int mp_hip_greeks_from_normals(mp_ctx* c, const mp_request* req, int32
     *n_estimates = 0;
     for (int j = 0; j < MP_MAX_GREEKS; ++j) nan_estimate(out[j]);
     return greeks_single(c, *req, estimator, normals, base, out, n_estimates);
+}
+
+// ===========================================================================
+// Multilevel Monte Carlo arithmetic Asian
+// ===========================================================================
+namespace {
+
+uint64_t derive_chunk_seed(uint64_t base, uint64_t idx) {
+    return splitmix64(base + idx * 0x9E3779B97F4A7C15ull);  // cpu.rs:8922-8924
+}
+
+uint64_t mlmc_fine(const mp_mlmc_config& c, uint32_t level) {
+    uint64_t s = c.base_steps;
+    for (uint32_t i = 0; i < level; ++i) s *= c.refinement_factor;
+    return s;
+}
+
+bool mlmc_structured(const mp_mlmc_config& c) { return c.sampling != MP_PSEUDORANDOM; }
+
+// validate_arithmetic_asian_mlmc_config (cpu.rs:3493-3510) plus this
+// engine's limits (levels, replicates, steps per request).
+int mlmc_validate(const mp_mlmc_config& c, bool need_paths) {
+    if (c.base_steps == 0) return fail(MP_E_INVALID_CONFIG, "base_steps must be > 0");
+    if (c.levels == 0) return fail(MP_E_INVALID_CONFIG, "levels must be > 0");
+    if (c.levels > MP_MLMC_MAX_LEVELS) return fail(MP_E_UNSUPPORTED, "at most 16 levels");
+    if (c.refinement_factor < 2) return fail(MP_E_INVALID_CONFIG, "refinement_factor must be >= 2");
+    if (c.scramble_replicates == 0) return fail(MP_E_INVALID_CONFIG, "scramble_replicates must be > 0");
+    if (c.scramble_replicates > MP_MLMC_MAX_REPLICATES)
+        return fail(MP_E_UNSUPPORTED, "at most 64 scramble replicates");
+    if (c.sampling != MP_PSEUDORANDOM && c.sampling != MP_RANDOMIZED_HALTON &&
+        c.sampling != MP_LATIN_HYPERCUBE && c.sampling != MP_SCRAMBLED_SOBOL &&
+        c.sampling != MP_SCRAMBLED_SOBOL_BRIDGE)
+        return fail(MP_E_INVALID_CONFIG, "unknown sampling method " + std::to_string(c.sampling));
+    if (c.precision != 0 && c.precision != 32 && c.precision != 64)
+        return fail(MP_E_INVALID_CONFIG, "precision must be 32 or 64");
+    if (!(c.s0 > 0.0) || !(c.t > 0.0) || !(c.sigma >= 0.0) || !finite(c.k) || !finite(c.r))
+        return fail(MP_E_INVALID_CONFIG, "s0 and t must be > 0, sigma >= 0, k and r finite");
+    double steps = (double)c.base_steps;
+    for (uint32_t l = 1; l < c.levels; ++l) steps *= (double)c.refinement_factor;
+    if (steps > 2147483647.0) return fail(MP_E_UNSUPPORTED, "finest level exceeds 2^31 - 1 steps");
+    if (mlmc_structured(c) && c.sampling != MP_SCRAMBLED_SOBOL && c.sampling != MP_SCRAMBLED_SOBOL_BRIDGE &&
+        steps > (double)MP_SAMPLER_MAX_DIMS)
+        return fail(MP_E_UNSUPPORTED, "Halton and Latin hypercube levels take at most 65536 steps");
+    if (need_paths)
+        for (uint32_t l = 0; l < c.levels; ++l) {
+            if (c.paths_per_level[l] == 0) return fail(MP_E_INVALID_CONFIG, "paths_per_level values must be > 0");
+            if (c.paths_per_level[l] > 0xFFFFFFFFull)
+                return fail(MP_E_UNSUPPORTED, "n_paths above 2^32 per level is not supported");
+        }
+    return MP_OK;
+}
+
+struct mlmc_unit {
+    uint64_t seed;    // the estimator's seed (a replicate's derived seed)
+    uint32_t level;
+    uint64_t paths;
+};
+
+// One level of one estimator as a work-queue request whose tiles sum the
+// discounted level terms (finalize writes the raw sum and sum of squares).
+int mlmc_prepare(const mp_mlmc_config& c, const mlmc_unit& u, prepared& P) {
+    std::memset(&P.d, 0, sizeof(P.d));
+    P.schedule.clear();
+    P.sampler.clear();
+    P.kind = KIND_TILE;
+    P.precision = c.precision == 64 ? 64 : 32;
+    mp_dreq& d = P.d;
+    const uint64_t fine = mlmc_fine(c, u.level);
+    const uint64_t lseed = derive_chunk_seed(u.seed, u.level);
+    d.model = MP_MODEL_GBM;
+    d.payoff = MP_PAY_ASIAN_MLMC;
+    d.n_steps = (int32_t)fine;
+    d.n_strikes = 1;
+    d.strikes[0] = c.k;
+    d.key0 = (uint32_t)lseed;
+    d.key1 = (uint32_t)c.contract_id;
+    d.seed_hi = (uint32_t)(lseed >> 32);
+    d.n_paths = u.paths;
+    d.s0 = c.s0;
+    d.log_s0 = std::log(c.s0);
+    d.rate = c.r;
+    d.discount = std::exp(-c.r * c.t);
+    d.log_barrier_rel = -1e300;
+    d.dt = c.t / (double)fine;
+    d.sqrt_dt = std::sqrt(d.dt);
+    d.drift_dt = (c.r - 0.5 * c.sigma * c.sigma) * d.dt;
+    d.vol_dt = c.sigma * std::sqrt(d.dt);
+    d.sigma = c.sigma;
+    d.technique = MP_TECH_RAW_MOMENTS;
+    if (u.level > 0) {
+        const double coarse_dt = c.t / (double)(fine / c.refinement_factor);
+        d.mlmc_refine = (int32_t)c.refinement_factor;
+        d.mlmc_coarse_drift = (c.r - 0.5 * c.sigma * c.sigma) * coarse_dt;
+        d.mlmc_coarse_vol = c.sigma * std::sqrt(coarse_dt);
+    }
+    d.mlmc_sqrt_refine = std::sqrt((double)c.refinement_factor);
+    if (c.sampling == MP_RANDOMIZED_HALTON || c.sampling == MP_LATIN_HYPERCUBE) {
+        d.sampling = c.sampling == MP_RANDOMIZED_HALTON ? MP_SAMPLER_HALTON : MP_SAMPLER_LHS;
+        d.sampler_n = u.paths;
+        build_sampler(d.sampling, lseed, u.paths, fine, P.sampler);
+    } else if (c.sampling != MP_PSEUDORANDOM) {
+        d.sampling = MP_SAMPLER_SOBOL;  // ScrambledSobol::new(level seed): key0 is its low word
+    }
+    P.n_paths_req = u.paths;
+    P.n_steps_cfg = fine;
+    d.tile_paths = tile_paths_for(d.n_paths, fine);
+    d.n_tiles = (uint32_t)((d.n_paths + d.tile_paths - 1) / d.tile_paths);
+    return MP_OK;
+}
+
+// Raw (sum, sum of squares) of every unit, all units in one launch.
+int mlmc_moments(mp_ctx* c, const mp_mlmc_config& cfg, const std::vector<mlmc_unit>& units,
+                 std::vector<double>& sums, std::vector<double>& sqs, double* ms) {
+    const int prec = cfg.precision == 64 ? 64 : 32;
+    mp_batch*& bp = c->bat[prec == 64 ? 1 : 0];
+    if (!bp) {
+        bp = new mp_batch();
+        int rc = batch_init(c, bp, c->stream, prec);
+        if (rc != MP_OK) return rc;
+    }
+    mp_batch* b = bp;
+    b->reqs.clear();
+    b->req_index.clear();
+    for (size_t i = 0; i < units.size(); ++i) {
+        prepared P;
+        int rc = mlmc_prepare(cfg, units[i], P);
+        if (rc != MP_OK) return rc;
+        b->reqs.push_back(std::move(P));
+        b->req_index.push_back(i);
+    }
+    int rc = batch_build(b);
+    if (rc != MP_OK) return rc;
+    rc = batch_enqueue(b);
+    if (rc != MP_OK) return rc;
+    std::vector<mp_result> res(units.size());
+    rc = batch_collect(b, res.data(), true);
+    if (rc != MP_OK) return rc;
+    sums.resize(units.size());
+    sqs.resize(units.size());
+    for (size_t i = 0; i < units.size(); ++i) {
+        sums[i] = res[i].price;      // raw-moment finalize: column 0 = sum
+        sqs[i] = res[i].std_error;   // column 1 = sum of squares
+    }
+    if (ms) *ms = units.empty() ? 0.0 : res[0].runtime_ms;
+    return MP_OK;
+}
+
+std::vector<mlmc_unit> mlmc_units(const mp_mlmc_config& c, uint64_t seed, const uint64_t* paths,
+                                  uint64_t pilot) {
+    std::vector<mlmc_unit> u;
+    for (uint32_t l = 0; l < c.levels; ++l) u.push_back(mlmc_unit{seed, l, paths ? paths[l] : pilot});
+    return u;
+}
+
+// measure_arithmetic_asian_mlmc_pilot_levels (cpu.rs:3567-3598)
+int mlmc_pilot(mp_ctx* c, const mp_mlmc_config& cfg, uint64_t pilot, mp_mlmc_allocation& a) {
+    std::vector<double> s, q;
+    int rc = mlmc_moments(c, cfg, mlmc_units(cfg, cfg.seed, nullptr, pilot), s, q, nullptr);
+    if (rc != MP_OK) return rc;
+    std::memset(&a, 0, sizeof(a));
+    a.n_levels = cfg.levels;
+    const double n = (double)pilot;
+    for (uint32_t l = 0; l < cfg.levels; ++l) {
+        mp_mlmc_allocation_level& L = a.levels[l];
+        L.level = l;
+        L.fine_steps = mlmc_fine(cfg, l);
+        L.coarse_steps = l == 0 ? 0 : L.fine_steps / cfg.refinement_factor;
+        L.pilot_paths = pilot;
+        const double mean = s[l] / n;
+        L.pilot_variance = std::max((q[l] / n) - mean * mean, 1.0e-18);
+        L.cost_per_path = L.fine_steps + L.coarse_steps;
+        L.recommended_paths = 1;
+    }
+    return MP_OK;
+}
+
+uint64_t sat_add(uint64_t a, uint64_t b) { return a > UINT64_MAX - b ? UINT64_MAX : a + b; }
+uint64_t sat_mul(uint64_t a, uint64_t b) { return (b != 0 && a > UINT64_MAX / b) ? UINT64_MAX : a * b; }
+uint64_t f64_to_u64_sat(double x) {  // Rust's `as usize` (saturating, NaN -> 0)
+    if (!(x > 0.0)) return 0;
+    if (x >= 18446744073709551616.0) return UINT64_MAX;
+    return (uint64_t)x;
+}
+
+// Recommended paths: by a step budget (tune) or a target variance (solve).
+void mlmc_allocate(mp_mlmc_allocation& a, double numer, bool by_variance) {
+    double denom = 0.0;
+    for (uint32_t l = 0; l < a.n_levels; ++l)
+        denom += std::sqrt(a.levels[l].pilot_variance * (double)a.levels[l].cost_per_path);
+    uint64_t updates = 0;
+    for (uint32_t l = 0; l < a.n_levels; ++l) {
+        mp_mlmc_allocation_level& L = a.levels[l];
+        const double w = std::sqrt(L.pilot_variance / (double)L.cost_per_path);
+        const double raw = by_variance ? std::ceil(denom * w / numer) : std::ceil(numer * w / denom);
+        L.recommended_paths = std::max<uint64_t>(f64_to_u64_sat(raw), 2);
+        updates = sat_add(updates, sat_mul(L.recommended_paths, L.cost_per_path));
+    }
+    a.estimated_step_updates = updates;
+    a.target_step_updates = by_variance ? updates : (uint64_t)numer;
+}
+
+double mlmc_plan_stderr(const mp_mlmc_allocation& a, uint32_t reps) {
+    double v = 0.0;
+    for (uint32_t l = 0; l < a.n_levels; ++l) v += a.levels[l].pilot_variance / (double)a.levels[l].recommended_paths;
+    return std::sqrt(v / (double)reps);
+}
+
+void mlmc_fill_level(const mp_mlmc_config& cfg, uint32_t l, uint64_t paths, double sum, double sq,
+                     mp_mlmc_level& L) {
+    L.level = l;
+    L.paths = paths;
+    L.fine_steps = mlmc_fine(cfg, l);
+    L.coarse_steps = l == 0 ? 0 : L.fine_steps / cfg.refinement_factor;
+    const double n = (double)paths;
+    L.mean = sum / n;
+    L.variance = std::max((sq / n) - L.mean * L.mean, 0.0);
+    L.std_error = std::sqrt(L.variance / n);
+    L.cost_step_updates = sat_mul(paths, L.fine_steps + L.coarse_steps);
+}
+
+}  // namespace
+
+MP_EXPORT int mp_hip_mlmc_price(mp_ctx* c, const mp_mlmc_config* cfgp, mp_mlmc_result* out) {
+    if (!c || !cfgp || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    const mp_mlmc_config& cfg = *cfgp;
+    int rc = mlmc_validate(cfg, true);
+    if (rc != MP_OK) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    std::memset(out, 0, sizeof(*out));
+    const uint32_t R = (mlmc_structured(cfg) && cfg.scramble_replicates > 1) ? cfg.scramble_replicates : 1;
+    std::vector<mlmc_unit> units;
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint64_t seed = R == 1 ? cfg.seed : derive_chunk_seed(cfg.seed, r);
+        auto u = mlmc_units(cfg, seed, cfg.paths_per_level, 0);
+        units.insert(units.end(), u.begin(), u.end());
+    }
+    std::vector<double> s, q;
+    double ms = 0.0;
+    rc = mlmc_moments(c, cfg, units, s, q, &ms);
+    if (rc != MP_OK) return rc;
+    const uint32_t nl = cfg.levels;
+    out->n_levels = nl;
+    out->scramble_replicates = R;
+    out->runtime_ms = ms;
+    // one estimator per replicate (arithmetic_asian_call_price_mlmc_single_cpu)
+    std::vector<mp_mlmc_level> lv((size_t)R * nl);
+    std::vector<double> est(R), se(R);
+    for (uint32_t r = 0; r < R; ++r) {
+        double price = 0.0, var_contrib = 0.0;
+        for (uint32_t l = 0; l < nl; ++l) {
+            mp_mlmc_level& L = lv[(size_t)r * nl + l];
+            mlmc_fill_level(cfg, l, cfg.paths_per_level[l], s[(size_t)r * nl + l], q[(size_t)r * nl + l], L);
+            price += L.mean;
+            var_contrib += L.variance / (double)L.paths;
+            out->total_paths = sat_add(out->total_paths, L.paths);
+            out->total_step_updates = sat_add(out->total_step_updates, L.cost_step_updates);
+        }
+        est[r] = price;
+        se[r] = std::sqrt(var_contrib);
+        out->replicate_estimates[r] = price;
+    }
+    if (R == 1) {
+        out->price = est[0];
+        out->std_error = se[0];
+        for (uint32_t l = 0; l < nl; ++l) out->levels[l] = lv[l];
+        return MP_OK;
+    }
+    // replicated estimator (cpu.rs:3663-3751): mean and sample spread of the
+    // replicate estimates, per level the spread of the level means
+    double sum = 0.0;
+    for (uint32_t r = 0; r < R; ++r) sum += est[r];
+    out->price = sum / (double)R;
+    double ss = 0.0;
+    for (uint32_t r = 0; r < R; ++r) ss += (est[r] - out->price) * (est[r] - out->price);
+    out->std_error = std::sqrt(ss / (double)(R - 1) / (double)R);
+    for (uint32_t l = 0; l < nl; ++l) {
+        mp_mlmc_level L = lv[l];
+        double m = 0.0;
+        for (uint32_t r = 0; r < R; ++r) m += lv[(size_t)r * nl + l].mean;
+        m /= (double)R;
+        double v = 0.0;
+        for (uint32_t r = 0; r < R; ++r) {
+            const double dd = lv[(size_t)r * nl + l].mean - m;
+            v += dd * dd;
+        }
+        v /= (double)(R - 1);
+        L.paths = sat_mul(L.paths, R);
+        L.mean = m;
+        L.variance = v;
+        L.std_error = std::sqrt(v / (double)R);
+        L.cost_step_updates = sat_mul(L.cost_step_updates, R);
+        out->levels[l] = L;
+    }
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_mlmc_tune_allocation(mp_ctx* c, const mp_mlmc_config* cfgp, uint64_t target_step_updates,
+                                          uint64_t pilot_paths, mp_mlmc_allocation* out) {
+    if (!c || !cfgp || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    int rc = mlmc_validate(*cfgp, false);
+    if (rc != MP_OK) return rc;
+    if (target_step_updates == 0) return fail(MP_E_INVALID_CONFIG, "target_step_updates must be > 0");
+    if (pilot_paths <= 1) return fail(MP_E_INVALID_CONFIG, "pilot_paths_per_level must be > 1");
+    HIPCHK(hipSetDevice(c->device));
+    rc = mlmc_pilot(c, *cfgp, pilot_paths, *out);
+    if (rc != MP_OK) return rc;
+    mlmc_allocate(*out, (double)target_step_updates, false);
+    out->target_step_updates = target_step_updates;
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_mlmc_solve_tolerance(mp_ctx* c, const mp_mlmc_config* cfgp,
+                                          const mp_mlmc_tolerance* tol, mp_mlmc_tolerance_plan* out) {
+    if (!c || !cfgp || !tol || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    int rc = mlmc_validate(*cfgp, false);
+    if (rc != MP_OK) return rc;
+    // validate_arithmetic_asian_mlmc_tolerance_config (cpu.rs:3512-3529)
+    if (!(std::isfinite(tol->target_stderr) && tol->target_stderr > 0.0))
+        return fail(MP_E_INVALID_CONFIG, "target_stderr must be finite and > 0");
+    if (tol->pilot_paths_per_level <= 1) return fail(MP_E_INVALID_CONFIG, "pilot_paths_per_level must be > 1");
+    if (tol->min_step_updates == 0) return fail(MP_E_INVALID_CONFIG, "min_step_updates must be > 0");
+    if (tol->max_step_updates < tol->min_step_updates)
+        return fail(MP_E_INVALID_CONFIG, "max_step_updates must be >= min_step_updates");
+    HIPCHK(hipSetDevice(c->device));
+    const mp_mlmc_config& cfg = *cfgp;
+    const uint32_t reps = mlmc_structured(cfg) ? std::max<uint32_t>(cfg.scramble_replicates, 1) : 1;
+    mp_mlmc_config pilot = cfg;
+    pilot.scramble_replicates = 1;
+    std::memset(out, 0, sizeof(*out));
+    mp_mlmc_allocation a;
+    rc = mlmc_pilot(c, pilot, tol->pilot_paths_per_level, a);
+    if (rc != MP_OK) return rc;
+    mp_mlmc_allocation pilot_levels = a;  // the tune passes re-measure with the same seed
+    mlmc_allocate(a, tol->target_stderr * tol->target_stderr * (double)reps, true);
+    double est = mlmc_plan_stderr(a, reps);
+    uint64_t updates = sat_mul(a.estimated_step_updates, reps);
+    if (updates < tol->min_step_updates) {
+        const uint64_t per = std::max<uint64_t>((tol->min_step_updates + reps - 1) / reps, 1);
+        a = pilot_levels;
+        mlmc_allocate(a, (double)per, false);
+        a.target_step_updates = per;
+        est = mlmc_plan_stderr(a, reps);
+        updates = sat_mul(a.estimated_step_updates, reps);
+    }
+    int hit = 0;
+    if (updates > tol->max_step_updates) {
+        hit = 1;
+        const uint64_t per = std::max<uint64_t>(tol->max_step_updates / reps, 1);
+        a = pilot_levels;
+        mlmc_allocate(a, (double)per, false);
+        a.target_step_updates = per;
+        est = mlmc_plan_stderr(a, reps);
+        updates = sat_mul(a.estimated_step_updates, reps);
+    }
+    out->target_stderr = tol->target_stderr;
+    out->estimated_stderr = est;
+    out->target_met = est <= tol->target_stderr && updates <= tol->max_step_updates;
+    out->max_step_updates_hit = hit;
+    out->scramble_replicates = reps;
+    out->estimated_step_updates = updates;
+    out->allocation = a;
+    out->recommended = cfg;
+    for (uint32_t l = 0; l < cfg.levels; ++l) out->recommended.paths_per_level[l] = a.levels[l].recommended_paths;
+    if (cfg.sampling == MP_PSEUDORANDOM) out->recommended.scramble_replicates = 1;
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_mlmc_level_from_normals(mp_ctx* c, const mp_mlmc_config* cfgp, uint32_t level,
+                                             uint64_t paths, const double* normals, double* sum_sq) {
+    if (!c || !cfgp || !normals || !sum_sq) return fail(MP_E_INVALID_CONFIG, "null argument");
+    int rc = mlmc_validate(*cfgp, false);
+    if (rc != MP_OK) return rc;
+    if (level >= cfgp->levels) return fail(MP_E_INVALID_CONFIG, "level out of range");
+    if (paths == 0 || paths > 0xFFFFFFFFull) return fail(MP_E_INVALID_CONFIG, "paths must be in 1..2^32-1");
+    HIPCHK(hipSetDevice(c->device));
+    mp_mlmc_config cfg = *cfgp;
+    cfg.sampling = MP_PSEUDORANDOM;
+    prepared P;
+    rc = mlmc_prepare(cfg, mlmc_unit{cfg.seed, level, paths}, P);
+    if (rc != MP_OK) return rc;
+    mp_result r;
+    rc = run_tiles_single(c, P, P.precision, &r, normals, nullptr, nullptr, nullptr, nullptr);
+    if (rc != MP_OK) return rc;
+    sum_sq[0] = r.price;
+    sum_sq[1] = r.std_error;
+    return MP_OK;
 }
 
 MP_EXPORT int mp_hip_lsm_from_normals(mp_ctx* c, const mp_request* req, const double* normals,

@@ -10,14 +10,15 @@
 // Control-variate tiles use the first five slots of a partial row for the
 // discounted moments sum P, sum P^2, sum C, sum C^2, sum P*C (one strike).
 enum mp_technique { MP_TECH_STANDARD = 0, MP_TECH_ANTITHETIC = 1, MP_TECH_CONTROL_VARIATE = 2 };
-// Finalize mode for single-pass Greek estimators (never a request's
-// technique): the eight columns of a partial row are (sum, sum of squares)
-// of four per-path estimates, written out raw.
-#define MP_TECH_GREEK_MOMENTS 3
+// Finalize mode for single-pass Greek estimators and multilevel levels
+// (never a caller's technique): the eight columns of a partial row are
+// (sum, sum of squares) of up to four per-path estimates, written out raw.
+#define MP_TECH_RAW_MOMENTS 3
 enum mp_greek_est { MP_GREEK_EST_PATHWISE = 1, MP_GREEK_EST_LIKELIHOOD_RATIO = 2 };
 // Structured samplers drawn per (point, dimension) in the work-queue and
 // basket kernels (the ABI's sampling codes).
-enum mp_structured { MP_SAMPLER_NONE = 0, MP_SAMPLER_HALTON = 1, MP_SAMPLER_LHS = 2 };
+enum mp_structured { MP_SAMPLER_NONE = 0, MP_SAMPLER_HALTON = 1, MP_SAMPLER_LHS = 2,
+                     MP_SAMPLER_SOBOL = 3 /* multilevel levels only: one Sobol point per path */ };
 
 // One dimension of a structured sampler.  Randomized Halton: a = prime base,
 // shift = Cranley-Patterson shift.  Latin hypercube: a = stride multiplier,
@@ -33,7 +34,10 @@ struct mp_sampler_dim {
 #define MP_LSM_MAX_BLOCKS 512
 
 enum mp_model { MP_MODEL_GBM = 0, MP_MODEL_HESTON = 1, MP_MODEL_MERTON = 2 };
-enum mp_payoff { MP_PAY_EUROPEAN = 0, MP_PAY_ASIAN = 1, MP_PAY_DOWN_AND_OUT = 2, MP_PAY_LOOKBACK = 3 };
+enum mp_payoff { MP_PAY_EUROPEAN = 0, MP_PAY_ASIAN = 1, MP_PAY_DOWN_AND_OUT = 2, MP_PAY_LOOKBACK = 3,
+                 // one level of a multilevel arithmetic Asian: the discounted
+                 // fine payoff minus the coupled coarse payoff (level 0: fine only)
+                 MP_PAY_ASIAN_MLMC = 4 };
 
 // Per-request descriptor resident in HBM.  All fields are uniform across a
 // wave, so the kernels read them with scalar loads.
@@ -63,6 +67,9 @@ struct mp_dreq {
     uint64_t sampler_n;       // points the sampler is built over (requested n_paths)
     double v0, kappa, theta, xi, rho, rho_perp, rate, dt, sqrt_dt;
     double sigma;             // GBM volatility (single-pass Greeks)
+    int32_t mlmc_refine;      // multilevel: fine steps per coarse step (0 at level 0)
+    int32_t pad_mlmc;
+    double mlmc_coarse_drift, mlmc_coarse_vol, mlmc_sqrt_refine;
     double mu_j, sigma_j, jump_mean_dt;
     double cum[MP_POISSON_TERMS];
 };
@@ -117,9 +124,11 @@ struct tile_launch {
     double* terminal_out;
     double* payoff_out;
     const mp_sampler_dim* sampler;  // structured-sampler tables of every request
+    const uint32_t* nib;            // Sobol nibble tables (multilevel Sobol levels)
     int grid;
     int block;
     int ppl;                    // paths interleaved per lane
+    int mlmc;                   // every request is a multilevel level (own kernel)
 };
 
 hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s);

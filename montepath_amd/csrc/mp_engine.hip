@@ -164,6 +164,32 @@ __device__ __forceinline__ void merton_apply(path_state<R>& st, const model_para
     observe(st, m);
 }
 
+// Coarse path of a multilevel level (MlmcCoupled, cpu.rs:6812-6849): the sum
+// of `refine` fine normals over sqrt(refine) drives one coarse step.
+template <class R>
+struct mlmc_coarse {
+    R x;             // coarse log-return
+    mp_kahan<R> acc; // running sum of coarse S/S0
+    R zsum;          // fine normals of the current coarse step
+    int cnt;
+};
+
+template <class R>
+__device__ __forceinline__ void mlmc_step(path_state<R>& st, mlmc_coarse<R>& c, const model_params<R>& m,
+                                          R z, int refine, R coarse_drift, R coarse_vol, R sqrt_refine) {
+    st.x += m.drift_dt + m.vol_dt * z;
+    st.acc.add(mp_exp(st.x));
+    if (refine > 0) {
+        c.zsum += z;
+        if (++c.cnt == refine) {
+            c.x += coarse_drift + coarse_vol * (c.zsum / sqrt_refine);
+            c.acc.add(mp_exp(c.x));
+            c.zsum = R(0);
+            c.cnt = 0;
+        }
+    }
+}
+
 // Undiscounted payoff for strike k (discounting happens once, in FP64, in the
 // finalize kernel).
 template <class R>
@@ -241,6 +267,16 @@ __device__ __noinline__ double structured_normal(const mp_sampler_dim* tab, cons
     return mp_acklam<double>(u);
 }
 
+template <class R, bool U>
+struct sobol_cursor;
+
+// Normal of a structured source at (point, dimension): randomized Halton and
+// Latin hypercube from their tables; scrambled Sobol (multilevel levels) from
+// the nibble tables, block seed and 256-dimension wrap of the Sobol kernels.
+template <class R>
+__device__ __forceinline__ R structured_draw(const mp::tile_launch& L, const mp_dreq& q, uint64_t point,
+                                             uint32_t dim);
+
 // Adds one unit's contribution to the FP64 sums.  Standard: the payoff per
 // strike.  Antithetic: the pair estimate (payoff(+z) + payoff(-z)) / 2 per
 // strike (the reference's block estimate, cpu.rs:7807-7835).  Control
@@ -291,6 +327,10 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
     R kk[MP_NS_MAX];
 #pragma unroll
     for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = R(q.strikes[j]);
+    // multilevel coupling (PAY == MP_PAY_ASIAN_MLMC)
+    const int refine = q.mlmc_refine;
+    const R c_drift = R(q.mlmc_coarse_drift), c_vol = R(q.mlmc_coarse_vol);
+    const R sqrt_ref = R(q.mlmc_sqrt_refine);
     // unit = path, or antithetic pair (the pair's second path takes -z)
     const uint64_t pbeg = (uint64_t)(tile - q.tile0) * q.tile_paths;
     const uint64_t pend = min(pbeg + (uint64_t)q.tile_paths, q.n_paths);
@@ -301,12 +341,19 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
 
     for (uint64_t base = pbeg; base < pend; base += (uint64_t)MP_WAVE * PPL) {
         path_state<R> st[PPL], sa[PPL];
+        mlmc_coarse<R> cs[PPL];
         uint64_t pid[PPL];
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             pid[j] = base + lane + (uint64_t)MP_WAVE * j;
             state_init(st[j], q);
             state_init(sa[j], q);
+            if constexpr (PAY == MP_PAY_ASIAN_MLMC) {
+                cs[j].x = R(0);
+                cs[j].acc.init();
+                cs[j].zsum = R(0);
+                cs[j].cnt = 0;
+            }
         }
         for (int s0i = 0; s0i < n_steps; s0i += 4) {
             const uint32_t blk = (uint32_t)(s0i >> 2);
@@ -322,7 +369,10 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
                     } else if constexpr (SRC == 2) {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
-                            z[e] = e < nsub ? R(structured_normal(L.sampler, q, pr, (uint32_t)(s0i + e))) : R(0);
+                            z[e] = e < nsub ? (PAY == MP_PAY_ASIAN_MLMC
+                                                   ? structured_draw<R>(L, q, pr, (uint32_t)(s0i + e))
+                                                   : R(structured_normal(L.sampler, q, pr, (uint32_t)(s0i + e))))
+                                            : R(0);
                     } else {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
@@ -331,8 +381,12 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         if (e < nsub) {
-                            gbm_step(st[j], m, z[e]);
-                            if (tech == MP_TECH_ANTITHETIC) gbm_step(sa[j], m, -z[e]);
+                            if constexpr (PAY == MP_PAY_ASIAN_MLMC) {
+                                mlmc_step(st[j], cs[j], m, z[e], refine, c_drift, c_vol, sqrt_ref);
+                            } else {
+                                gbm_step(st[j], m, z[e]);
+                                if (tech == MP_TECH_ANTITHETIC) gbm_step(sa[j], m, -z[e]);
+                            }
                         }
                     }
                 } else if (m.model == MP_MODEL_HESTON) {
@@ -399,6 +453,23 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
         }
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
+            if constexpr (PAY == MP_PAY_ASIAN_MLMC) {
+                // the level term, discounted per path as the reference sums it:
+                // disc * (fine avg - K)+ - disc * (coarse avg - K)+
+                if (pid[j] < pend) {
+                    const R fine_avg = s0 * (st[j].acc.s * (R(1) / R(n_steps)));
+                    double d = q.discount * (double)(fine_avg > kk[0] ? fine_avg - kk[0] : R(0));
+                    if (refine > 0) {
+                        const R coarse_avg = s0 * (cs[j].acc.s * (R(1) / R(n_steps / refine)));
+                        d -= q.discount * (double)(coarse_avg > kk[0] ? coarse_avg - kk[0] : R(0));
+                    }
+                    sum[0] += d;
+                    sq[0] += d * d;
+                    if (L.terminal_out) L.terminal_out[pid[j]] = (double)(s0 * mp_exp(st[j].x));
+                    if (L.payoff_out) L.payoff_out[pid[j]] = d;
+                }
+                continue;
+            }
             if (pid[j] < pend) {
                 accumulate_unit(st[j], sa[j], m, q, tech, n_steps, s0, kk, ns, sum, sq);
                 if (L.terminal_out) L.terminal_out[pid[j]] = (double)(s0 * mp_exp(st[j].x));
@@ -422,7 +493,9 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
     }
 }
 
-template <class R, int SRC, int PPL>
+// ML: a launch of multilevel levels only (their own kernel, so the coarse
+// path state does not raise the register count of the general kernel).
+template <class R, int SRC, int PPL, bool ML>
 __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
     __shared__ unsigned int queue_slot[256 / MP_WAVE];
     const int lane = threadIdx.x & (MP_WAVE - 1);
@@ -436,7 +509,17 @@ __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
         const uint32_t tile = L.order ? L.order[qpos] : qpos;
         const uint32_t ri = L.tile_req[tile];
         const mp_dreq& q = L.reqs[ri];
-        if (SRC == 0 && q.sampling != MP_SAMPLER_NONE) {
+        if constexpr (ML) {
+            // multilevel levels, one path per lane
+            if constexpr (SRC == 1) {
+                tile_body<R, 1, 1, MP_MODEL_GBM, MP_PAY_ASIAN_MLMC, -1>(L, tile, q, lane);
+            } else {
+                if (q.sampling != MP_SAMPLER_NONE)
+                    tile_body<R, 2, 1, MP_MODEL_GBM, MP_PAY_ASIAN_MLMC, -1>(L, tile, q, lane);
+                else
+                    tile_body<R, 0, 1, MP_MODEL_GBM, MP_PAY_ASIAN_MLMC, -1>(L, tile, q, lane);
+            }
+        } else if (SRC == 0 && q.sampling != MP_SAMPLER_NONE) {
             // randomized Halton / Latin hypercube requests
             if constexpr (SRC == 0) tile_body<R, 2, PPL, -1, -1, -1>(L, tile, q, lane);
         } else if constexpr (sizeof(R) == 4 && SRC == 0) {
@@ -519,7 +602,7 @@ __global__ __launch_bounds__(64 * MP_NS_MAX) void finalize_kernel(const mp_dreq*
     const int w = threadIdx.x >> 6;
     const double n = (double)q.n_paths;
     const bool cv = q.technique == MP_TECH_CONTROL_VARIATE;
-    const bool moments = q.technique == MP_TECH_GREEK_MOMENTS;
+    const bool moments = q.technique == MP_TECH_RAW_MOMENTS;
     const int n_cols = moments ? MP_PARTIAL_STRIDE : cv ? 5 : 2 * q.n_strikes;
     // Lane l adds tiles l, l + 64, ... in that order; sixteen tiles' column
     // pairs are loaded before any is added.
@@ -677,6 +760,25 @@ struct sobol_cursor {
         return mp_rev32(mp_lk(xrev(dd), mp_hash32(dd ^ seed_o)));
     }
 };
+
+// Scrambled-Sobol normal of (point, dim) for multilevel levels
+// (fill_scrambled_sobol_point_standard_normals: 2^16-point blocks, the block
+// seed advanced by the golden ratio; key0 = the level seed).  Out of line, so
+// the bodies that never draw Sobol keep their registers.
+template <class R>
+__device__ __noinline__ R sobol_point_normal(const uint32_t* nib, uint32_t seed, uint64_t point,
+                                             uint32_t dim) {
+    sobol_cursor<R, false> c;
+    c.init(nib, seed + (uint32_t)(point >> 16) * 0x9E3779B9u, (uint32_t)(point & 0xFFFFu));
+    return c.normal((int)dim);
+}
+
+template <class R>
+__device__ __forceinline__ R structured_draw(const mp::tile_launch& L, const mp_dreq& q, uint64_t point,
+                                             uint32_t dim) {
+    if (q.sampling == MP_SAMPLER_SOBOL) return sobol_point_normal<R>(L.nib, q.key0, point, dim);
+    return R(structured_normal(L.sampler, q, point, dim));
+}
 
 // PAY / SCH >= 0 fix the payoff / Heston scheme at compile time (the
 // request's fields must then hold those values); -1 reads them at run time.
@@ -1444,15 +1546,19 @@ hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s) {
 
 template <class R, int SRC>
 static hipError_t tiles_ppl(const tile_launch& L, hipStream_t s) {
+    if (L.mlmc) {
+        hipLaunchKernelGGL((tile_kernel<R, SRC, 1, true>), dim3(L.grid), dim3(L.block), 0, s, L);
+        return hipGetLastError();
+    }
     switch (L.ppl) {
         case 2:
-            hipLaunchKernelGGL((tile_kernel<R, SRC, 2>), dim3(L.grid), dim3(L.block), 0, s, L);
+            hipLaunchKernelGGL((tile_kernel<R, SRC, 2, false>), dim3(L.grid), dim3(L.block), 0, s, L);
             break;
         case 4:
-            hipLaunchKernelGGL((tile_kernel<R, SRC, 4>), dim3(L.grid), dim3(L.block), 0, s, L);
+            hipLaunchKernelGGL((tile_kernel<R, SRC, 4, false>), dim3(L.grid), dim3(L.block), 0, s, L);
             break;
         default:
-            hipLaunchKernelGGL((tile_kernel<R, SRC, 1>), dim3(L.grid), dim3(L.block), 0, s, L);
+            hipLaunchKernelGGL((tile_kernel<R, SRC, 1, false>), dim3(L.grid), dim3(L.block), 0, s, L);
     }
     return hipGetLastError();
 }

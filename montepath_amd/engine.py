@@ -147,6 +147,68 @@ GREEK_ESTIMATORS = {
 _ESTIMATOR_NAMES = {v: k for k, v in GREEK_ESTIMATORS.items()}
 
 
+MLMC_DEFAULTS = {"s0": 100.0, "k": 100.0, "r": 0.03, "sigma": 0.2, "t": 1.0, "base_steps": 16,
+                 "levels": 4, "refinement_factor": 2,
+                 "paths_per_level": (50_000, 25_000, 12_500, 6_250), "seed": 42,
+                 "sampling": "pseudorandom", "scramble_replicates": 1, "precision": 64,
+                 "contract_id": 0}
+
+
+def make_mlmc_config(**kw) -> A.MlmcConfig:
+    """mp_mlmc_config from ArithmeticAsianMlmcConfig fields (defaults
+    cpu.rs:1551-1568); precision defaults to FP64 like the reference."""
+    unknown = set(kw) - set(MLMC_DEFAULTS)
+    if unknown:
+        raise TypeError(f"unknown MLMC config keys: {sorted(unknown)}")
+    v = dict(MLMC_DEFAULTS)
+    v.update(kw)
+    c = A.MlmcConfig()
+    for f in ("s0", "k", "r", "sigma", "t"):
+        setattr(c, f, float(v[f]))
+    c.base_steps, c.levels = int(v["base_steps"]), int(v["levels"])
+    c.refinement_factor = int(v["refinement_factor"])
+    c.scramble_replicates = int(v["scramble_replicates"])
+    smp = v["sampling"]
+    c.sampling = SAMPLING[smp] if isinstance(smp, str) else int(smp)
+    c.precision = int(v["precision"])
+    c.seed, c.contract_id = int(v["seed"]), int(v["contract_id"])
+    paths = list(v["paths_per_level"])
+    if len(paths) > A.MLMC_MAX_LEVELS:
+        raise ValueError("at most 16 levels")
+    for i, p in enumerate(paths):
+        c.paths_per_level[i] = int(p)
+    return c
+
+
+def _mlmc_dict(r: A.MlmcResult) -> dict:
+    """ArithmeticAsianMlmcResult (cpu.rs:1570-1591) as a dict."""
+    lv = []
+    for i in range(r.n_levels):
+        L = r.levels[i]
+        lv.append({"level": int(L.level), "paths": int(L.paths), "fine_steps": int(L.fine_steps),
+                   "coarse_steps": int(L.coarse_steps) or None, "mean": L.mean,
+                   "variance": L.variance, "stderr": L.std_error,
+                   "cost_step_updates": int(L.cost_step_updates)})
+    return {"price": r.price, "stderr": r.std_error, "total_paths": int(r.total_paths),
+            "total_step_updates": int(r.total_step_updates),
+            "scramble_replicates": int(r.scramble_replicates),
+            "replicate_estimates": [r.replicate_estimates[i] for i in range(r.scramble_replicates)],
+            "levels": lv, "runtime_ms": r.runtime_ms}
+
+
+def _allocation_dict(a: A.MlmcAllocation) -> dict:
+    lv = []
+    for i in range(a.n_levels):
+        L = a.levels[i]
+        lv.append({"level": int(L.level), "fine_steps": int(L.fine_steps),
+                   "coarse_steps": int(L.coarse_steps) or None, "pilot_paths": int(L.pilot_paths),
+                   "pilot_variance": L.pilot_variance, "cost_per_path": int(L.cost_per_path),
+                   "recommended_paths": int(L.recommended_paths)})
+    return {"target_step_updates": int(a.target_step_updates),
+            "estimated_step_updates": int(a.estimated_step_updates),
+            "paths_per_level": [x["recommended_paths"] for x in lv], "levels": lv}
+
+
 def _greek_dict(e: A.GreekEstimate) -> dict:
     """GreekEstimate (cpu.rs:157-169) as a dict; None where the estimator has no value."""
     opt = lambda x: None if x != x else x
@@ -286,6 +348,45 @@ class Engine:
                                                     _estimator_code(estimator), _pd(z),
                                                     C.byref(base), est, C.byref(count)))
         return PriceResult.of(base), [_greek_dict(est[j]) for j in range(count.value)]
+
+    # ------------------------------------------------------------ multilevel
+    def mlmc_price(self, cfg: A.MlmcConfig) -> dict:
+        out = A.MlmcResult()
+        A.check(self.lib.mp_hip_mlmc_price(self._ctx, C.byref(cfg), C.byref(out)))
+        return _mlmc_dict(out)
+
+    def mlmc_tune_allocation(self, cfg: A.MlmcConfig, target_step_updates: int,
+                             pilot_paths: int) -> dict:
+        out = A.MlmcAllocation()
+        A.check(self.lib.mp_hip_mlmc_tune_allocation(self._ctx, C.byref(cfg), target_step_updates,
+                                                     pilot_paths, C.byref(out)))
+        return _allocation_dict(out)
+
+    def mlmc_solve_tolerance(self, cfg: A.MlmcConfig, target_stderr: float,
+                             pilot_paths_per_level: int = 2_048, min_step_updates: int = 100_000,
+                             max_step_updates: int = 10_000_000) -> dict:
+        tol = A.MlmcTolerance(target_stderr, pilot_paths_per_level, min_step_updates,
+                              max_step_updates)
+        out = A.MlmcTolerancePlan()
+        A.check(self.lib.mp_hip_mlmc_solve_tolerance(self._ctx, C.byref(cfg), C.byref(tol),
+                                                     C.byref(out)))
+        alloc = _allocation_dict(out.allocation)
+        return {"target_stderr": out.target_stderr, "estimated_stderr": out.estimated_stderr,
+                "target_met": bool(out.target_met),
+                "max_step_updates_hit": bool(out.max_step_updates_hit),
+                "scramble_replicates": int(out.scramble_replicates),
+                "estimated_step_updates": int(out.estimated_step_updates),
+                "paths_per_level": alloc["paths_per_level"], "allocation": alloc,
+                "recommended_config": out.recommended}
+
+    def mlmc_level_from_normals(self, cfg: A.MlmcConfig, level: int, normals) -> tuple:
+        """(sum, sum of squares) of one level's discounted terms; normals
+        [path][fine step] (transposed to the engine's step-major layout)."""
+        z = np.ascontiguousarray(np.asarray(normals, dtype=np.float64).T)
+        out = np.zeros(2)
+        A.check(self.lib.mp_hip_mlmc_level_from_normals(self._ctx, C.byref(cfg), level, z.shape[1],
+                                                        _pd(z), _pd(out)))
+        return out[0], out[1]
 
     def price_with_paths(self, req: A.Request):
         """Price and return (result, terminal values, undiscounted payoffs)."""

@@ -15,11 +15,12 @@ Every call runs on the GPU through the C-ABI; there is no CPU fallback.
 """
 from __future__ import annotations
 
+import math
 import threading
 from typing import Any, Callable, Mapping
 
 from . import _abi as A
-from .engine import Engine, make_request
+from .engine import MLMC_DEFAULTS, Engine, make_mlmc_config, make_request
 
 KNOWN_ROCM_FUNCTIONS = (
     "price_european_call_rocm",
@@ -33,6 +34,9 @@ KNOWN_ROCM_FUNCTIONS = (
     "price_bermudan_put_rocm",
     "greeks_rocm",
     "price_all_current_greeks_bump_and_revalue_rocm",
+    "arithmetic_asian_mlmc_rocm",
+    "solve_arithmetic_asian_mlmc_tolerance_rocm",
+    "compare_arithmetic_asian_mlmc_reference_rocm",
 )
 
 NATIVE_MODULE = "montepath_amd._lib.libmontepath_hip"
@@ -451,3 +455,114 @@ def price_all_current_greeks_bump_and_revalue_rocm(n_paths: int, n_steps: int,
         raise ValueError(f"native ROCm execution failed: {err}") from err
     return [_greek_report(w, "bump_and_revalue", c, base, est)
             for (w, c, _), (base, est) in zip(built, out)]
+
+
+# --------------------------------------------------------- multilevel Asian
+def _mlmc_parts(config: Mapping):
+    """asian_mlmc_config (lib.rs:490-510): GBM keys and aliases, base_steps,
+    levels, refinement_factor, paths_per_level, seed, sampling,
+    scramble_replicates (defaults cpu.rs:1551-1568)."""
+    d = MLMC_DEFAULTS
+    c = {
+        "s0": _f(config, ["s0", "spot"], d["s0"]), "k": _f(config, ["k", "strike"], d["k"]),
+        "r": _f(config, ["r", "rate"], d["r"]), "sigma": _f(config, ["sigma", "volatility"], d["sigma"]),
+        "t": _f(config, ["t", "maturity"], d["t"]),
+        "base_steps": _u(config, ["base_steps"], d["base_steps"]),
+        "levels": _u(config, ["levels"], d["levels"]),
+        "refinement_factor": _u(config, ["refinement_factor"], d["refinement_factor"]),
+        "seed": _u(config, ["seed"], d["seed"]), "sampling": _sampling(config),
+        "scramble_replicates": _u(config, ["scramble_replicates"], d["scramble_replicates"]),
+    }
+    ppl = config.get("paths_per_level")
+    c["paths_per_level"] = list(d["paths_per_level"]) if ppl is None else [int(x) for x in ppl]
+    if len(c["paths_per_level"]) != c["levels"]:
+        raise ValueError("paths_per_level must contain one positive path count per level")
+    ek = _engine_keys(config)
+    kw = dict(c)
+    kw["precision"] = ek.get("precision", 64)
+    if "contract_id" in ek:
+        kw["contract_id"] = ek["contract_id"]
+    c.update(ek)
+    return c, kw
+
+
+def _mlmc_call(fn):
+    try:
+        return fn()
+    except A.MontepathHipError as err:
+        raise ValueError(f"native ROCm execution failed: {err}") from err
+
+
+def arithmetic_asian_mlmc_rocm(config: Mapping) -> dict:
+    """Multilevel arithmetic Asian (the reference's arithmetic_asian_mlmc,
+    lib.rs:218-230 -> arithmetic_asian_call_price_mlmc_cpu): every level of
+    every replicate in one GPU launch.  Response as workload_response
+    (lib.rs:709-728): {values: ArithmeticAsianMlmcResult, manifest, warnings,
+    stderr}."""
+    c, kw = _mlmc_parts(config)
+    res = _mlmc_call(lambda: _engine().mlmc_price(make_mlmc_config(**kw)))
+    return {
+        "values": res,
+        "manifest": {
+            "schema_version": "python-native-runtime.v1", "workload": "arithmetic_asian_mlmc",
+            "backend": "rocm_hip", "execution_mode": "native_extension_rocm",
+            "function": "arithmetic_asian_mlmc_rocm", "native_module": NATIVE_MODULE,
+            "config": c, "reproducibility": REPRODUCIBILITY,
+            "performance_claim": "use benchmark artifacts for timing claims",
+        },
+        "warnings": [],
+        "stderr": res["stderr"],
+    }
+
+
+def solve_arithmetic_asian_mlmc_tolerance_rocm(config: Mapping, tolerance: Mapping) -> dict:
+    """solve_arithmetic_asian_mlmc_tolerance_cpu (cpu.rs:3352-3423): paths per
+    level for a target standard error (defaults cpu.rs:1620-1629)."""
+    c, kw = _mlmc_parts(config)
+    plan = _mlmc_call(lambda: _engine().mlmc_solve_tolerance(
+        make_mlmc_config(**kw), float(tolerance.get("target_stderr", 0.05)),
+        int(tolerance.get("pilot_paths_per_level", 2_048)),
+        int(tolerance.get("min_step_updates", 100_000)),
+        int(tolerance.get("max_step_updates", 10_000_000))))
+    rec = dict(c)
+    rec["paths_per_level"] = list(plan["paths_per_level"])
+    if rec["sampling"] == "pseudorandom":
+        rec["scramble_replicates"] = 1
+    plan["recommended_config"] = rec
+    return plan
+
+
+def compare_arithmetic_asian_mlmc_reference_rocm(config: Mapping, reference: Mapping) -> dict:
+    """compare_arithmetic_asian_mlmc_reference_cpu (cpu.rs:3425-3491): the
+    multilevel estimate against a standard Asian run (both on the GPU)."""
+    c, kw = _mlmc_parts(config)
+    est = _mlmc_call(lambda: _engine().mlmc_price(make_mlmc_config(**kw)))
+    rc, rkw = _arithmetic_asian_call_parts(reference)
+    ref = _mlmc_call(lambda: _engine().price(make_request("arithmetic_asian_call", **rkw)))
+    error = est["price"] - ref.price
+    combined = math.sqrt(est["stderr"] ** 2 + ref.stderr ** 2)
+    if combined > 0.0:
+        units = error / combined
+    else:
+        units = 0.0 if error == 0.0 else math.copysign(math.inf, error)
+    eps = 2.220446049250313e-16
+    warnings = []
+    if any(abs(c[key] - rc[key]) > eps for key in ("s0", "k", "r", "sigma", "t")):
+        warnings.append("MLMC config and reference config do not share the same market parameters")
+    finest = c["base_steps"] * c["refinement_factor"] ** (c["levels"] - 1)
+    if rc["n_steps"] < finest:
+        warnings.append("reference step count is below the finest MLMC level; comparison includes "
+                        "time-discretization mismatch")
+    if abs(units) > 4.0:
+        warnings.append(f"MLMC estimate is {abs(units):.3f} combined standard errors from the "
+                        "reference run")
+    return {
+        "workload": "arithmetic_asian_call", "backend": "rocm_hip",
+        "reference_name": "high_budget_arithmetic_asian_standard_mc",
+        "sampling": c["sampling"], "levels": c["levels"], "estimate_price": est["price"],
+        "estimate_stderr": est["stderr"], "estimate_step_updates": est["total_step_updates"],
+        "reference_price": ref.price, "reference_stderr": ref.stderr,
+        "reference_paths": rc["n_paths"], "reference_steps": rc["n_steps"], "error": error,
+        "abs_error": abs(error), "combined_stderr": combined,
+        "error_combined_stderr_units": units, "warnings": warnings,
+    }

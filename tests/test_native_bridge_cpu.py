@@ -42,6 +42,12 @@ class FakeEngine:
                                for g in names]))
         return out[0] if single else out
 
+    def mlmc_price(self, cfg):
+        self.mlmc_cfgs = getattr(self, "mlmc_cfgs", []) + [cfg]
+        return {"price": 5.3, "stderr": 0.05, "total_paths": 1, "total_step_updates": 1,
+                "scramble_replicates": 1, "replicate_estimates": [5.3],
+                "levels": [{"level": 0, "coarse_steps": None}], "runtime_ms": 0.1}
+
 
 @pytest.fixture
 def fake():
@@ -188,3 +194,22 @@ def test_all_workload_greeks_run_as_one_batch(fake):
 def test_bad_greek_requests_raise_value_error(fake, args, msg):
     with pytest.raises(ValueError, match=msg):
         native.greeks_rocm(*args)
+
+
+def test_mlmc_config_keys_and_defaults(fake):
+    out = native.arithmetic_asian_mlmc_rocm({"spot": 95.0, "levels": 3, "base_steps": 8,
+                                             "paths_per_level": [400, 200, 100],
+                                             "sampling": "latin-hypercube",
+                                             "scramble_replicates": 4})
+    c = fake.mlmc_cfgs[-1]
+    assert (c.s0, c.k, c.levels, c.base_steps, c.refinement_factor) == (95.0, 100.0, 3, 8, 2)
+    assert list(c.paths_per_level[:3]) == [400, 200, 100]
+    assert c.sampling == A.LATIN_HYPERCUBE and c.scramble_replicates == 4
+    # the reference prices multilevel runs in FP64
+    assert c.precision == 64 and c.seed == 42
+    assert out["stderr"] == 0.05 and out["manifest"]["workload"] == "arithmetic_asian_mlmc"
+    native.arithmetic_asian_mlmc_rocm({})
+    c = fake.mlmc_cfgs[-1]
+    assert list(c.paths_per_level[:4]) == [50_000, 25_000, 12_500, 6_250] and c.base_steps == 16
+    with pytest.raises(ValueError, match="one positive path count per level"):
+        native.arithmetic_asian_mlmc_rocm({"levels": 3})
