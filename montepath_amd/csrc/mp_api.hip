@@ -14,6 +14,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <limits>
@@ -139,7 +140,11 @@ struct mp_ctx {
     dbuf normals, uniforms, jumps, term, pay;
     dbuf plan, basket_par;
     dbuf lsm_paths, lsm_cash, lsm_ex, lsm_coeff, lsm_valid, lsm_gram, lsm_par, lsm_cnt,
-        lsm_part, lsm_out, plain_pay;
+        lsm_part, lsm_out, lsm_dates, plain_pay;
+    // one-launch LSM pricing when the request's grid fits on the device at
+    // once; MONTEPATH_LSM_MULTI_LAUNCH=1 at open selects the per-date launches
+    bool lsm_fused = true;
+    int lsm_blocks_per_cu[2][2][5] = {};  // [fp64][normals][log2 J], 0 = not yet queried
     dbuf draws;
     dbuf sampler;
 };
@@ -169,19 +174,29 @@ void sobol_directions(uint32_t dirs[256][32]) {
     }
 }
 
-// Nibble tables: nib[d][j][n] = XOR of direction numbers 4j..4j+3 selected
-// by the bits of n, so a 16-bit index needs four lookups.
-void sobol_nibbles(const uint32_t dirs[256][32], std::vector<uint32_t>& nib) {
-    nib.assign(256 * 64, 0u);
-    for (int d = 0; d < 256; ++d)
-        for (int j = 0; j < 4; ++j)
-            for (int n = 0; n < 16; ++n) {
+// Index tables: for each dimension, 128 16-bit words -- entries 0..63 are
+// the XORs of direction numbers 0..5 selected by a 6-bit value, 64..95 of
+// numbers 6..10 and 96..127 of numbers 11..15 (5-bit values) -- so a 16-bit
+// index needs three lookups.  Stored bit-reversed (the kernels scramble the
+// reversed point); the first 16 direction numbers have their low 16 bits
+// clear, so the reversed XORs fit 16 bits.  Returns false if they do not.
+bool sobol_index_tables(const uint32_t dirs[256][32], std::vector<uint32_t>& words) {
+    static const int lo[3] = {0, 6, 11}, nb[3] = {6, 5, 5}, at[3] = {0, 64, 96};
+    std::vector<uint16_t> tab(256 * 128, 0);
+    for (int d = 0; d < 256; ++d) {
+        for (int k = 0; k < 16; ++k)
+            if (dirs[d][k] & 0xFFFFu) return false;
+        for (int j = 0; j < 3; ++j)
+            for (int n = 0; n < (1 << nb[j]); ++n) {
                 uint32_t x = 0;
-                for (int b = 0; b < 4; ++b)
-                    if ((n >> b) & 1) x ^= dirs[d][4 * j + b];
-                // bit-reversed: the kernels scramble the reversed point
-                nib[d * 64 + j * 16 + n] = __builtin_bitreverse32(x);
+                for (int b = 0; b < nb[j]; ++b)
+                    if ((n >> b) & 1) x ^= dirs[d][lo[j] + b];
+                tab[d * 128 + at[j] + n] = (uint16_t)__builtin_bitreverse32(x);
             }
+    }
+    words.assign(tab.size() / 2, 0u);
+    std::memcpy(words.data(), tab.data(), tab.size() * sizeof(uint16_t));
+    return true;
 }
 
 // Streaming Brownian-bridge plan (preorder of the reference's
@@ -945,7 +960,9 @@ int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const
     if (normals_host) HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, nn * 8, hipMemcpyHostToDevice, c->stream));
     const double* bp = c->basket_par.as<double>();
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(mp::launch_basket(c->reqs.as<mp_dreq>(), c->sampler.as<mp_sampler_dim>(), na, bp, bp + na * na, bp + na * na + na,
+    HIPCHK(mp::launch_basket(c->reqs.as<mp_dreq>(), c->sampler.as<mp_sampler_dim>(), na,
+                             d.sampling != MP_SAMPLER_NONE, d.technique, bp,
+                             bp + na * na, bp + na * na + na,
                              bp + na * na + 2 * na, bp + na * na + 3 * na,
                              normals_host ? c->normals.as<double>() : nullptr, c->partial.as<double>(),
                              term_host ? c->term.as<double>() : nullptr,
@@ -1004,8 +1021,30 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     }
     const size_t rb = precision == 64 ? 8 : 4;
     const int block = 256;
-    const int grid = (int)std::max<uint64_t>(
-        1, std::min<uint64_t>({(n + block - 1) / block, (uint64_t)c->n_cu * 2, (uint64_t)MP_LSM_MAX_BLOCKS}));
+    // regression dates: schedule without its last element, latest first
+    const int ns = (int)P.schedule.size();
+    std::vector<int> reg_dates;
+    for (int i = ns - 2; i >= 0; --i) reg_dates.push_back((int)P.schedule[i]);
+    const int n_reg = (int)reg_dates.size();
+    // one launch: J paths per thread, the fewest that keep every block resident
+    int fused_j = 0, fused_grid = 0;
+    if (c->lsm_fused) {
+        for (int lj = 0; lj < 5 && !fused_j; ++lj) {
+            const int J = 1 << lj;
+            const uint64_t g = (n + (uint64_t)block * J - 1) / ((uint64_t)block * J);
+            int& bpc = c->lsm_blocks_per_cu[precision == 64][normals_host != nullptr][lj];
+            if (bpc == 0) bpc = std::max(-1, mp::lsm_fused_blocks_per_cu(precision, normals_host != nullptr, J));
+            if (bpc <= 0) continue;
+            if (g <= (uint64_t)MP_LSM_MAX_BLOCKS && g <= (uint64_t)bpc * (uint64_t)c->n_cu) {
+                fused_j = J;
+                fused_grid = (int)std::max<uint64_t>(1, g);
+            }
+        }
+    }
+    const int grid = fused_j ? fused_grid
+                             : (int)std::max<uint64_t>(1, std::min<uint64_t>({(n + block - 1) / block,
+                                                                              (uint64_t)c->n_cu * 2,
+                                                                              (uint64_t)MP_LSM_MAX_BLOCKS}));
     ENSURE(c->lsm_paths, (size_t)(m - 1) * n * rb);
     ENSURE(c->lsm_cash, n * 8);
     ENSURE(c->lsm_ex, n * 2);
@@ -1013,27 +1052,40 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     ENSURE(c->lsm_valid, (size_t)(m + 1) * 4);
     ENSURE(c->lsm_gram, (size_t)grid * 16 * 8);
     ENSURE(c->lsm_par, blob_bytes);
-    ENSURE(c->lsm_cnt, (size_t)(m + 2) * 4);
+    // counters: tickets [m + 2], then (one launch) date flags [m + 2] and the error flag
+    ENSURE(c->lsm_cnt, (size_t)(2 * (m + 2) + 1) * 4);
     ENSURE(c->lsm_part, (size_t)grid * 4 * 8);
     ENSURE(c->lsm_out, 8 * 8);
+    ENSURE(c->lsm_dates, (size_t)std::max(n_reg, 1) * 4);
     if (normals_host) ENSURE(c->normals, (size_t)m * n * 8);
     HIPCHK(hipMemcpyAsync(c->lsm_par.p, blob.data(), blob_bytes, hipMemcpyHostToDevice, c->stream));
+    if (n_reg)
+        HIPCHK(hipMemcpyAsync(c->lsm_dates.p, reg_dates.data(), (size_t)n_reg * 4, hipMemcpyHostToDevice,
+                              c->stream));
     if (normals_host)
         HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, (size_t)m * n * 8, hipMemcpyHostToDevice, c->stream));
     const mp_lsm_params* dp = c->lsm_par.as<mp_lsm_params>();
     unsigned int* cnt = c->lsm_cnt.as<unsigned int>();
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(mp::launch_lsm_fill(cnt, m + 2, c->lsm_valid.as<int>(), c->lsm_coeff.as<double>(), m + 1,
-                               c->stream));
+    HIPCHK(mp::launch_lsm_fill(cnt, 2 * (m + 2) + 1, c->lsm_valid.as<int>(), c->lsm_coeff.as<double>(),
+                               m + 1, c->stream));
+    int launches = 2;
+    if (fused_j) {
+        // 2^20 polls of at least 128 cycles each: far beyond any legitimate wait
+        HIPCHK(mp::launch_lsm_fused(dp, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
+                                    c->lsm_dates.as<int>(), n_reg, c->lsm_coeff.as<double>(),
+                                    c->lsm_valid.as<int>(), c->lsm_gram.as<double>(), c->lsm_part.as<double>(),
+                                    cnt, cnt + (m + 2), c->lsm_ex.as<uint16_t>(), c->lsm_out.as<double>(),
+                                    cnt + 2 * (m + 2), 1u << 20, fused_j, grid, precision, c->stream));
+    } else {
     HIPCHK(mp::launch_lsm_paths(dp, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
                                 c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), precision, grid,
                                 block, c->stream));
-    // regression dates: schedule without its last element, latest first; each
-    // pass also applies the exercise rule of the date regressed before it
-    const int ns = (int)P.schedule.size();
-    int prev = 0, launches = 3;
-    for (int i = ns - 2; i >= 0; --i) {
-        const int date = (int)P.schedule[i];
+    // each pass also applies the exercise rule of the date regressed before it
+    int prev = 0;
+    launches = 3;
+    for (int i = 0; i < n_reg; ++i) {
+        const int date = reg_dates[i];
         HIPCHK(mp::launch_lsm_step(dp, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
                                    c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, date,
                                    c->lsm_gram.as<double>(), cnt, precision, grid, block, c->stream));
@@ -1049,9 +1101,13 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     HIPCHK(mp::launch_lsm_final(dp, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), c->lsm_valid.as<int>(),
                                 c->lsm_part.as<double>(), cnt + (m + 1), c->lsm_out.as<double>(), grid, block,
                                 c->stream));
+    }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     double res[8];
+    unsigned int wait_err = 0;
     HIPCHK(hipMemcpyAsync(res, c->lsm_out.p, 5 * 8, hipMemcpyDeviceToHost, c->stream));
+    if (fused_j)
+        HIPCHK(hipMemcpyAsync(&wait_err, cnt + 2 * (m + 2), 4, hipMemcpyDeviceToHost, c->stream));
     std::vector<double> coeff;
     if (coeff_host) {
         coeff.resize((size_t)(m + 1) * 4);
@@ -1063,6 +1119,9 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
         HIPCHK(hipMemcpyAsync(ex.data(), c->lsm_ex.p, n * 2, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (wait_err)
+        return fail(MP_E_HIP, "LSM: a block waited too long for the other blocks of its launch "
+                              "(the device was shared with other work); result discarded");
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     c->last.main_kernel_ms = ms;
@@ -1229,11 +1288,13 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
     c->max_lds = prop.sharedMemPerBlock;
     c->name = prop.name;
     c->arch = prop.gcnArchName;
+    if (const char* v = std::getenv("MONTEPATH_LSM_MULTI_LAUNCH")) c->lsm_fused = !(v[0] == '1');
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     sobol_directions(c->dirs);
     std::vector<uint32_t> nib;
-    sobol_nibbles(c->dirs, nib);
+    if (!sobol_index_tables(c->dirs, nib))
+        return fail(MP_E_UNSUPPORTED, "Sobol direction numbers 0..15 have low bits set");
     HIPCHK(hipMalloc(&c->d_nib, nib.size() * 4));
     HIPCHK(hipMemcpy(c->d_nib, nib.data(), nib.size() * 4, hipMemcpyHostToDevice));
     *out = c.release();

@@ -195,7 +195,8 @@ __device__ __forceinline__ R mp_acklam(R p) {
 // tail (w >= 5, 0.34% of draws) is rare.  p lies on the 2^-23 grid in
 // [0,1) and arrives as the bits of the float 1 + p; 4p(1-p) is formed
 // exactly up to one rounding.  p == 0 returns the FP64 reference value at
-// f64::MIN_POSITIVE.
+// f64::MIN_POSITIVE; it lies in the tail (log2 0 = -inf), so the test for
+// it costs nothing on the central branch.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float mp_inv_normal_f32(uint32_t fbits) {
     // fbits = 0x3f800000 | t23 is the float 1 + p, p = t23 * 2^-23
@@ -228,9 +229,10 @@ __device__ __forceinline__ float mp_inv_normal_f32(uint32_t fbits) {
         r = 0.0133485785f + r * w;
         r = 1.41658103f + r * w;
         r = 4.00643444f + r * w;
+        // p == 0: x = -1, so r * x below is exactly the tail value
+        r = fbits == 0x3f800000u ? 37.51937936814848f : r;
     }
-    const float z = r * x;
-    return fbits == 0x3f800000u ? -37.51937936814848f : z;  // p == 0
+    return r * x;
 }
 
 // max(v, 0) for finite v in one instruction (v_med3 with FLT_MAX; a plain

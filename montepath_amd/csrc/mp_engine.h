@@ -31,7 +31,7 @@ struct mp_sampler_dim {
 #define MP_SAMPLER_MAX_DIMS 65536
 // LSM kernels run at most this many blocks: the last block reads every
 // block's partial sums in one round trip (two rows per thread).
-#define MP_LSM_MAX_BLOCKS 512
+#define MP_LSM_MAX_BLOCKS 1024
 
 enum mp_model { MP_MODEL_GBM = 0, MP_MODEL_HESTON = 1, MP_MODEL_MERTON = 2 };
 enum mp_payoff { MP_PAY_EUROPEAN = 0, MP_PAY_ASIAN = 1, MP_PAY_DOWN_AND_OUT = 2, MP_PAY_LOOKBACK = 3,
@@ -124,7 +124,7 @@ struct tile_launch {
     double* terminal_out;
     double* payoff_out;
     const mp_sampler_dim* sampler;  // structured-sampler tables of every request
-    const uint32_t* nib;            // Sobol nibble tables (multilevel Sobol levels)
+    const uint32_t* nib;            // Sobol index tables (multilevel Sobol levels)
     int grid;
     int block;
     int ppl;                    // paths interleaved per lane
@@ -141,11 +141,12 @@ hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, i
                       double* terminal_out, double* payoff_out, int precision, int grid, int block,
                       bool nib_lds, hipStream_t s);
 size_t qmc_lds_bytes(int precision, int factors, int n_slots, int block, bool nib_lds);
-hipError_t launch_basket(const mp_dreq* req, const mp_sampler_dim* sampler, int n_assets, const double* chol,
-                         const double* drift, const double* vol, const double* log_s0,
-                         const double* weight, const double* normals, double* partial,
-                         double* terminal_out, double* payoff_out, int precision, int grid,
-                         int block, hipStream_t s);
+// structured: Halton / Latin hypercube draws; technique: MP_TECH_*
+hipError_t launch_basket(const mp_dreq* req, const mp_sampler_dim* sampler, int n_assets,
+                         bool structured, int technique, const double* chol, const double* drift,
+                         const double* vol, const double* log_s0, const double* weight,
+                         const double* normals, double* partial, double* terminal_out,
+                         double* payoff_out, int precision, int grid, int block, hipStream_t s);
 // Single-pass terminal-GBM Greeks (estimator mp_greek_est): per-block
 // partial rows of four estimates' (sum, sum of squares).
 hipError_t launch_greeks(const mp_dreq* req, int estimator, const double* normals, double* partial,
@@ -159,6 +160,15 @@ hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* ca
                            uint16_t* ex, double* coeff_all, int* valid_all, int step_update,
                            int step_gram, double* gram_partial, unsigned int* counters,
                            int precision, int grid, int block, hipStream_t s);
+// Whole pricing in one launch (paths per thread J in {1,2,4,8,16}, grid
+// blocks of 256 threads, all resident at once); err is set when a wait for
+// another block timed out.
+int lsm_fused_blocks_per_cu(int precision, bool normals, int J);
+hipError_t launch_lsm_fused(const mp_lsm_params* p, const double* normals, void* paths,
+                            const int* reg_dates, int n_reg, double* coeff_all, int* valid_all,
+                            double* gram_partial, double* fin_partial, unsigned int* tickets,
+                            unsigned int* flags, uint16_t* ex, double* out, unsigned int* err,
+                            unsigned int spin_limit, int J, int grid, int precision, hipStream_t s);
 hipError_t launch_lsm_final(const mp_lsm_params* p, const double* cash, const uint16_t* ex,
                             const int* valid_all, double* partial, unsigned int* counter,
                             double* out, int grid, int block, hipStream_t s);

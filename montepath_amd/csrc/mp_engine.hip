@@ -20,7 +20,7 @@
 
 namespace {
 
-constexpr int kNibWords = 256 * 4 * 16;  // Sobol nibble tables: [dim][nibble position][16]
+constexpr int kNibWords = 256 * 128 / 2;  // Sobol index tables: [dim][128] 16-bit words (64 KiB)
 
 // ---------------------------------------------------------------------------
 // Per-path model state
@@ -272,7 +272,7 @@ struct sobol_cursor;
 
 // Normal of a structured source at (point, dimension): randomized Halton and
 // Latin hypercube from their tables; scrambled Sobol (multilevel levels) from
-// the nibble tables, block seed and 256-dimension wrap of the Sobol kernels.
+// the index tables, block seed and 256-dimension wrap of the Sobol kernels.
 template <class R>
 __device__ __forceinline__ R structured_draw(const mp::tile_launch& L, const mp_dreq& q, uint64_t point,
                                              uint32_t dim);
@@ -689,15 +689,19 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* dst) {
 // ---------------------------------------------------------------------------
 // Sobol + Brownian bridge kernel (GBM with F = 1, Heston with F = 2).
 // One block per tile of blockDim paths (grid-stride, static assignment);
-// nibble tables of the direction numbers in LDS, bridge stack per lane in LDS,
+// index tables of the direction numbers in LDS, bridge stack per lane in LDS,
 // plan ops read with scalar loads.
 // ---------------------------------------------------------------------------
 // Scrambled Sobol points of one lane (one path index).  Dimensions come in
 // blocks of 256 ("offsets") that share a seed, seed_o = bseed + offset; the
 // point index is Owen-shuffled within its 2^16-point block once per offset,
-// which fixes four per-lane pointers into the nibble tables.  The tables hold
-// direction numbers bit-reversed, so the XOR of four table words is the
-// reversed point that the Laine-Karras permutation takes directly.  With
+// which fixes three per-lane pointers into the index tables.  A point with a
+// 16-bit index has only its top 16 bits set (direction number k < 16 is
+// m_k 2^-(k+1) with m_k < 2^(k+1)), so the tables hold the bit-reversed
+// XORs as 16-bit words: entries 0..63 cover index bits 0..5, 64..95 bits
+// 6..10 and 96..127 bits 11..15 of each dimension.  The XOR of three words
+// (one v_bitop3) is the reversed point that the Laine-Karras permutation
+// takes directly.  With
 // U = true the caller guarantees bseed is the same in every lane of the wave
 // (paths of one tile never straddle a 2^16 block), so the per-dimension
 // hash runs on the scalar unit and the permutation uses 24-bit multiplies
@@ -705,7 +709,8 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* dst) {
 template <class R, bool U>
 struct sobol_cursor {
     const uint32_t* nib;
-    const uint32_t *t0, *t1, *t2, *t3;
+    const uint16_t *t0, *t1, *t2;  // dimension 0 of the current offset
+    const uint16_t *c0, *c1, *c2;  // the walk position (seek / advance)
     uint32_t bseed, seed_o, local;
     int off;
     __device__ __forceinline__ void init(const uint32_t* tables, uint32_t block_seed, uint32_t local_index) {
@@ -718,36 +723,57 @@ struct sobol_cursor {
         off = o;
         seed_o = bseed + (uint32_t)o;
         const uint32_t s = mp_sobol_shuffle(local, mp_hash32(seed_o));
-        t0 = nib + (s & 15u);
-        t1 = nib + 16 + ((s >> 4) & 15u);
-        t2 = nib + 32 + ((s >> 8) & 15u);
-        t3 = nib + 48 + (s >> 12);
+        const uint16_t* tab = reinterpret_cast<const uint16_t*>(nib);
+        t0 = tab + (s & 63u);
+        t1 = tab + 64 + ((s >> 6) & 31u);
+        t2 = tab + 96 + (s >> 11);
+        c0 = t0;
+        c1 = t1;
+        c2 = t2;
     }
     __device__ __forceinline__ uint32_t xrev(uint32_t dd) const {
-        const uint32_t o = dd * 64u;
-        return __builtin_amdgcn_bitop3_b32(t0[o], t1[o], t2[o], 0x96) ^ t3[o];
+        const uint32_t o = dd * 128u;
+        return __builtin_amdgcn_bitop3_b32((uint32_t)t0[o], (uint32_t)t1[o], (uint32_t)t2[o], 0x96);
+    }
+    // Sequential walks over the dimensions of the current offset keep three
+    // lane pointers that move once per group of dimensions, so the table
+    // reads of a group take immediate offsets.  set_offset places the walk at
+    // dimension 0, xrev_rel(j) reads dimension (walk position + j) and
+    // advance(n) moves the walk n dimensions on.
+    __device__ __forceinline__ void advance(int n) {
+        c0 += n * 128;
+        c1 += n * 128;
+        c2 += n * 128;
+    }
+    __device__ __forceinline__ uint32_t xrev_rel(int j) const {
+        return __builtin_amdgcn_bitop3_b32((uint32_t)c0[j * 128], (uint32_t)c1[j * 128],
+                                           (uint32_t)c2[j * 128], 0x96);
     }
     // Top 23 bits of the scrambled point of dimension dd (< 256) of the
-    // current offset, from the 24-bit-exact permutation.
-    __device__ __forceinline__ uint32_t scrambled_rev(uint32_t dd) const {
+    // current offset whose reversed unscrambled point is x, from the
+    // 24-bit-exact permutation.
+    __device__ __forceinline__ uint32_t scrambled_rev(uint32_t x, uint32_t dd) const {
         const uint32_t hs = mp_hash32(dd ^ seed_o);
         uint32_t y;
         if constexpr (U)
-            y = mp_lk_low24_su(xrev(dd), hs);
+            y = mp_lk_low24_su(x, hs);
         else
-            y = mp_lk_low24(xrev(dd), hs);
+            y = mp_lk_low24(x, hs);
         return mp_rev32(y);
     }
     // Standard normal of dimension dd of the current offset: inverse normal
     // of the path precision at the 23-bit uniform p = top 23 bits * 2^-23.
-    __device__ __forceinline__ R normal_at(uint32_t dd) const {
+    __device__ __forceinline__ R normal_of(uint32_t x, uint32_t dd) const {
         if constexpr (sizeof(R) == 4) {
             // (0x7f:point) >> 9 = the bits of the float 1 + p
-            return mp_inv_normal_f32(__builtin_amdgcn_alignbit(0x7fu, scrambled_rev(dd), 9));
+            return mp_inv_normal_f32(__builtin_amdgcn_alignbit(0x7fu, scrambled_rev(x, dd), 9));
         } else {
-            return mp_acklam<R>(R(scrambled_rev(dd) >> 9) * R(0x1p-23));
+            return mp_acklam<R>(R(scrambled_rev(x, dd) >> 9) * R(0x1p-23));
         }
     }
+    __device__ __forceinline__ R normal_at(uint32_t dd) const { return normal_of(xrev(dd), dd); }
+    // Dimension dd, which is the walk position + j.
+    __device__ __forceinline__ R normal_rel(uint32_t dd, int j) const { return normal_of(xrev_rel(j), dd); }
     // Any dimension g, moving the offset as needed.
     __device__ __forceinline__ R normal(int g) {
         if ((g >> 8) != off) set_offset(g >> 8);
@@ -859,8 +885,9 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
                 const int s1 = min(n_steps, s0i + spo);
                 for (int s = s0i; s < s1; ++s) {
                     const uint32_t dd = (uint32_t)((s - s0i) * F);
-                    const R z0 = cur.normal_at(dd);
-                    const R z1 = F == 2 ? cur.normal_at(dd + 1) : R(0);
+                    const R z0 = cur.normal_rel(dd, 0);
+                    const R z1 = F == 2 ? cur.normal_rel(dd + 1, 1) : R(0);
+                    cur.advance(F);
                     if constexpr (kInc)
                         emit(z0 * z_to_step, z1 * z_to_step);
                     else
@@ -875,7 +902,8 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
             // subtracts two O(1) values.  Stack slot s holds D of the interval
             // whose right end is owned by that slot.
 #pragma unroll
-            for (int f = 0; f < F; ++f) stack[(0 * F + f) * bd + tid] = sqrt_t * cur.normal_at((uint32_t)f);
+            for (int f = 0; f < F; ++f) stack[(0 * F + f) * bd + tid] = sqrt_t * cur.normal_rel((uint32_t)f, f);
+            cur.advance(F);
             if (qp.n_ops == 0) {
                 const R i0 = kInc ? stack[tid] : stack[tid] * to_step;
                 const R i1 = F == 2 ? (kInc ? stack[bd + tid] : stack[bd + tid] * to_step) : R(0);
@@ -902,11 +930,12 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
                     R dl[F], dr[F];
 #pragma unroll
                     for (int f = 0; f < F; ++f) {
-                        const R z = cur.normal_at(dd + f);
+                        const R z = cur.normal_rel(dd + f, f);
                         const R d = stack[(slot * F + f) * bd + tid];
                         dl[f] = rw * d + sd * z;
                         dr[f] = d - dl[f];
                     }
+                    cur.advance(F);
                     if (flags & 1) {
                         if constexpr (kInc)
                             emit(dl[0], F == 2 ? dl[F - 1] : R(0));
@@ -952,7 +981,11 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
 // n-asset basket (stepped, Cholesky-correlated shocks), NA assets.
 // Philox stream a drives asset a's independent normals.
 // ---------------------------------------------------------------------------
-template <class R, int NA, int SRC>
+// SRC: 0 Philox, 1 caller normals, 2 structured sampler (Halton / Latin
+// hypercube; out-of-line draws, so the Philox body carries no call).  TECH
+// is the request's technique (MP_TECH_*), compiled in: a standard request
+// keeps one path state per asset and two moments.
+template <class R, int NA, int SRC, int TECH>
 __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__ reqp,
                                                     const double* __restrict__ chol,
                                                     const double* __restrict__ drift,
@@ -976,7 +1009,7 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
     }
     const int n_steps = q.n_steps;
     const R kk = R(q.strikes[0]);
-    const int tech = q.technique;
+    constexpr bool ANTI = TECH == MP_TECH_ANTITHETIC;
     for (uint32_t tile = blockIdx.x; tile < q.n_tiles; tile += gridDim.x) {
         // unit = path, or antithetic pair (second path xa takes -z)
         const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
@@ -991,15 +1024,13 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 if constexpr (SRC == 0) {
-                    if (q.sampling != MP_SAMPLER_NONE) {
-                        // structured sampler: asset a of step s is dimension s * NA + a
+                    mp_normals4<R>((uint32_t)p, (uint32_t)(s0i >> 2), (uint32_t)a, q.seed_hi, q.key0,
+                                   q.key1, z[a]);
+                } else if constexpr (SRC == 2) {
+                    // structured sampler: asset a of step s is dimension s * NA + a
 #pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            z[a][e] = e < nsub ? R(structured_normal(sampler, q, pr, (uint32_t)((s0i + e) * NA + a))) : R(0);
-                    } else {
-                        mp_normals4<R>((uint32_t)p, (uint32_t)(s0i >> 2), (uint32_t)a, q.seed_hi,
-                                       q.key0, q.key1, z[a]);
-                    }
+                    for (int e = 0; e < 4; ++e)
+                        z[a][e] = e < nsub ? R(structured_normal(sampler, q, pr, (uint32_t)((s0i + e) * NA + a))) : R(0);
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
@@ -1016,7 +1047,7 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
                         for (int b = 0; b <= a; ++b) c += L[a][b] * z[b][e];
                         x[a] += dr[a] + vo[a] * c;
                         // the correlated shock of -z is exactly -c
-                        if (tech == MP_TECH_ANTITHETIC) xa[a] += dr[a] + vo[a] * -c;
+                        if constexpr (ANTI) xa[a] += dr[a] + vo[a] * -c;
                     }
                 }
             }
@@ -1025,14 +1056,14 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             basket += w[a] * (s0a[a] * mp_exp(x[a]));
-            if (tech == MP_TECH_ANTITHETIC) basket_a += w[a] * (s0a[a] * mp_exp(xa[a]));
+            if constexpr (ANTI) basket_a += w[a] * (s0a[a] * mp_exp(xa[a]));
         }
         const R pay = basket > kk ? basket - kk : R(0);
         // standard: payoff, payoff^2; antithetic: the pair estimate; control
         // variate: discounted payoff and control (discount * basket at T) moments
         double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (active) {
-            if (tech == MP_TECH_CONTROL_VARIATE) {
+            if constexpr (TECH == MP_TECH_CONTROL_VARIATE) {
                 const double pd = q.discount * (double)pay, cd = q.discount * (double)basket;
                 v[0] = pd;
                 v[1] = pd * pd;
@@ -1041,15 +1072,14 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
                 v[4] = pd * cd;
             } else {
                 double est = (double)pay;
-                if (tech == MP_TECH_ANTITHETIC)
-                    est = 0.5 * (est + (double)(basket_a > kk ? basket_a - kk : R(0)));
+                if constexpr (ANTI) est = 0.5 * (est + (double)(basket_a > kk ? basket_a - kk : R(0)));
                 v[0] = est;
                 v[1] = est * est;
             }
             if (terminal_out) terminal_out[p] = (double)basket;
             if (payoff_out) payoff_out[p] = (double)pay;
         }
-        if (tech == MP_TECH_CONTROL_VARIATE) {
+        if constexpr (TECH == MP_TECH_CONTROL_VARIATE) {
             block_sum_store<5>(v, partial + (uint64_t)tile * MP_PARTIAL_STRIDE);
         } else {
             double vv[2] = {v[0], v[1]};
@@ -1267,23 +1297,28 @@ __device__ __forceinline__ bool publish_and_ticket(double (&v)[NV], double* dst,
 }
 
 // Fixed-order FP64 reduction of NV values over nb <= MP_LSM_MAX_BLOCKS block
-// partials laid out [block][stride], 256 threads: thread t loads rows t and
-// t + 256 in one round trip and adds them; 16 threads per value then sum 16
-// of those in order, and one thread per value sums the 16 results.
+// partials laid out [block][stride], 256 threads: thread t loads rows t,
+// t + 256, ... (at most four) in one round trip and adds them in that order;
+// 16 threads per value then sum 16 of those in order, and one thread per
+// value sums the 16 results.
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* partial, int nb, int stride,
                                                 double* tot /* shared, NV */) {
     __shared__ double row[256][NV + 1];
     __shared__ double red[NV][17];
     const int t = threadIdx.x;
-    double a[NV], b[NV];
+    double a[4][NV];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            a[r][i] = t + 256 * r < nb ? mp_load_coherent(partial + (uint64_t)(t + 256 * r) * stride + i) : 0.0;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        a[i] = t < nb ? mp_load_coherent(partial + (uint64_t)t * stride + i) : 0.0;
-        b[i] = t + 256 < nb ? mp_load_coherent(partial + (uint64_t)(t + 256) * stride + i) : 0.0;
+        double v = a[0][i] + a[1][i];
+        if (nb > 512) v = (v + a[2][i]) + a[3][i];
+        row[t][i] = v;
     }
-#pragma unroll
-    for (int i = 0; i < NV; ++i) row[t][i] = a[i] + b[i];
     __syncthreads();
     const int vi = t >> 4, j = t & 15;
     if (vi < NV) {
@@ -1445,6 +1480,235 @@ __global__ __launch_bounds__(256) void lsm_final_kernel(const mp_lsm_params* __r
         out[3] = tot[3];
         out[4] = (double)rs;
         *counter = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The whole Longstaff-Schwartz pricing in one launch (requests up to
+// 16 x 256 x MP_LSM_MAX_BLOCKS paths).  Thread t of block b owns paths
+// b*256 + t + j*G*256 (j < J, G blocks): it generates them forward, writes
+// S at dates 1..M-1 step-major, and keeps their discounted cash flows and
+// exercise dates in registers for the whole backward pass.  Per regression
+// date each block publishes its Gram/RHS sums and takes a ticket; the last
+// block reduces all blocks' sums in fixed order, solves the 4x4 system,
+// publishes the coefficients and raises the date's flag, which the other
+// blocks wait for (a bounded wait: after spin_limit polls a block records
+// an error and carries on, so every wave reaches the end).  The block count
+// must not exceed what the device holds at once (checked by the host).
+// Arithmetic and its order per path equal lsm_paths_kernel, lsm_step_kernel
+// and lsm_final_kernel; only the grouping of the FP64 Gram sums differs.
+// ---------------------------------------------------------------------------
+template <class R, int SRC, int J>
+__global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __restrict__ Pp,
+                                                       const double* __restrict__ normals,
+                                                       R* __restrict__ paths,
+                                                       const int* __restrict__ reg_dates, int n_reg,
+                                                       double* coeff_all, int* valid_all,
+                                                       double* gram_partial, double* fin_partial,
+                                                       unsigned int* tickets, unsigned int* flags,
+                                                       uint16_t* ex_out, double* out,
+                                                       unsigned int* err, unsigned int spin_limit) {
+    const mp_lsm_params& P = *Pp;
+    const uint64_t n = P.n_paths;
+    const int m = P.n_steps;
+    const double k = P.k, inv_k = P.inv_k;
+    const int deg = P.degree, nc = P.coeffs;
+    const double* disc_pow = reinterpret_cast<const double*>(Pp + 1);
+    const double* growth = disc_pow + (m + 1);
+    const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t p0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+
+    // forward paths (lsm_paths_kernel's arithmetic)
+    double cash[J];
+    int exd[J];
+    {
+        const R drift = R(P.drift_dt), vol = R(P.vol_dt), s0 = R(P.s0);
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const uint64_t p = p0 + (uint64_t)j * T;
+            cash[j] = 0.0;
+            exd[j] = m;
+            if (p >= n) continue;
+            R x = R(0);
+            R sT = s0;
+            for (int s0i = 0; s0i < m; s0i += 4) {
+                R z[4];
+                const int nsub = min(4, m - s0i);
+                if constexpr (SRC == 0) {
+                    mp_normals4<R>((uint32_t)p, (uint32_t)(s0i >> 2), 0u, P.seed_hi, P.key0, P.key1, z);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) z[e] = e < nsub ? R(normals[(uint64_t)(s0i + e) * n + p]) : R(0);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (e < nsub) {
+                        x += drift + vol * z[e];
+                        const int step = s0i + e + 1;
+                        const R S = s0 * mp_exp(x);
+                        if (step < m) paths[(uint64_t)(step - 1) * n + p] = S;
+                        sT = S;
+                    }
+                }
+            }
+            const double d = k - (double)sT;
+            cash[j] = (d > 0.0 ? d : 0.0) * P.disc_m;
+        }
+    }
+
+    __shared__ double tot[15];
+    __shared__ double beta_s[4];
+    __shared__ int valid_s;
+    // prefetch the prices of the first regression date
+    R sd[J];
+    if (n_reg > 0) {
+        const R* pd = paths + (uint64_t)(reg_dates[0] - 1) * n;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const uint64_t p = p0 + (uint64_t)j * T;
+            sd[j] = p < n ? pd[p] : R(k);
+        }
+    }
+    for (int i = 0; i < n_reg; ++i) {
+        const int date = reg_dates[i];
+        const double grow = growth[date];
+        double acc[15];
+#pragma unroll
+        for (int q = 0; q < 15; ++q) acc[q] = 0.0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const double s = (double)sd[j];
+            const double imm = k - s > 0.0 ? k - s : 0.0;
+            if (imm > 0.0) {  // false for p >= n (sd = k)
+                double bb[4];
+                laguerre(s * inv_k, deg, bb);
+                const double cont = cash[j] * grow;
+                int e = 0;
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = a; b < 4; ++b) acc[e++] += bb[a] * bb[b];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) acc[10 + a] += bb[a] * cont;
+                acc[14] += 1.0;
+            }
+        }
+        const bool last = publish_and_ticket<15>(acc, gram_partial + (uint64_t)blockIdx.x * 16, &tickets[i]);
+        if (last) {
+            reduce_partials<15>(gram_partial, gridDim.x, 16, tot);
+            if (threadIdx.x == 0) {
+                double mm[4][4], rhs[4];
+                int e = 0;
+                for (int a = 0; a < 4; ++a)
+                    for (int b = a; b < 4; ++b) {
+                        mm[a][b] = tot[e];
+                        mm[b][a] = tot[e];
+                        ++e;
+                    }
+                for (int a = 0; a < 4; ++a) rhs[a] = tot[10 + a];
+                int ok = 0;
+                if (tot[14] >= (double)nc) ok = lsm_solve4(mm, rhs, nc) ? 1 : 0;
+                for (int a = 0; a < 4; ++a) {
+                    const double c = (ok && a < nc) ? rhs[a] : 0.0;
+                    beta_s[a] = c;
+                    mp_store_coherent(coeff_all + date * 4 + a, c);
+                }
+                valid_s = ok;
+                __hip_atomic_store(valid_all + date, ok, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                mp_vmem_drain_release();
+                __hip_atomic_store(flags + i, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (threadIdx.x == 0) {
+            unsigned int polls = 0;
+            while (__hip_atomic_load(flags + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                __builtin_amdgcn_s_sleep(2);
+                // once any wait has timed out, no block waits again
+                if (++polls > spin_limit ||
+                    __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            mp_vmem_drain_acquire();
+            for (int a = 0; a < 4; ++a) beta_s[a] = mp_load_coherent(coeff_all + date * 4 + a);
+            valid_s = __hip_atomic_load(valid_all + date, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // prefetch the next date's prices while the block waits
+        R sn[J];
+        if (i + 1 < n_reg) {
+            const R* pn = paths + (uint64_t)(reg_dates[i + 1] - 1) * n;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const uint64_t p = p0 + (uint64_t)j * T;
+                sn[j] = p < n ? pn[p] : R(k);
+            }
+        }
+        __syncthreads();
+        if (valid_s) {
+            double beta[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) beta[a] = beta_s[a];
+            const double dpow = disc_pow[date];
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const double s = (double)sd[j];
+                const double imm = k - s > 0.0 ? k - s : 0.0;
+                if (imm > 0.0) {
+                    double bb[4];
+                    laguerre(s * inv_k, deg, bb);
+                    double cont = 0.0;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) cont += beta[a] * bb[a];
+                    if (imm > (cont > 0.0 ? cont : 0.0)) {
+                        cash[j] = imm * dpow;
+                        exd[j] = date;
+                    }
+                }
+            }
+        }
+        if (i + 1 < n_reg) {
+#pragma unroll
+            for (int j = 0; j < J; ++j) sd[j] = sn[j];
+        }
+        __syncthreads();  // beta_s / valid_s are rewritten at the next date
+    }
+
+    // price, standard error and exercise counts (lsm_final_kernel)
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const uint64_t p = p0 + (uint64_t)j * T;
+        if (p >= n) continue;
+        const double c = cash[j];
+        v[0] += c;
+        v[1] += c * c;
+        if (exd[j] < m)
+            v[2] += 1.0;
+        else if (c > 0.0)
+            v[3] += 1.0;
+        if (ex_out) ex_out[p] = (uint16_t)exd[j];
+    }
+    if (!publish_and_ticket<4>(v, fin_partial + (uint64_t)blockIdx.x * 4, &tickets[n_reg])) return;
+    reduce_partials<4>(fin_partial, gridDim.x, 4, tot);
+    if (threadIdx.x == 0) {
+        int rs = 0;
+        for (int s = 1; s <= m; ++s) rs += __hip_atomic_load(valid_all + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1 : 0;
+        const double nn = (double)n;
+        const double mean = tot[0] / nn;
+        const double var = tot[1] / nn - mean * mean;
+        double price = mean, se = sqrt(var > 0.0 ? var : 0.0) / sqrt(nn);
+        if (P.allow_intrinsic) {
+            const double intrinsic = P.k - P.s0 > 0.0 ? P.k - P.s0 : 0.0;
+            if (intrinsic > price) {
+                price = intrinsic;
+                se = 0.0;
+            }
+        }
+        out[0] = price;
+        out[1] = se;
+        out[2] = tot[2];
+        out[3] = tot[3];
+        out[4] = (double)rs;
     }
 }
 
@@ -1622,29 +1886,50 @@ hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, i
     return hipGetLastError();
 }
 
-template <class R, int NA>
-static hipError_t basket_na(const mp_dreq* req, const double* chol, const double* drift,
-                            const double* vol, const double* logs0, const double* weight,
-                            const double* normals, const mp_sampler_dim* sampler, double* partial,
-                            double* term, double* pay, int grid, int block, hipStream_t s) {
-    if (normals)
-        hipLaunchKernelGGL((basket_kernel<R, NA, 1>), dim3(grid), dim3(block), 0, s, req, chol,
+template <class R, int NA, int TECH>
+static void basket_src(int src, const mp_dreq* req, const double* chol, const double* drift,
+                       const double* vol, const double* logs0, const double* weight,
+                       const double* normals, const mp_sampler_dim* sampler, double* partial,
+                       double* term, double* pay, int grid, int block, hipStream_t s) {
+    if (src == 1)
+        hipLaunchKernelGGL((basket_kernel<R, NA, 1, TECH>), dim3(grid), dim3(block), 0, s, req, chol,
+                           drift, vol, logs0, weight, normals, sampler, partial, term, pay);
+    else if (src == 2)
+        hipLaunchKernelGGL((basket_kernel<R, NA, 2, TECH>), dim3(grid), dim3(block), 0, s, req, chol,
                            drift, vol, logs0, weight, normals, sampler, partial, term, pay);
     else
-        hipLaunchKernelGGL((basket_kernel<R, NA, 0>), dim3(grid), dim3(block), 0, s, req, chol,
+        hipLaunchKernelGGL((basket_kernel<R, NA, 0, TECH>), dim3(grid), dim3(block), 0, s, req, chol,
                            drift, vol, logs0, weight, normals, sampler, partial, term, pay);
+}
+
+template <class R, int NA>
+static hipError_t basket_na(int src, int tech, const mp_dreq* req, const double* chol,
+                            const double* drift, const double* vol, const double* logs0,
+                            const double* weight, const double* normals, const mp_sampler_dim* sampler,
+                            double* partial, double* term, double* pay, int grid, int block,
+                            hipStream_t s) {
+    if (tech == MP_TECH_ANTITHETIC)
+        basket_src<R, NA, MP_TECH_ANTITHETIC>(src, req, chol, drift, vol, logs0, weight, normals,
+                                              sampler, partial, term, pay, grid, block, s);
+    else if (tech == MP_TECH_CONTROL_VARIATE)
+        basket_src<R, NA, MP_TECH_CONTROL_VARIATE>(src, req, chol, drift, vol, logs0, weight, normals,
+                                                   sampler, partial, term, pay, grid, block, s);
+    else
+        basket_src<R, NA, MP_TECH_STANDARD>(src, req, chol, drift, vol, logs0, weight, normals,
+                                            sampler, partial, term, pay, grid, block, s);
     return hipGetLastError();
 }
 
 template <class R>
-static hipError_t basket_r(int na, const mp_dreq* req, const double* chol, const double* drift,
-                           const double* vol, const double* logs0, const double* weight,
-                           const double* normals, const mp_sampler_dim* sampler, double* partial,
-                           double* term, double* pay, int grid, int block, hipStream_t s) {
-#define MP_BASKET_CASE(N)                                                                       \
-    case N:                                                                                     \
-        return basket_na<R, N>(req, chol, drift, vol, logs0, weight, normals, sampler, partial, \
-                               term, pay, grid, block, s);
+static hipError_t basket_r(int na, int src, int tech, const mp_dreq* req, const double* chol,
+                           const double* drift, const double* vol, const double* logs0,
+                           const double* weight, const double* normals, const mp_sampler_dim* sampler,
+                           double* partial, double* term, double* pay, int grid, int block,
+                           hipStream_t s) {
+#define MP_BASKET_CASE(N)                                                                      \
+    case N:                                                                                    \
+        return basket_na<R, N>(src, tech, req, chol, drift, vol, logs0, weight, normals, sampler, \
+                               partial, term, pay, grid, block, s);
     switch (na) {
         MP_BASKET_CASE(1)
         MP_BASKET_CASE(2)
@@ -1661,15 +1946,16 @@ static hipError_t basket_r(int na, const mp_dreq* req, const double* chol, const
 }
 
 hipError_t launch_basket(const mp_dreq* req, const mp_sampler_dim* sampler, int n_assets,
-                         const double* chol, const double* drift, const double* vol,
-                         const double* log_s0, const double* weight, const double* normals,
-                         double* partial, double* terminal_out, double* payoff_out, int precision,
-                         int grid, int block, hipStream_t s) {
+                         bool structured, int technique, const double* chol, const double* drift,
+                         const double* vol, const double* log_s0, const double* weight,
+                         const double* normals, double* partial, double* terminal_out,
+                         double* payoff_out, int precision, int grid, int block, hipStream_t s) {
+    const int src = normals ? 1 : structured ? 2 : 0;
     if (precision == 64)
-        return basket_r<double>(n_assets, req, chol, drift, vol, log_s0, weight, normals, sampler,
-                                partial, terminal_out, payoff_out, grid, block, s);
-    return basket_r<float>(n_assets, req, chol, drift, vol, log_s0, weight, normals, sampler,
-                           partial, terminal_out, payoff_out, grid, block, s);
+        return basket_r<double>(n_assets, src, technique, req, chol, drift, vol, log_s0, weight,
+                                normals, sampler, partial, terminal_out, payoff_out, grid, block, s);
+    return basket_r<float>(n_assets, src, technique, req, chol, drift, vol, log_s0, weight, normals,
+                           sampler, partial, terminal_out, payoff_out, grid, block, s);
 }
 
 template <class R>
@@ -1729,6 +2015,45 @@ hipError_t launch_lsm_paths(const mp_lsm_params* p, const double* normals, void*
                                normals, (float*)paths, cash, ex);
     }
     return hipGetLastError();
+}
+
+template <class R, int SRC>
+static const void* lsm_fused_fn(int J) {
+    switch (J) {
+        case 1: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 1>);
+        case 2: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 2>);
+        case 4: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 4>);
+        case 8: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 8>);
+        case 16: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 16>);
+        default: return nullptr;
+    }
+}
+
+static const void* lsm_fused_select(int precision, bool normals, int J) {
+    if (precision == 64) return normals ? lsm_fused_fn<double, 1>(J) : lsm_fused_fn<double, 0>(J);
+    return normals ? lsm_fused_fn<float, 1>(J) : lsm_fused_fn<float, 0>(J);
+}
+
+int lsm_fused_blocks_per_cu(int precision, bool normals, int J) {
+    const void* f = lsm_fused_select(precision, normals, J);
+    int nb = 0;
+    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
+}
+
+hipError_t launch_lsm_fused(const mp_lsm_params* p, const double* normals, void* paths,
+                            const int* reg_dates, int n_reg, double* coeff_all, int* valid_all,
+                            double* gram_partial, double* fin_partial, unsigned int* tickets,
+                            unsigned int* flags, uint16_t* ex, double* out, unsigned int* err,
+                            unsigned int spin_limit, int J, int grid, int precision, hipStream_t s) {
+    const void* f = lsm_fused_select(precision, normals != nullptr, J);
+    if (!f || grid < 1 || grid > MP_LSM_MAX_BLOCKS) return hipErrorInvalidValue;
+    void* args[] = {&p, &normals, &paths, &reg_dates, &n_reg, &coeff_all, &valid_all, &gram_partial,
+                    &fin_partial, &tickets, &flags, &ex, &out, &err, &spin_limit};
+    return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, s);
 }
 
 hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* cash,
