@@ -1035,7 +1035,10 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
             int& bpc = c->lsm_blocks_per_cu[precision == 64][normals_host != nullptr][lj];
             if (bpc == 0) bpc = std::max(-1, mp::lsm_fused_blocks_per_cu(precision, normals_host != nullptr, J));
             if (bpc <= 0) continue;
-            if (g <= (uint64_t)MP_LSM_MAX_BLOCKS && g <= (uint64_t)bpc * (uint64_t)c->n_cu) {
+            // write-once slots of every date: at most 64 MiB
+            const uint64_t slot_bytes = (uint64_t)n_reg * (16 * g + 16 * 32 + 8) * 8;
+            if (g <= (uint64_t)MP_LSM_MAX_BLOCKS && g <= (uint64_t)bpc * (uint64_t)c->n_cu &&
+                slot_bytes <= (64ull << 20)) {
                 fused_j = J;
                 fused_grid = (int)std::max<uint64_t>(1, g);
             }
@@ -1050,10 +1053,15 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     ENSURE(c->lsm_ex, n * 2);
     ENSURE(c->lsm_coeff, (size_t)(m + 1) * 4 * 8);
     ENSURE(c->lsm_valid, (size_t)(m + 1) * 4);
-    ENSURE(c->lsm_gram, (size_t)grid * 16 * 8);
+    // per-date launches: block partials [grid][16]; one launch: write-once
+    // slots [n_reg][16][grid], group totals [n_reg][16][32] and coefficient
+    // broadcasts [n_reg][8]
+    const size_t n_slots = fused_j ? (size_t)n_reg * (16 * grid + 16 * 32 + 8) : 0;
+    ENSURE(c->lsm_gram, std::max((size_t)grid * 16, n_slots) * 8);
     ENSURE(c->lsm_par, blob_bytes);
-    // counters: tickets [m + 2], then (one launch) date flags [m + 2] and the error flag
-    ENSURE(c->lsm_cnt, (size_t)(2 * (m + 2) + 1) * 4);
+    // counters: per-date launches use tickets [m + 2]; one launch uses the
+    // final ticket [0] and the error flag [1]
+    ENSURE(c->lsm_cnt, (size_t)(m + 2) * 4);
     ENSURE(c->lsm_part, (size_t)grid * 4 * 8);
     ENSURE(c->lsm_out, 8 * 8);
     ENSURE(c->lsm_dates, (size_t)std::max(n_reg, 1) * 4);
@@ -1067,16 +1075,21 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     const mp_lsm_params* dp = c->lsm_par.as<mp_lsm_params>();
     unsigned int* cnt = c->lsm_cnt.as<unsigned int>();
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(mp::launch_lsm_fill(cnt, 2 * (m + 2) + 1, c->lsm_valid.as<int>(), c->lsm_coeff.as<double>(),
-                               m + 1, c->stream));
+    HIPCHK(mp::launch_lsm_fill(cnt, m + 2, c->lsm_valid.as<int>(), c->lsm_coeff.as<double>(), m + 1,
+                               c->stream));
     int launches = 2;
     if (fused_j) {
-        // 2^20 polls of at least 128 cycles each: far beyond any legitimate wait
+        double* slots = c->lsm_gram.as<double>();
+        if (n_slots) HIPCHK(mp::launch_slot_fill(slots, n_slots, c->stream));
+        // 2^20 polls of at least 64 cycles and one memory round trip each:
+        // far beyond any legitimate wait; cnt[0] is the final ticket, cnt[1] the error flag
         HIPCHK(mp::launch_lsm_fused(dp, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
                                     c->lsm_dates.as<int>(), n_reg, c->lsm_coeff.as<double>(),
-                                    c->lsm_valid.as<int>(), c->lsm_gram.as<double>(), c->lsm_part.as<double>(),
-                                    cnt, cnt + (m + 2), c->lsm_ex.as<uint16_t>(), c->lsm_out.as<double>(),
-                                    cnt + 2 * (m + 2), 1u << 20, fused_j, grid, precision, c->stream));
+                                    c->lsm_valid.as<int>(), slots, slots + (size_t)n_reg * 16 * grid,
+                                    slots + (size_t)n_reg * 16 * (grid + 32), c->lsm_part.as<double>(), cnt, c->lsm_ex.as<uint16_t>(),
+                                    c->lsm_out.as<double>(), cnt + 1, 1u << 20, fused_j, grid, precision,
+                                    c->stream));
+        ++launches;
     } else {
     HIPCHK(mp::launch_lsm_paths(dp, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
                                 c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), precision, grid,
@@ -1106,8 +1119,7 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     double res[8];
     unsigned int wait_err = 0;
     HIPCHK(hipMemcpyAsync(res, c->lsm_out.p, 5 * 8, hipMemcpyDeviceToHost, c->stream));
-    if (fused_j)
-        HIPCHK(hipMemcpyAsync(&wait_err, cnt + 2 * (m + 2), 4, hipMemcpyDeviceToHost, c->stream));
+    if (fused_j) HIPCHK(hipMemcpyAsync(&wait_err, cnt + 1, 4, hipMemcpyDeviceToHost, c->stream));
     std::vector<double> coeff;
     if (coeff_host) {
         coeff.resize((size_t)(m + 1) * 4);

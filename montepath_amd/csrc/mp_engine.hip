@@ -1296,29 +1296,17 @@ __device__ __forceinline__ bool publish_and_ticket(double (&v)[NV], double* dst,
     return is_last;
 }
 
-// Fixed-order FP64 reduction of NV values over nb <= MP_LSM_MAX_BLOCKS block
-// partials laid out [block][stride], 256 threads: thread t loads rows t,
-// t + 256, ... (at most four) in one round trip and adds them in that order;
-// 16 threads per value then sum 16 of those in order, and one thread per
-// value sums the 16 results.
+// Fixed-order FP64 sum of NV values over nb <= MP_LSM_MAX_BLOCKS rows,
+// 256 threads: thread t first adds its rows t, t + 256, ... in that order
+// (the caller's rowsum), then 16 threads per value sum 16 of those in
+// order, and one thread per value sums the 16 results into tot (shared).
 template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* partial, int nb, int stride,
-                                                double* tot /* shared, NV */) {
+__device__ __forceinline__ void sum_rows(const double (&rowsum)[NV], double* tot) {
     __shared__ double row[256][NV + 1];
     __shared__ double red[NV][17];
     const int t = threadIdx.x;
-    double a[4][NV];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
-            a[r][i] = t + 256 * r < nb ? mp_load_coherent(partial + (uint64_t)(t + 256 * r) * stride + i) : 0.0;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double v = a[0][i] + a[1][i];
-        if (nb > 512) v = (v + a[2][i]) + a[3][i];
-        row[t][i] = v;
-    }
+    for (int i = 0; i < NV; ++i) row[t][i] = rowsum[i];
     __syncthreads();
     const int vi = t >> 4, j = t & 15;
     if (vi < NV) {
@@ -1335,6 +1323,106 @@ __device__ __forceinline__ void reduce_partials(const double* partial, int nb, i
         tot[t] = s.value();
     }
     __syncthreads();
+}
+
+// The same over block partials laid out [block][stride], published with
+// mp_store_coherent before a ticket (publish_and_ticket).
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* partial, int nb, int stride,
+                                                double* tot /* shared, NV */) {
+    const int t = threadIdx.x;
+    double a[4][NV];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            a[r][i] = t + 256 * r < nb ? mp_load_coherent(partial + (uint64_t)(t + 256 * r) * stride + i) : 0.0;
+    double v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        v[i] = a[0][i] + a[1][i];
+        if (nb > 512) v[i] = (v[i] + a[2][i]) + a[3][i];
+    }
+    sum_rows<NV>(v, tot);
+}
+
+// Write-once slots for values published to other blocks: filled with this
+// bit pattern (a signalling NaN, which no arithmetic produces) before the
+// launch, written once with a coherent 64-bit store, and read by polling
+// until the pattern is gone.  A slot needs no fence or flag: its one store
+// is the signal.
+#define MP_SLOT_EMPTY 0x7FF45E471E570B5Eull
+
+__device__ __forceinline__ bool mp_slot_read(const double* p, double& v) {
+    const unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = __longlong_as_double((long long)b);
+    return b != MP_SLOT_EMPTY;
+}
+
+// Bounded poll accounting: true when the caller should stop waiting (its
+// polls ran out, or another wait already did), after recording the error.
+__device__ __forceinline__ bool mp_poll_give_up(unsigned int& polls, unsigned int limit, unsigned int* err) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++polls > limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+    return false;
+}
+
+// Fixed-order sum of NV <= 16 values over the rows row0 .. row0 + nrows - 1
+// (nrows <= 32) of write-once slots laid out [value][stride], waiting until
+// they have arrived; 256 threads, result in out (shared).  Thread (v, j)
+// reads rows j and j + 16 of value v -- two loads per lane, because a
+// coherent load costs the issuing CU about one request per lane -- and adds
+// them; one thread per value then sums the 16 pairs in order (Neumaier).
+template <int NV>
+__device__ __forceinline__ void reduce_group(const double* base, uint64_t stride, int row0, int nrows,
+                                             unsigned int spin_limit, unsigned int* err, double* out) {
+    __shared__ double pair[16][17];
+    const int t = threadIdx.x, v = t >> 4, j = t & 15;
+    if (v < NV) {
+        double a0 = 0.0, a1 = 0.0;
+        unsigned int polls = 0;
+        for (;;) {
+            bool full = true;
+            if (j < nrows) full &= mp_slot_read(base + (uint64_t)v * stride + row0 + j, a0);
+            if (j + 16 < nrows) full &= mp_slot_read(base + (uint64_t)v * stride + row0 + j + 16, a1);
+            if (full || mp_poll_give_up(polls, spin_limit, err)) break;
+        }
+        pair[v][j] = a0 + a1;
+    }
+    __syncthreads();
+    if (t < NV) {
+        mp_neumaier acc;
+        acc.init();
+        for (int q = 0; q < 16; ++q) acc.add(pair[t][q]);
+        out[t] = acc.value();
+    }
+    __syncthreads();
+}
+
+// Block sum of NV values; thread i < NV stores value i to dst[i * stride]
+// as a write-once slot.
+template <int NV>
+__device__ __forceinline__ void block_sum_to_slots(double (&v)[NV], double* dst, uint64_t stride) {
+    __shared__ double red[16][NV];
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = mp_wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[wave][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double acc = 0.0;
+        for (int w = 0; w < nw; ++w) acc += red[w][threadIdx.x];
+        mp_store_coherent(dst + threadIdx.x * stride, acc);
+    }
 }
 
 // One pass per exercise date: apply the exercise rule of date step_update
@@ -1489,14 +1577,19 @@ __global__ __launch_bounds__(256) void lsm_final_kernel(const mp_lsm_params* __r
 // b*256 + t + j*G*256 (j < J, G blocks): it generates them forward, writes
 // S at dates 1..M-1 step-major, and keeps their discounted cash flows and
 // exercise dates in registers for the whole backward pass.  Per regression
-// date each block publishes its Gram/RHS sums and takes a ticket; the last
-// block reduces all blocks' sums in fixed order, solves the 4x4 system,
-// publishes the coefficients and raises the date's flag, which the other
-// blocks wait for (a bounded wait: after spin_limit polls a block records
-// an error and carries on, so every wave reaches the end).  The block count
-// must not exceed what the device holds at once (checked by the host).
-// Arithmetic and its order per path equal lsm_paths_kernel, lsm_step_kernel
-// and lsm_final_kernel; only the grouping of the FP64 Gram sums differs.
+// date i every block stores its Gram/RHS sums into write-once slots
+// slots[i][value][block]; block g < NG waits for the sums of blocks
+// 32g .. 32g + 31 and stores their fixed-order total into gslots[i][value][g];
+// block 0 waits for the NG group totals, adds them in group order, solves
+// the 4x4 system and stores the coefficients and the validity flag into the
+// write-once slots bcast[i][0..4], which the other blocks wait for.  (A
+// coherent load costs the issuing CU about one request per lane, so one
+// block polling every block's sums would take ~15 us per date.)  Every wait is bounded: after spin_limit polls a
+// block records an error and carries on, so every wave reaches the end.
+// The block count must not exceed what the device holds at once (checked
+// by the host).  Arithmetic and its order per path equal lsm_paths_kernel,
+// lsm_step_kernel and lsm_final_kernel; only the grouping of the FP64 Gram
+// sums differs (paths per thread, then wave, block and fixed block order).
 // ---------------------------------------------------------------------------
 template <class R, int SRC, int J>
 __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __restrict__ Pp,
@@ -1504,8 +1597,8 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
                                                        R* __restrict__ paths,
                                                        const int* __restrict__ reg_dates, int n_reg,
                                                        double* coeff_all, int* valid_all,
-                                                       double* gram_partial, double* fin_partial,
-                                                       unsigned int* tickets, unsigned int* flags,
+                                                       double* slots, double* gslots, double* bcast,
+                                                       double* fin_partial, unsigned int* ticket,
                                                        uint16_t* ex_out, double* out,
                                                        unsigned int* err, unsigned int spin_limit) {
     const mp_lsm_params& P = *Pp;
@@ -1515,7 +1608,8 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
     const int deg = P.degree, nc = P.coeffs;
     const double* disc_pow = reinterpret_cast<const double*>(Pp + 1);
     const double* growth = disc_pow + (m + 1);
-    const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+    const int G = gridDim.x;
+    const uint64_t T = (uint64_t)G * blockDim.x;
     const uint64_t p0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 
     // forward paths (lsm_paths_kernel's arithmetic)
@@ -1556,7 +1650,7 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
         }
     }
 
-    __shared__ double tot[15];
+    __shared__ double tot[15], gsum[15];
     __shared__ double beta_s[4];
     __shared__ int valid_s;
     // prefetch the prices of the first regression date
@@ -1593,9 +1687,25 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
                 acc[14] += 1.0;
             }
         }
-        const bool last = publish_and_ticket<15>(acc, gram_partial + (uint64_t)blockIdx.x * 16, &tickets[i]);
-        if (last) {
-            reduce_partials<15>(gram_partial, gridDim.x, 16, tot);
+        double* slot_i = slots + (uint64_t)i * 16 * G;
+        double* gslot_i = gslots + (uint64_t)i * 16 * 32;
+        double* bc = bcast + (uint64_t)i * 8;
+        block_sum_to_slots<15>(acc, slot_i + blockIdx.x, (uint64_t)G);
+        // blocks 0 .. NG-1 each reduce a group of 32 blocks' sums
+        const int NG = (G + 31) >> 5;
+        if ((int)blockIdx.x < NG) {
+            const int g = blockIdx.x;
+            reduce_group<15>(slot_i, (uint64_t)G, 32 * g, min(32, G - 32 * g), spin_limit, err, gsum);
+            if (NG > 1 && threadIdx.x < 15) mp_store_coherent(gslot_i + threadIdx.x * 32 + g, gsum[threadIdx.x]);
+        }
+        if (blockIdx.x == 0) {
+            // the group sums, in group order
+            if (NG > 1) {
+                reduce_group<15>(gslot_i, 32, 0, NG, spin_limit, err, tot);
+            } else if (threadIdx.x < 15) {
+                tot[threadIdx.x] = gsum[threadIdx.x];
+            }
+            __syncthreads();
             if (threadIdx.x == 0) {
                 double mm[4][4], rhs[4];
                 int e = 0;
@@ -1608,30 +1718,29 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
                 for (int a = 0; a < 4; ++a) rhs[a] = tot[10 + a];
                 int ok = 0;
                 if (tot[14] >= (double)nc) ok = lsm_solve4(mm, rhs, nc) ? 1 : 0;
+                double c[4];
                 for (int a = 0; a < 4; ++a) {
-                    const double c = (ok && a < nc) ? rhs[a] : 0.0;
-                    beta_s[a] = c;
-                    mp_store_coherent(coeff_all + date * 4 + a, c);
+                    c[a] = (ok && a < nc) ? rhs[a] : 0.0;
+                    beta_s[a] = c[a];
+                    mp_store_coherent(bc + a, c[a]);
                 }
+                mp_store_coherent(bc + 4, ok ? 1.0 : 0.0);
                 valid_s = ok;
+                // host outputs and the regression count (read after the final ticket)
+                for (int a = 0; a < 4; ++a) mp_store_coherent(coeff_all + date * 4 + a, c[a]);
                 __hip_atomic_store(valid_all + date, ok, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                mp_vmem_drain_release();
-                __hip_atomic_store(flags + i, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         } else if (threadIdx.x == 0) {
+            double bv[5];
             unsigned int polls = 0;
-            while (__hip_atomic_load(flags + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-                __builtin_amdgcn_s_sleep(2);
-                // once any wait has timed out, no block waits again
-                if (++polls > spin_limit ||
-                    __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
+            for (;;) {
+                bool full = true;
+#pragma unroll
+                for (int a = 0; a < 5; ++a) full &= mp_slot_read(bc + a, bv[a]);
+                if (full || mp_poll_give_up(polls, spin_limit, err)) break;
             }
-            mp_vmem_drain_acquire();
-            for (int a = 0; a < 4; ++a) beta_s[a] = mp_load_coherent(coeff_all + date * 4 + a);
-            valid_s = __hip_atomic_load(valid_all + date, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int a = 0; a < 4; ++a) beta_s[a] = bv[a];
+            valid_s = bv[4] != 0.0;
         }
         // prefetch the next date's prices while the block waits
         R sn[J];
@@ -1688,7 +1797,7 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
             v[3] += 1.0;
         if (ex_out) ex_out[p] = (uint16_t)exd[j];
     }
-    if (!publish_and_ticket<4>(v, fin_partial + (uint64_t)blockIdx.x * 4, &tickets[n_reg])) return;
+    if (!publish_and_ticket<4>(v, fin_partial + (uint64_t)blockIdx.x * 4, ticket)) return;
     reduce_partials<4>(fin_partial, gridDim.x, 4, tot);
     if (threadIdx.x == 0) {
         int rs = 0;
@@ -2044,15 +2153,28 @@ int lsm_fused_blocks_per_cu(int precision, bool normals, int J) {
     return nb;
 }
 
+__global__ __launch_bounds__(256) void slot_fill_kernel(unsigned long long* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = MP_SLOT_EMPTY;
+}
+
+hipError_t launch_slot_fill(double* slots, uint64_t n, hipStream_t s) {
+    const uint64_t blocks = std::min<uint64_t>(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(slot_fill_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)), dim3(256), 0, s,
+                       reinterpret_cast<unsigned long long*>(slots), n);
+    return hipGetLastError();
+}
+
 hipError_t launch_lsm_fused(const mp_lsm_params* p, const double* normals, void* paths,
                             const int* reg_dates, int n_reg, double* coeff_all, int* valid_all,
-                            double* gram_partial, double* fin_partial, unsigned int* tickets,
-                            unsigned int* flags, uint16_t* ex, double* out, unsigned int* err,
+                            double* slots, double* gslots, double* bcast, double* fin_partial,
+                            unsigned int* ticket, uint16_t* ex, double* out, unsigned int* err,
                             unsigned int spin_limit, int J, int grid, int precision, hipStream_t s) {
     const void* f = lsm_fused_select(precision, normals != nullptr, J);
     if (!f || grid < 1 || grid > MP_LSM_MAX_BLOCKS) return hipErrorInvalidValue;
-    void* args[] = {&p, &normals, &paths, &reg_dates, &n_reg, &coeff_all, &valid_all, &gram_partial,
-                    &fin_partial, &tickets, &flags, &ex, &out, &err, &spin_limit};
+    void* args[] = {&p, &normals, &paths, &reg_dates, &n_reg, &coeff_all, &valid_all, &slots,
+                    &gslots, &bcast, &fin_partial, &ticket, &ex, &out, &err, &spin_limit};
     return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, s);
 }
 

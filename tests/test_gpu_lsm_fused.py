@@ -1,7 +1,8 @@
 """One-launch Longstaff-Schwartz pricing against the per-date launches.
 
 The one-launch kernel keeps each thread's cash flows in registers and meets
-the other blocks at a bounded device-wide wait per exercise date; the
+the other blocks through bounded waits on write-once slots per exercise
+date; the
 per-date path (one launch per date) is pinned to the oracle elsewhere.  The
 two group the FP64 Gram sums differently, so the bar is: the same number of
 regression dates, prices within 1e-10 relative, standard errors within
@@ -50,7 +51,8 @@ def test_one_launch_equals_per_date_launches(engine, per_date_engine, kind, n, m
     launches_a = engine.last_timing()["main_launches"]
     b = per_date_engine.price(req)
     launches_b = per_date_engine.last_timing()["main_launches"]
-    assert launches_a == 2 and launches_b >= 3
+    # one launch plus two fills, against a launch per date
+    assert launches_a == 3 and launches_b > 3
     assert a.regression_steps == b.regression_steps
     assert abs(a.price - b.price) <= 1e-10 * b.price
     assert abs(a.stderr - b.stderr) <= 1e-8 * b.stderr
@@ -66,7 +68,7 @@ def test_one_launch_exercise_dates_match_on_identical_normals(engine, per_date_e
     z = np.random.default_rng(11).standard_normal((m, n))
     req = make_request("american_put", n_paths=n, n_steps=m, precision=64, seed=1, **LSM)
     a, coeff_a, ex_a = engine.lsm_from_normals(req, z)
-    assert engine.last_timing()["main_launches"] == 2
+    assert engine.last_timing()["main_launches"] == 3
     b, coeff_b, ex_b = per_date_engine.lsm_from_normals(req, z)
     assert a.regression_steps == b.regression_steps == m - 1
     assert abs(a.price - b.price) <= 1e-10 * b.price
