@@ -1412,7 +1412,7 @@ __device__ __forceinline__ void block_sum_to_slots(double (&v)[NV], double* dst,
     const int wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = mp_wave_sum(v[i]);
+    for (int i = 0; i < NV; ++i) v[i] = mp_wave_sum_dpp(v[i]);
     if (lane == 0) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) red[wave][i] = v[i];
