@@ -392,7 +392,9 @@ int mp_hip_mlmc_level_from_normals(mp_ctx* ctx, const mp_mlmc_config* cfg, uint3
 
 /* Price one request and also return, per path, the terminal value (basket
  * value for a basket) and the undiscounted payoff for strike 0, from the
- * engine's own draws (Philox or Sobol).  Same kernels as mp_hip_price. */
+ * engine's own draws (Philox or Sobol).  Same kernels as mp_hip_price.  An
+ * antithetic request fills the first ceil(n/2) entries (the first path of
+ * each pair) and zeroes the rest. */
 int mp_hip_price_with_paths(mp_ctx* ctx, const mp_request* req, mp_result* out,
                             double* terminal_out, double* payoff_out);
 

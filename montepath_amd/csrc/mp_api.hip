@@ -145,6 +145,9 @@ struct mp_ctx {
     // once; MONTEPATH_LSM_MULTI_LAUNCH=1 at open selects the per-date launches
     bool lsm_fused = true;
     int lsm_blocks_per_cu[2][2][5] = {};  // [fp64][normals][log2 J], 0 = not yet queried
+    // FP32 Sobol-bridge requests on 16 * 2^J steps keep the bridge in
+    // registers; MONTEPATH_QMC_STACK_BRIDGE=1 at open selects the LDS stack
+    bool qmc_tree = true;
     dbuf draws;
     dbuf sampler;
 };
@@ -841,6 +844,10 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     if (P.bridge) bridge_plan((uint64_t)d.n_steps, r.t, ops, n_slots);
     qp.n_ops = (int)ops.size();
     qp.n_slots = n_slots;
+    // FP32 bridges on 16 * 2^J steps run in registers when the Sobol tables
+    // sit in LDS: no LDS stack, two 1024-thread blocks per CU
+    const int tree = P.bridge && precision == 32 && c->qmc_tree ? mp::qmc_tree_levels((uint64_t)d.n_steps) : -1;
+    qp.tree_levels = -1;
     // block size: largest power of two <= 1024 whose LDS fits
     // block size: largest power of two <= 1024 whose LDS (plus 2 KB of static
     // reduction space) fits; the Sobol tables move to global memory if the
@@ -857,6 +864,11 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
             block /= 2;
         if (mp::qmc_lds_bytes(precision, P.factors, n_slots, block, false) + reserve > c->max_lds)
             return fail(MP_E_UNSUPPORTED, "Brownian-bridge stack does not fit in LDS for this step count");
+    }
+    if (nib_lds && tree >= 0) {
+        qp.tree_levels = tree;
+        n_slots = 0;
+        block = 1024;
     }
     const size_t lds = mp::qmc_lds_bytes(precision, P.factors, n_slots, block, nib_lds) + reserve;
     const int per_cu = std::max<int>(1, (int)(c->max_lds / std::max<size_t>(lds, 1)));
@@ -1301,6 +1313,7 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
     c->name = prop.name;
     c->arch = prop.gcnArchName;
     if (const char* v = std::getenv("MONTEPATH_LSM_MULTI_LAUNCH")) c->lsm_fused = !(v[0] == '1');
+    if (const char* v = std::getenv("MONTEPATH_QMC_STACK_BRIDGE")) c->qmc_tree = !(v[0] == '1');
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     sobol_directions(c->dirs);
@@ -1577,15 +1590,24 @@ MP_EXPORT int mp_hip_price_with_paths(mp_ctx* c, const mp_request* req, mp_resul
     if (rc != MP_OK) return rc;
     switch (P.kind) {
         case KIND_QMC:
-            return run_qmc(c, *req, P, P.precision, out, terminal_out, payoff_out);
+            rc = run_qmc(c, *req, P, P.precision, out, terminal_out, payoff_out);
+            break;
         case KIND_BASKET:
-            return run_basket(c, *req, P, P.precision, nullptr, out, terminal_out, payoff_out);
+            rc = run_basket(c, *req, P, P.precision, nullptr, out, terminal_out, payoff_out);
+            break;
         case KIND_LSM:
             return fail(MP_E_INCOMPATIBLE_INPUT, "LSM per-path output: use mp_hip_lsm_from_normals");
         default:
-            return run_tiles_single(c, P, P.precision, out, nullptr, nullptr, nullptr, terminal_out,
-                                    payoff_out);
+            rc = run_tiles_single(c, P, P.precision, out, nullptr, nullptr, nullptr, terminal_out,
+                                  payoff_out);
     }
+    // antithetic requests price ceil(n/2) pairs: entries past the pairs are zero
+    if (rc == MP_OK && P.d.n_paths < req->n_paths) {
+        const size_t tail = (size_t)(req->n_paths - P.d.n_paths);
+        std::memset(terminal_out + P.d.n_paths, 0, tail * sizeof(double));
+        std::memset(payoff_out + P.d.n_paths, 0, tail * sizeof(double));
+    }
+    return rc;
 }
 
 // ===========================================================================

@@ -32,6 +32,7 @@ struct mp_sampler_dim {
 // LSM kernels run at most this many blocks: the last block reads every
 // block's partial sums in one round trip (two rows per thread).
 #define MP_LSM_MAX_BLOCKS 1024
+#define MP_TREE_MAX_LEVELS 5
 
 enum mp_model { MP_MODEL_GBM = 0, MP_MODEL_HESTON = 1, MP_MODEL_MERTON = 2 };
 enum mp_payoff { MP_PAY_EUROPEAN = 0, MP_PAY_ASIAN = 1, MP_PAY_DOWN_AND_OUT = 2, MP_PAY_LOOKBACK = 3,
@@ -92,6 +93,8 @@ struct mp_qmc_params {
     int32_t bridge;       // 0: dims in step order, 1: Brownian bridge
     int32_t n_ops;
     int32_t n_slots;
+    int32_t tree_levels;  // J when the bridge runs in registers (n_steps = 16 * 2^J), else -1
+    int32_t pad;
     double sqrt_t;
     double inv_sqrt_dt;
 };
@@ -141,6 +144,9 @@ hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, i
                       double* terminal_out, double* payoff_out, int precision, int grid, int block,
                       bool nib_lds, hipStream_t s);
 size_t qmc_lds_bytes(int precision, int factors, int n_slots, int block, bool nib_lds);
+// J with n_steps = 16 * 2^J (J <= MP_TREE_MAX_LEVELS), else -1: the step
+// counts whose Brownian bridge runs in registers (no LDS stack)
+int qmc_tree_levels(uint64_t n_steps);
 // structured: Halton / Latin hypercube draws; technique: MP_TECH_*
 hipError_t launch_basket(const mp_dreq* req, const mp_sampler_dim* sampler, int n_assets,
                          bool structured, int technique, const double* chol, const double* drift,

@@ -808,13 +808,14 @@ __device__ __forceinline__ R structured_draw(const mp::tile_launch& L, const mp_
 
 // PAY / SCH >= 0 fix the payoff / Heston scheme at compile time (the
 // request's fields must then hold those values); -1 reads them at run time.
-template <class R, int F, bool NIB_LDS, int PAY, int SCH>
-__global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ reqp,
-                                                  mp_qmc_params qp,
-                                                  const uint32_t* __restrict__ nib_g,
-                                                  const mp_bbop* __restrict__ plan,
-                                                  double* partial, double* terminal_out,
-                                                  double* payoff_out) {
+// TREE (FP32, step counts 16 * 2^J, J = qp.tree_levels <= MP_TREE_MAX_LEVELS): the
+// bridge runs in registers -- see the TREE branch -- and the LDS holds only
+// the Sobol tables.
+template <class R, int F, bool NIB_LDS, int PAY, int SCH, bool TREE>
+__device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const mp_qmc_params& qp,
+                                         const uint32_t* __restrict__ nib_g,
+                                         const mp_bbop* __restrict__ plan, double* partial,
+                                         double* terminal_out, double* payoff_out) {
     extern __shared__ uint32_t smem[];
     const uint32_t* nib = nib_g;
     R* stack = reinterpret_cast<R*>(smem);
@@ -892,6 +893,102 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
                         emit(z0 * z_to_step, z1 * z_to_step);
                     else
                         emit(z0, z1);
+                }
+            }
+        } else if constexpr (TREE) {
+            // Register bridge for n_steps = 16 * 2^J.  The reference's plan on a
+            // power-of-two grid is a complete binary tree: J outer levels above
+            // 2^J subtrees of 16 steps.  The subtrees are visited in time order;
+            // before subtree s the outer intervals are split from level
+            // J - ctz(s) (level 0 for s = 0) down to level J, keeping each
+            // split's right half in the register pending[level]; a subtree's
+            // 15 splits are static code.  Splits are consumed in the plan's
+            // preorder (op k uses dimensions (k+1)F + f, the plan's weights),
+            // with the stack kernel's arithmetic, so results are identical.
+            const int J = qp.tree_levels;
+            int k = 0, dims = 0, o = 0;
+            auto split = [&](const R (&d)[F], R (&l)[F], R (&r)[F]) {
+                if (dims == 256) {  // next 256-dimension offset
+                    ++o;
+                    cur.set_offset(o);
+                    dims = 0;
+                }
+                const mp_bbop& op = plan[k];
+                R rw, sd;
+                if constexpr (sizeof(R) == 4) {
+                    rw = op.rwf;
+                    sd = op.sdf;
+                } else {
+                    rw = op.rw;
+                    sd = op.sd;
+                }
+#pragma unroll
+                for (int f = 0; f < F; ++f) {
+                    const R z = cur.normal_rel((uint32_t)(dims + f), f);
+                    l[f] = rw * d[f] + sd * z;
+                    r[f] = d[f] - l[f];
+                }
+                cur.advance(F);
+                dims += F;
+                ++k;
+            };
+            auto leaf = [&](const R (&d)[F]) {
+                if constexpr (kInc)
+                    emit(d[0], F == 2 ? d[F - 1] : R(0));
+                else
+                    emit(d[0] * to_step, F == 2 ? d[F - 1] * to_step : R(0));
+            };
+            // two leaves from an interval of two steps
+            auto pair2 = [&](const R (&d)[F]) {
+                R a[F], b[F];
+                split(d, a, b);
+                leaf(a);
+                leaf(b);
+            };
+            auto quad4 = [&](const R (&d)[F]) {
+                R a[F], b[F];
+                split(d, a, b);
+                pair2(a);
+                pair2(b);
+            };
+            R cur_d[F], pending[MP_TREE_MAX_LEVELS + 1][F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) cur_d[f] = sqrt_t * cur.normal_rel((uint32_t)f, f);
+            cur.advance(F);
+            dims = F;
+            for (int sub = 0; sub < (1 << J); ++sub) {
+                const int lr = sub == 0 ? 0 : J - __builtin_ctz((unsigned)sub);
+#pragma unroll
+                for (int l = 1; l <= MP_TREE_MAX_LEVELS; ++l)
+                    if (l == lr) {
+#pragma unroll
+                        for (int f = 0; f < F; ++f) cur_d[f] = pending[l][f];
+                    }
+#pragma unroll
+                for (int l = 0; l < MP_TREE_MAX_LEVELS; ++l)
+                    if (l >= lr && l < J) {
+                        R a[F];
+                        split(cur_d, a, pending[l + 1]);
+#pragma unroll
+                        for (int f = 0; f < F; ++f) cur_d[f] = a[f];
+                    }
+                // the 16-step subtree: midpoints 8, 4, 2, 1, 3, 6, 5, 7, 12, ...
+                // (halves and quarters as loops, one copy of the quarter code)
+                R a8[F], b8[F];
+                split(cur_d, a8, b8);
+#pragma unroll 1
+                for (int h = 0; h < 2; ++h) {
+                    R d8[F], a4[F], b4[F];
+#pragma unroll
+                    for (int f = 0; f < F; ++f) d8[f] = h ? b8[f] : a8[f];
+                    split(d8, a4, b4);
+#pragma unroll 1
+                    for (int q = 0; q < 2; ++q) {
+                        R d4[F];
+#pragma unroll
+                        for (int f = 0; f < F; ++f) d4[f] = q ? b4[f] : a4[f];
+                        quad4(d4);
+                    }
                 }
             }
         } else {
@@ -975,6 +1072,25 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
         }
         block_sum_store<2 * MP_NS_MAX>(v, partial + (uint64_t)tile * MP_PARTIAL_STRIDE);
     }
+}
+
+template <class R, int F, bool NIB_LDS, int PAY, int SCH>
+__global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ reqp, mp_qmc_params qp,
+                                                  const uint32_t* __restrict__ nib_g,
+                                                  const mp_bbop* __restrict__ plan, double* partial,
+                                                  double* terminal_out, double* payoff_out) {
+    qmc_body<R, F, NIB_LDS, PAY, SCH, false>(reqp, qp, nib_g, plan, partial, terminal_out, payoff_out);
+}
+
+// FP32 register bridge: at most 64 VGPRs, so two 1024-thread blocks (the
+// LDS then holds two copies of the Sobol tables) share a CU -- eight waves
+// per SIMD.
+template <int F, int PAY, int SCH>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+void qmc_tree_kernel(const mp_dreq* __restrict__ reqp, mp_qmc_params qp,
+                     const uint32_t* __restrict__ nib_g, const mp_bbop* __restrict__ plan,
+                     double* partial, double* terminal_out, double* payoff_out) {
+    qmc_body<float, F, true, PAY, SCH, true>(reqp, qp, nib_g, plan, partial, terminal_out, payoff_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1948,6 +2064,12 @@ hipError_t launch_finalize(const mp_dreq* reqs, int n_reqs, const double* partia
     return hipGetLastError();
 }
 
+int qmc_tree_levels(uint64_t n_steps) {
+    for (int j = 0; j <= MP_TREE_MAX_LEVELS; ++j)
+        if (n_steps == (16ull << j)) return j;
+    return -1;
+}
+
 size_t qmc_lds_bytes(int precision, int factors, int n_slots, int block, bool nib_lds) {
     const size_t rb = precision == 64 ? 8 : 4;
     return (nib_lds ? (size_t)kNibWords * 4 : 0) + (size_t)n_slots * factors * block * rb;
@@ -1957,6 +2079,13 @@ template <class R, int F, int PAY, int SCH>
 static void qmc_launch_nl(bool nib_lds, dim3 g, dim3 b, size_t lds, hipStream_t s,
                           const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
                           const mp_bbop* plan, double* partial, double* term, double* pay) {
+    if constexpr (sizeof(R) == 4) {
+        if (nib_lds && qp.bridge && qp.tree_levels >= 0) {
+            hipLaunchKernelGGL((qmc_tree_kernel<F, PAY, SCH>), g, b, lds, s, req, qp, nib, plan, partial,
+                               term, pay);
+            return;
+        }
+    }
     if (nib_lds)
         hipLaunchKernelGGL((qmc_kernel<R, F, true, PAY, SCH>), g, b, lds, s, req, qp, nib, plan,
                            partial, term, pay);
