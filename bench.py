@@ -349,7 +349,7 @@ def main() -> int:
                      "traffic_source": f"profiles/{TRAFFIC_FILE}" if traffic else None,
                      "valu_issue": traffic.get("valu_issue_frac") if traffic else None,
                      "valu_insts_per_path_step": traffic.get("valu_insts_per_path_step") if traffic else None,
-                     "kernel": "qmc_kernel<float,2,true>", "kernel_ms": k_ms,
+                     "kernel": "qmc_tree_kernel<2,0,1>", "kernel_ms": k_ms,
                      "flop_per_path_step": FLOP_PER_PATH_STEP,
                      "note": "FP32 vector (VALU) work priced at the packed-FP32 peak 157.3 TFLOP/s "
                              "(78.6 without packed FMA, which this build does not emit); MFMA unused; "

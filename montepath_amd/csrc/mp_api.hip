@@ -845,7 +845,7 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     qp.n_ops = (int)ops.size();
     qp.n_slots = n_slots;
     // FP32 bridges on 16 * 2^J steps run in registers when the Sobol tables
-    // sit in LDS: no LDS stack, two 1024-thread blocks per CU
+    // sit in LDS (no LDS stack)
     const int tree = P.bridge && precision == 32 && c->qmc_tree ? mp::qmc_tree_levels((uint64_t)d.n_steps) : -1;
     qp.tree_levels = -1;
     // block size: largest power of two <= 1024 whose LDS fits

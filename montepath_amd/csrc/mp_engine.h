@@ -32,7 +32,7 @@ struct mp_sampler_dim {
 // LSM kernels run at most this many blocks: the last block reads every
 // block's partial sums in one round trip (two rows per thread).
 #define MP_LSM_MAX_BLOCKS 1024
-#define MP_TREE_MAX_LEVELS 5
+#define MP_TREE_MAX_LEVELS 4
 
 enum mp_model { MP_MODEL_GBM = 0, MP_MODEL_HESTON = 1, MP_MODEL_MERTON = 2 };
 enum mp_payoff { MP_PAY_EUROPEAN = 0, MP_PAY_ASIAN = 1, MP_PAY_DOWN_AND_OUT = 2, MP_PAY_LOOKBACK = 3,

@@ -43,24 +43,35 @@ struct model_params {
     int payoff, scheme, model;
 };
 
+// A request-wide parameter in the path precision.  FP32 values are moved to
+// a scalar register (the double-to-float conversion is a vector op), so they
+// hold no vector register for the kernel's lifetime; the value is unchanged.
+template <class R>
+__device__ __forceinline__ R mp_param(double x) {
+    if constexpr (sizeof(R) == 4)
+        return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)x)));
+    else
+        return x;
+}
+
 template <class R>
 __device__ __forceinline__ model_params<R> load_params(const mp_dreq& q) {
     model_params<R> m;
-    m.drift_dt = R(q.drift_dt);
-    m.vol_dt = R(q.vol_dt);
-    m.xb = R(q.log_barrier_rel);
-    m.kappa_dt = R(q.kappa * q.dt);
-    m.theta = R(q.theta);
-    m.xi_sqdt = R(q.xi * q.sqrt_dt);
-    m.rho = R(q.rho);
-    m.rho_perp = R(q.rho_perp);
-    m.rate_dt = R(q.rate * q.dt);
-    m.half_dt = R(0.5 * q.dt);
-    m.sqrt_dt = R(q.sqrt_dt);
-    m.sigma = R(q.vol_dt / q.sqrt_dt);
-    m.xi = R(q.xi);
-    m.mu_j = R(q.mu_j);
-    m.sigma_j = R(q.sigma_j);
+    m.drift_dt = mp_param<R>(q.drift_dt);
+    m.vol_dt = mp_param<R>(q.vol_dt);
+    m.xb = mp_param<R>(q.log_barrier_rel);
+    m.kappa_dt = mp_param<R>(q.kappa * q.dt);
+    m.theta = mp_param<R>(q.theta);
+    m.xi_sqdt = mp_param<R>(q.xi * q.sqrt_dt);
+    m.rho = mp_param<R>(q.rho);
+    m.rho_perp = mp_param<R>(q.rho_perp);
+    m.rate_dt = mp_param<R>(q.rate * q.dt);
+    m.half_dt = mp_param<R>(0.5 * q.dt);
+    m.sqrt_dt = mp_param<R>(q.sqrt_dt);
+    m.sigma = mp_param<R>(q.vol_dt / q.sqrt_dt);
+    m.xi = mp_param<R>(q.xi);
+    m.mu_j = mp_param<R>(q.mu_j);
+    m.sigma_j = mp_param<R>(q.sigma_j);
     m.payoff = q.payoff;
     m.scheme = q.scheme;
     m.model = q.model;
@@ -834,12 +845,12 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
     const int tech = PAY < 0 ? q.technique : MP_TECH_STANDARD;
     const int n_steps = q.n_steps;
     const int ns = q.n_strikes;
-    const R s0 = R(q.s0);
+    const R s0 = mp_param<R>(q.s0);
     R kk[MP_NS_MAX];
 #pragma unroll
-    for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = R(q.strikes[j]);
-    const R sqrt_t = R(qp.sqrt_t);
-    const R inv_sqrt_dt = R(qp.inv_sqrt_dt);
+    for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = mp_param<R>(q.strikes[j]);
+    const R sqrt_t = mp_param<R>(qp.sqrt_t);
+    const R inv_sqrt_dt = mp_param<R>(qp.inv_sqrt_dt);
     const int bd = blockDim.x;
     const int tid = threadIdx.x;
 
@@ -951,7 +962,7 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                 pair2(a);
                 pair2(b);
             };
-            R cur_d[F], pending[MP_TREE_MAX_LEVELS + 1][F];
+            R cur_d[F], pending[MP_TREE_MAX_LEVELS][F];  // pending[l - 1]: level l
 #pragma unroll
             for (int f = 0; f < F; ++f) cur_d[f] = sqrt_t * cur.normal_rel((uint32_t)f, f);
             cur.advance(F);
@@ -962,13 +973,13 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                 for (int l = 1; l <= MP_TREE_MAX_LEVELS; ++l)
                     if (l == lr) {
 #pragma unroll
-                        for (int f = 0; f < F; ++f) cur_d[f] = pending[l][f];
+                        for (int f = 0; f < F; ++f) cur_d[f] = pending[l - 1][f];
                     }
 #pragma unroll
                 for (int l = 0; l < MP_TREE_MAX_LEVELS; ++l)
                     if (l >= lr && l < J) {
                         R a[F];
-                        split(cur_d, a, pending[l + 1]);
+                        split(cur_d, a, pending[l]);
 #pragma unroll
                         for (int f = 0; f < F; ++f) cur_d[f] = a[f];
                     }
@@ -1082,12 +1093,12 @@ __global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ r
     qmc_body<R, F, NIB_LDS, PAY, SCH, false>(reqp, qp, nib_g, plan, partial, terminal_out, payoff_out);
 }
 
-// FP32 register bridge: at most 64 VGPRs, so two 1024-thread blocks (the
-// LDS then holds two copies of the Sobol tables) share a CU -- eight waves
-// per SIMD.
+// FP32 register bridge.  It needs ~80 VGPRs, so one 1024-thread block per
+// CU (four waves per SIMD) as with the LDS stack; a 64-VGPR build, which
+// fits two blocks per CU, spilled to scratch and ran 3% slower.  The gain
+// over the stack kernel is instructions: no stack traffic or addressing.
 template <int F, int PAY, int SCH>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
-void qmc_tree_kernel(const mp_dreq* __restrict__ reqp, mp_qmc_params qp,
+__global__ __launch_bounds__(1024) void qmc_tree_kernel(const mp_dreq* __restrict__ reqp, mp_qmc_params qp,
                      const uint32_t* __restrict__ nib_g, const mp_bbop* __restrict__ plan,
                      double* partial, double* terminal_out, double* payoff_out) {
     qmc_body<float, F, true, PAY, SCH, true>(reqp, qp, nib_g, plan, partial, terminal_out, payoff_out);
