@@ -142,8 +142,11 @@ struct mp_ctx {
     dbuf lsm_paths, lsm_cash, lsm_ex, lsm_coeff, lsm_valid, lsm_gram, lsm_par, lsm_cnt,
         lsm_part, lsm_out, lsm_dates, plain_pay;
     // one-launch LSM pricing when the request's grid fits on the device at
-    // once; MONTEPATH_LSM_MULTI_LAUNCH=1 at open selects the per-date launches
+    // once; MONTEPATH_LSM_MULTI_LAUNCH at open selects per-date launches:
+    // 1 matched to the one-launch pricing (same bits), 2 the plain grid-stride
+    // passes (the only per-date form for requests too large for one launch)
     bool lsm_fused = true;
+    bool lsm_matched = true;
     int lsm_blocks_per_cu[2][2][5] = {};  // [fp64][normals][log2 J], 0 = not yet queried
     // FP32 Sobol-bridge requests on 16 * 2^J steps keep the bridge in
     // registers; MONTEPATH_QMC_STACK_BRIDGE=1 at open selects the LDS stack
@@ -1038,9 +1041,12 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     std::vector<int> reg_dates;
     for (int i = ns - 2; i >= 0; --i) reg_dates.push_back((int)P.schedule[i]);
     const int n_reg = (int)reg_dates.size();
-    // one launch: J paths per thread, the fewest that keep every block resident
+    // the one-launch decomposition: J paths per thread, the fewest that keep
+    // every block resident.  It is priced in one launch, or (contexts opened
+    // with per-date launches, e.g. the server's) as matched per-date launches
+    // that reproduce the one-launch pricing bit for bit.
     int fused_j = 0, fused_grid = 0;
-    if (c->lsm_fused) {
+    {
         for (int lj = 0; lj < 5 && !fused_j; ++lj) {
             const int J = 1 << lj;
             const uint64_t g = (n + (uint64_t)block * J - 1) / ((uint64_t)block * J);
@@ -1056,10 +1062,14 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
             }
         }
     }
-    const int grid = fused_j ? fused_grid
-                             : (int)std::max<uint64_t>(1, std::min<uint64_t>({(n + block - 1) / block,
-                                                                              (uint64_t)c->n_cu * 2,
-                                                                              (uint64_t)MP_LSM_MAX_BLOCKS}));
+    const bool matched = fused_j && !c->lsm_fused && c->lsm_matched;
+    if (!c->lsm_fused && !c->lsm_matched) fused_j = 0;
+    if (matched) fused_j = 0;  // per-date launches on the one-launch grid
+    const int grid = (fused_j || matched)
+                         ? fused_grid
+                         : (int)std::max<uint64_t>(1, std::min<uint64_t>({(n + block - 1) / block,
+                                                                          (uint64_t)c->n_cu * 2,
+                                                                          (uint64_t)MP_LSM_MAX_BLOCKS}));
     ENSURE(c->lsm_paths, (size_t)(m - 1) * n * rb);
     ENSURE(c->lsm_cash, n * 8);
     ENSURE(c->lsm_ex, n * 2);
@@ -1069,11 +1079,12 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     // slots [n_reg][16][grid], group totals [n_reg][16][32] and coefficient
     // broadcasts [n_reg][8]
     const size_t n_slots = fused_j ? (size_t)n_reg * (16 * grid + 16 * 32 + 8) : 0;
-    ENSURE(c->lsm_gram, std::max((size_t)grid * 16, n_slots) * 8);
+    // matched per-date launches: block sums [16][grid], then group totals [16][32]
+    ENSURE(c->lsm_gram, std::max((size_t)grid * 16 + 16 * 32, n_slots) * 8);
     ENSURE(c->lsm_par, blob_bytes);
-    // counters: per-date launches use tickets [m + 2]; one launch uses the
-    // final ticket [0] and the error flag [1]
-    ENSURE(c->lsm_cnt, (size_t)(m + 2) * 4);
+    // counters: per-date launches use tickets [m + 2] and (matched) the error
+    // flag [m + 2]; one launch uses the final ticket [0] and the error flag [1]
+    ENSURE(c->lsm_cnt, (size_t)(m + 3) * 4);
     ENSURE(c->lsm_part, (size_t)grid * 4 * 8);
     ENSURE(c->lsm_out, 8 * 8);
     ENSURE(c->lsm_dates, (size_t)std::max(n_reg, 1) * 4);
@@ -1087,9 +1098,16 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     const mp_lsm_params* dp = c->lsm_par.as<mp_lsm_params>();
     unsigned int* cnt = c->lsm_cnt.as<unsigned int>();
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(mp::launch_lsm_fill(cnt, m + 2, c->lsm_valid.as<int>(), c->lsm_coeff.as<double>(), m + 1,
+    HIPCHK(mp::launch_lsm_fill(cnt, m + 3, c->lsm_valid.as<int>(), c->lsm_coeff.as<double>(), m + 1,
                                c->stream));
     int launches = 2;
+    // One-launch pricings need every block of their grid resident at once:
+    // two of them overlapping on one device could each hold part of it and
+    // wait (boundedly) on the rest, so they are serialized per device
+    // within the process, from launch to completion.
+    static std::mutex fused_mu[64];
+    std::unique_lock<std::mutex> fused_lock;
+    if (fused_j) fused_lock = std::unique_lock<std::mutex>(fused_mu[c->device & 63]);
     if (fused_j) {
         double* slots = c->lsm_gram.as<double>();
         if (n_slots) HIPCHK(mp::launch_slot_fill(slots, n_slots, c->stream));
@@ -1113,14 +1131,20 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
         const int date = reg_dates[i];
         HIPCHK(mp::launch_lsm_step(dp, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
                                    c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, date,
-                                   c->lsm_gram.as<double>(), cnt, precision, grid, block, c->stream));
-        prev = date;
+                                   c->lsm_gram.as<double>(), cnt, precision, grid, block, matched, c->stream));
         ++launches;
+        if (matched) {
+            HIPCHK(mp::launch_lsm_group(dp, c->lsm_gram.as<double>(), grid, c->lsm_gram.as<double>() + 16 * grid,
+                                        cnt + date, c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), date,
+                                        cnt + (m + 2), c->stream));
+            ++launches;
+        }
+        prev = date;
     }
     if (prev >= 1) {
         HIPCHK(mp::launch_lsm_step(dp, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
                                    c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, 0,
-                                   c->lsm_gram.as<double>(), cnt, precision, grid, block, c->stream));
+                                   c->lsm_gram.as<double>(), cnt, precision, grid, block, matched, c->stream));
         ++launches;
     }
     HIPCHK(mp::launch_lsm_final(dp, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), c->lsm_valid.as<int>(),
@@ -1132,6 +1156,7 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     unsigned int wait_err = 0;
     HIPCHK(hipMemcpyAsync(res, c->lsm_out.p, 5 * 8, hipMemcpyDeviceToHost, c->stream));
     if (fused_j) HIPCHK(hipMemcpyAsync(&wait_err, cnt + 1, 4, hipMemcpyDeviceToHost, c->stream));
+    if (matched) HIPCHK(hipMemcpyAsync(&wait_err, cnt + (m + 2), 4, hipMemcpyDeviceToHost, c->stream));
     std::vector<double> coeff;
     if (coeff_host) {
         coeff.resize((size_t)(m + 1) * 4);
@@ -1312,7 +1337,10 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
     c->max_lds = prop.sharedMemPerBlock;
     c->name = prop.name;
     c->arch = prop.gcnArchName;
-    if (const char* v = std::getenv("MONTEPATH_LSM_MULTI_LAUNCH")) c->lsm_fused = !(v[0] == '1');
+    if (const char* v = std::getenv("MONTEPATH_LSM_MULTI_LAUNCH")) {
+        c->lsm_fused = !(v[0] == '1' || v[0] == '2');
+        c->lsm_matched = v[0] != '2';
+    }
     if (const char* v = std::getenv("MONTEPATH_QMC_STACK_BRIDGE")) c->qmc_tree = !(v[0] == '1');
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -2396,8 +2424,6 @@ struct mp_server {
     uint64_t launches = 0;
     bool stop = false;
     std::vector<std::thread> workers;
-    mp_ctx* lsm_ctx = nullptr;  // context for requests outside the work queue
-    std::mutex lsm_mu;
 };
 
 namespace {
@@ -2410,6 +2436,10 @@ void server_worker(mp_server* s, int slot) {
     (void)hipSetDevice(s->ctx->device);
     mp_ctx* own = nullptr;
     if (mp_hip_open(s->ctx->device, &own) != MP_OK) return;
+    // other workers' launches share the device, so a one-launch LSM pricing
+    // (which needs its whole grid resident) could wait on them: price LSM
+    // requests with the matched per-date launches, which give the same bits
+    own->lsm_fused = false;
     for (;;) {
         std::vector<uint64_t> take;
         uint64_t bytes = 0;

@@ -162,10 +162,17 @@ hipError_t launch_lsm_fill(unsigned int* counters, int n_counters, int* valid, d
 hipError_t launch_lsm_paths(const mp_lsm_params* p, const double* normals, void* paths,
                             double* cash, uint16_t* ex, int precision, int grid, int block,
                             hipStream_t s);
+// matched: block sums to gram_partial[value][block] for lsm_group (the
+// one-launch kernel's reduction order); else the last block reduces and solves.
 hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* cash,
                            uint16_t* ex, double* coeff_all, int* valid_all, int step_update,
                            int step_gram, double* gram_partial, unsigned int* counters,
-                           int precision, int grid, int block, hipStream_t s);
+                           int precision, int grid, int block, bool matched, hipStream_t s);
+// One date's reduction of matched block sums: ceil(nb / 32) blocks; gpart
+// holds 16 x 32 doubles of group totals.
+hipError_t launch_lsm_group(const mp_lsm_params* p, const double* partial, int nb, double* gpart,
+                            unsigned int* counter, double* coeff_all, int* valid_all, int step_gram,
+                            unsigned int* err, hipStream_t s);
 // Whole pricing in one launch (paths per thread J in {1,2,4,8,16}, grid
 // blocks of 256 threads, all resident at once).  slots: n_reg * 16 * grid
 // doubles, gslots: n_reg * 16 * 32 and bcast: n_reg * 8, all filled by

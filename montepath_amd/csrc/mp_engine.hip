@@ -1390,6 +1390,25 @@ __device__ bool lsm_solve4(double (&m)[4][4], double (&rhs)[4], int dim) {
     return true;
 }
 
+// Regression coefficients from the reduced sums tot (10 Gram entries, 4
+// right-hand sides, the in-the-money count), one lane: the reference's
+// skip rules (count < coefficients, pivot <= 1e-12).  Returns validity.
+__device__ __forceinline__ int lsm_coefficients(const double* tot, int nc, double (&c)[4]) {
+    double mm[4][4], rhs[4];
+    int e = 0;
+    for (int a = 0; a < 4; ++a)
+        for (int b = a; b < 4; ++b) {
+            mm[a][b] = tot[e];
+            mm[b][a] = tot[e];
+            ++e;
+        }
+    for (int a = 0; a < 4; ++a) rhs[a] = tot[10 + a];
+    int ok = 0;
+    if (tot[14] >= (double)nc) ok = lsm_solve4(mm, rhs, nc) ? 1 : 0;
+    for (int a = 0; a < 4; ++a) c[a] = (ok && a < nc) ? rhs[a] : 0.0;
+    return ok;
+}
+
 // Block sum of NV values published for other blocks (coherent stores),
 // then a ticket; true in the block that arrives last, which may then read
 // every block's partials with mp_load_coherent.
@@ -1557,7 +1576,13 @@ __device__ __forceinline__ void block_sum_to_slots(double (&v)[NV], double* dst,
 // step_gram (if >= 1) over in-the-money paths; the last block to finish
 // reduces all blocks' sums in fixed order and solves for the coefficients.
 // cash holds cash flows discounted to t0; growth[s] = disc_step^-s.
-template <class R>
+// MATCHED (launched with lsm_fused_kernel's grid): the block sums are formed
+// as lsm_fused_kernel forms them (DPP wave sums) and stored to
+// gram_partial[value][block]; lsm_group_kernel then reduces them in that
+// kernel's order, so the pricing equals the one-launch pricing bit for bit.
+// Each thread already visits its paths p0, p0 + T, ... in the one-launch
+// kernel's order with the same per-path arithmetic.
+template <class R, bool MATCHED>
 __global__ __launch_bounds__(256) void lsm_step_kernel(const mp_lsm_params* __restrict__ Pp,
                                                       const R* __restrict__ paths,
                                                       double* cash, uint16_t* ex,
@@ -1631,6 +1656,10 @@ __global__ __launch_bounds__(256) void lsm_step_kernel(const mp_lsm_params* __re
         }
     }
     if (step_gram < 1) return;
+    if constexpr (MATCHED) {
+        block_sum_to_slots<15>(acc, gram_partial + blockIdx.x, (uint64_t)gridDim.x);
+        return;
+    }
     if (!publish_and_ticket<15>(acc, gram_partial + (uint64_t)blockIdx.x * 16, &counters[step_gram]))
         return;
     __shared__ double tot[15];
@@ -1650,6 +1679,43 @@ __global__ __launch_bounds__(256) void lsm_step_kernel(const mp_lsm_params* __re
         for (int i = 0; i < 4; ++i) coeff_all[step_gram * 4 + i] = (ok && i < nc) ? rhs[i] : 0.0;
         valid_all[step_gram] = ok;
         counters[step_gram] = 0u;  // ready for the next pricing
+    }
+}
+
+// Reduction of one date's matched block sums (partial[value][block], nb
+// blocks) in lsm_fused_kernel's order: block g sums the 32 blocks of its
+// group, the last block to finish sums the group totals and solves.
+__global__ __launch_bounds__(256) void lsm_group_kernel(const mp_lsm_params* __restrict__ Pp,
+                                                       const double* partial, int nb,
+                                                       double* gpart, unsigned int* counter,
+                                                       double* coeff_all, int* valid_all,
+                                                       int step_gram, unsigned int* err) {
+    __shared__ double gsum[15], tot[15];
+    const int g = blockIdx.x;
+    const int ng = gridDim.x;
+    reduce_group<15>(partial, (uint64_t)nb, 32 * g, min(32, nb - 32 * g), 1u, err, gsum);
+    if (ng > 1) {
+        if (threadIdx.x < 15) mp_store_coherent(gpart + threadIdx.x * 32 + g, gsum[threadIdx.x]);
+        if (threadIdx.x < MP_WAVE) mp_vmem_drain_release();
+        __shared__ bool is_last;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            is_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                      (unsigned int)ng - 1;
+        __syncthreads();
+        if (!is_last) return;
+        mp_vmem_drain_acquire();
+        reduce_group<15>(gpart, 32, 0, ng, 1u, err, tot);
+    } else if (threadIdx.x < 15) {
+        tot[threadIdx.x] = gsum[threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double c[4];
+        const int ok = lsm_coefficients(tot, Pp->coeffs, c);
+        for (int a = 0; a < 4; ++a) coeff_all[step_gram * 4 + a] = c[a];
+        valid_all[step_gram] = ok;
+        *counter = 0u;  // ready for the next pricing
     }
 }
 
@@ -1834,20 +1900,9 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
             }
             __syncthreads();
             if (threadIdx.x == 0) {
-                double mm[4][4], rhs[4];
-                int e = 0;
-                for (int a = 0; a < 4; ++a)
-                    for (int b = a; b < 4; ++b) {
-                        mm[a][b] = tot[e];
-                        mm[b][a] = tot[e];
-                        ++e;
-                    }
-                for (int a = 0; a < 4; ++a) rhs[a] = tot[10 + a];
-                int ok = 0;
-                if (tot[14] >= (double)nc) ok = lsm_solve4(mm, rhs, nc) ? 1 : 0;
                 double c[4];
+                const int ok = lsm_coefficients(tot, nc, c);
                 for (int a = 0; a < 4; ++a) {
-                    c[a] = (ok && a < nc) ? rhs[a] : 0.0;
                     beta_s[a] = c[a];
                     mp_store_coherent(bc + a, c[a]);
                 }
@@ -2318,19 +2373,41 @@ hipError_t launch_lsm_fused(const mp_lsm_params* p, const double* normals, void*
     return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, s);
 }
 
-hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* cash,
-                           uint16_t* ex, double* coeff_all, int* valid_all, int step_update,
-                           int step_gram, double* gram_partial, unsigned int* counters,
-                           int precision, int grid, int block, hipStream_t s) {
-    if (grid < 1 || grid > MP_LSM_MAX_BLOCKS || block != 256) return hipErrorInvalidValue;
+template <bool MATCHED>
+static void lsm_step_m(const mp_lsm_params* p, const void* paths, double* cash, uint16_t* ex,
+                       double* coeff_all, int* valid_all, int step_update, int step_gram,
+                       double* gram_partial, unsigned int* counters, int precision, int grid,
+                       int block, hipStream_t s) {
     if (precision == 64)
-        hipLaunchKernelGGL((lsm_step_kernel<double>), dim3(grid), dim3(block), 0, s, p,
+        hipLaunchKernelGGL((lsm_step_kernel<double, MATCHED>), dim3(grid), dim3(block), 0, s, p,
                            (const double*)paths, cash, ex, coeff_all, valid_all, step_update,
                            step_gram, gram_partial, counters);
     else
-        hipLaunchKernelGGL((lsm_step_kernel<float>), dim3(grid), dim3(block), 0, s, p,
+        hipLaunchKernelGGL((lsm_step_kernel<float, MATCHED>), dim3(grid), dim3(block), 0, s, p,
                            (const float*)paths, cash, ex, coeff_all, valid_all, step_update,
                            step_gram, gram_partial, counters);
+}
+
+hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* cash,
+                           uint16_t* ex, double* coeff_all, int* valid_all, int step_update,
+                           int step_gram, double* gram_partial, unsigned int* counters,
+                           int precision, int grid, int block, bool matched, hipStream_t s) {
+    if (grid < 1 || grid > MP_LSM_MAX_BLOCKS || block != 256) return hipErrorInvalidValue;
+    if (matched)
+        lsm_step_m<true>(p, paths, cash, ex, coeff_all, valid_all, step_update, step_gram,
+                         gram_partial, counters, precision, grid, block, s);
+    else
+        lsm_step_m<false>(p, paths, cash, ex, coeff_all, valid_all, step_update, step_gram,
+                          gram_partial, counters, precision, grid, block, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_lsm_group(const mp_lsm_params* p, const double* partial, int nb, double* gpart,
+                            unsigned int* counter, double* coeff_all, int* valid_all, int step_gram,
+                            unsigned int* err, hipStream_t s) {
+    if (nb < 1 || nb > MP_LSM_MAX_BLOCKS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(lsm_group_kernel, dim3((nb + 31) / 32), dim3(256), 0, s, p, partial, nb, gpart,
+                       counter, coeff_all, valid_all, step_gram, err);
     return hipGetLastError();
 }
 
