@@ -7,8 +7,9 @@ simulation, FP32 path arithmetic, FP64 compensated payoff sums.  The
 parameters (S0, v0, kappa, theta, xi, rho, r, T) are the published
 precision configuration's.  Prints one JSON line.
 
-Secondary measurements (same line, under "extra"): the 4,096-contract
-batched persistent launch against plain one-launch-per-request FP64 pricing
+Secondary measurements (same line, under "extra"): the reference's own
+CPU-runnable configuration (100,000 x 64 European, FP64) on GPU and CPU, the
+4,096-contract batched persistent launch against plain one-launch-per-request FP64 pricing
 of the same contracts, the 2^20-path 50-date American put (LSM) against the
 HBM roofline, the five-asset 2^22 x 64 basket, and a short Poisson serving
 trace with p50/p99.  `--only <section>` runs one of them alone (for
@@ -174,6 +175,42 @@ def bench_basket(eng, reps: int = 5) -> dict:
             "price": res.price, "stderr": res.stderr}
 
 
+def bench_plain_c1(eng, threads: int, reps: int = 20) -> dict:
+    """The reference's own CPU-runnable configuration: European call under
+    GBM, S0=K=100, r=0.03, sigma=0.2, T=1, 100,000 paths x 64 log-Euler steps,
+    FP64, seed 42, standard technique.  GPU: the engine's FP64 work-queue
+    pricing (Philox draws) and the plain one-launch FP64 kernel; CPU: the
+    reference's stepwise pricer (xorshift64*/Box-Muller, std::thread chunks;
+    runtime/cpu.rs:3228-3262) restated in C, on this host's cores.  Published
+    (Apple M2, all cores): 11.315 ms."""
+    from montepath_amd.engine import make_request
+    from oracle import cpu as O
+
+    n, m = 100_000, 64
+    req = make_request("european_call", n_paths=n, n_steps=m, seed=42, precision=64)
+    eng.price(req)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = eng.price(req)
+    gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    eng.price_plain_fp64(req)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.price_plain_fp64(req)
+    plain_ms = (time.perf_counter() - t0) / reps * 1e3
+    cfg = O.Gbm(n_paths=n, n_steps=m, seed=42, n_threads=threads)
+    O.gbm_price(cfg)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        cpu_price, _ = O.gbm_price(cfg)
+    cpu_ms = (time.perf_counter() - t0) / 3 * 1e3
+    return {"n_paths": n, "n_steps": m, "precision": "fp64", "gpu_ms": gpu_ms,
+            "gpu_path_steps_per_s": n * m / (gpu_ms * 1e-3), "plain_fp64_ms": plain_ms,
+            "cpu_ms": cpu_ms, "cpu_threads": threads, "cpu_path_steps_per_s": n * m / (cpu_ms * 1e-3),
+            "gpu_price": res.price, "gpu_stderr": res.stderr, "cpu_price": cpu_price,
+            "black_scholes": 9.413403383853016, "reference_m2_ms": 11.315}
+
+
 SERVE_KINDS = ("european_call", "arithmetic_asian_call", "heston_european_call",
                "merton_jump_diffusion_call", "american_put")
 
@@ -278,7 +315,7 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip batch and serving")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--only", choices=["batch", "lsm", "basket", "serve"], default=None,
+    ap.add_argument("--only", choices=["c1", "batch", "lsm", "basket", "serve"], default=None,
                     help="run one secondary section by itself (for profiling) and print its JSON")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run three device-to-device copies of this many MiB (the "
@@ -297,7 +334,8 @@ def main() -> int:
     info = eng.device_info()
     log(f"[bench] {info['device_id']} {info['name']} {info['arch']} CUs={info['compute_units']}")
     if args.only:
-        section = {"batch": lambda: bench_batch(eng), "lsm": lambda: bench_lsm(eng),
+        section = {"c1": lambda: bench_plain_c1(eng, 16), "batch": lambda: bench_batch(eng),
+                   "lsm": lambda: bench_lsm(eng),
                    "basket": lambda: bench_basket(eng),
                    "serve": lambda: bench_serve(eng, args.serve_seconds, args.serve_rate)}[args.only]()
         eng.close()
@@ -371,6 +409,9 @@ def main() -> int:
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3e} path-steps/s")
     if not args.no_extra:
         extra = {}
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        extra["plain_european_100k_x_64_fp64"] = bench_plain_c1(eng, max(1, min(threads, 16)))
+        log(f"[bench] c1 {extra['plain_european_100k_x_64_fp64']}")
         extra["batched_4096_contracts"] = bench_batch(eng)
         log(f"[bench] batch {extra['batched_4096_contracts']}")
         extra["lsm_american_put_2p20x50"] = bench_lsm(eng)
@@ -379,6 +420,16 @@ def main() -> int:
         log(f"[bench] basket {extra['basket_5_assets_2p22x64']}")
         extra["serving"] = bench_serve(eng, args.serve_seconds, args.serve_rate)
         log(f"[bench] serving {extra['serving']}")
+        # the stated targets: batched >= 50x the CPU baseline's path-steps/s
+        # and >= 5x plain one-launch-per-request FP64 GPU pricing; p99 < 5 ms
+        # for requests of 10^6 paths or more
+        b, sv = extra["batched_4096_contracts"], extra["serving"]
+        extra["targets"] = {
+            "batched_over_cpu_baseline": (b["path_steps_per_s"] / out["cpu_baseline"]["value"]
+                                          if "cpu_baseline" in out else None),
+            "batched_over_plain_fp64_gpu": b["speedup_vs_plain_fp64"],
+            "p99_ms_1e6plus_paths": sv["p99_ms_1e6plus_paths"],
+        }
         out["extra"] = extra
     eng.close()
     print(json.dumps(out), flush=True)
