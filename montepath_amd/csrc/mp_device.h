@@ -125,53 +125,42 @@ __device__ __forceinline__ double mp_wave_sum(double v) {
     return v;
 }
 
-// Sum over the 64 lanes of a wave with DPP moves (no LDS traffic), in a fixed
-// order: pairs, quads, half-rows, rows (quad_perm, row_half_mirror,
-// row_mirror), then rows 0+1 and 2+3 (row_bcast:15) and all four
-// (row_bcast:31); lane 63 ends with the total, which every lane returns.
+// A double moved between lanes by a DPP pattern within 16-lane rows: 0 swaps
+// adjacent lanes, 1 adjacent pairs (quad_perm [1,0,3,2] / [2,3,0,1]), 2
+// mirrors each half-row, 3 each row.
 __device__ __forceinline__ double mp_dpp_f64(double v, int ctrl_id) {
     const long long b = __double_as_longlong(v);
     int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
     switch (ctrl_id) {
         case 0:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xf, 0xf, false);
             hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xf, 0xf, false);
             break;
         case 1:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xf, 0xf, false);
             hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xf, 0xf, false);
             break;
         case 2:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xf, 0xf, false);  // row_half_mirror
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xf, 0xf, false);
             hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xf, 0xf, false);
             break;
-        case 3:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xf, 0xf, false);  // row_mirror
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xf, 0xf, false);
-            break;
-        case 4:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x142, 0xa, 0xf, false);
-            break;
         default:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xc, 0xf, false);
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xf, 0xf, false);
             break;
     }
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
-__device__ __forceinline__ double mp_wave_sum_dpp(double v) {
+// Sum over each 16-lane row of a wave in a fixed order (pairs, quads,
+// half-rows, rows) with DPP moves, no LDS traffic; every lane of a row
+// returns that row's sum.
+__device__ __forceinline__ double mp_row_sum_dpp(double v) {
     v += mp_dpp_f64(v, 0);
     v += mp_dpp_f64(v, 1);
     v += mp_dpp_f64(v, 2);
     v += mp_dpp_f64(v, 3);
-    v += mp_dpp_f64(v, 4);  // lanes of rows 0 and 2 add 0
-    v += mp_dpp_f64(v, 5);  // lanes of rows 0 and 1 add 0
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+    return v;
 }
 
 // Neumaier summation of a sequence of doubles, used for every cross-tile and

@@ -1552,21 +1552,22 @@ __device__ __forceinline__ void reduce_group(const double* base, uint64_t stride
 // Block sum of NV values; thread i < NV stores value i to dst[i * stride]
 // as a write-once slot.
 template <int NV>
+// (DPP sums within each 16-lane row, then the row sums of every wave added
+// in order by one thread per value.)
 __device__ __forceinline__ void block_sum_to_slots(double (&v)[NV], double* dst, uint64_t stride) {
-    __shared__ double red[16][NV];
+    __shared__ double red[64][NV];
     const int lane = threadIdx.x & (MP_WAVE - 1);
-    const int wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
+    const int nrows = blockDim.x >> 4;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = mp_wave_sum_dpp(v[i]);
-    if (lane == 0) {
+    for (int i = 0; i < NV; ++i) v[i] = mp_row_sum_dpp(v[i]);
+    if ((lane & 15) == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) red[wave][i] = v[i];
+        for (int i = 0; i < NV; ++i) red[threadIdx.x >> 4][i] = v[i];
     }
     __syncthreads();
     if (threadIdx.x < NV) {
         double acc = 0.0;
-        for (int w = 0; w < nw; ++w) acc += red[w][threadIdx.x];
+        for (int r = 0; r < nrows; ++r) acc += red[r][threadIdx.x];
         mp_store_coherent(dst + threadIdx.x * stride, acc);
     }
 }
