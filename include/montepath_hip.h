@@ -275,7 +275,13 @@ int mp_hip_device_count(int* count);
 
 /* Open a context on one device: stream, arenas, Sobol tables (reference
  * RuntimeBackend::compile, backend/mod.rs:203-207, and the Metal runtime
- * cache set-up, backend/metal.rs:686-751). */
+ * cache set-up, backend/metal.rs:686-751).  Environment read here:
+ * MONTEPATH_LSM_MULTI_LAUNCH=1 prices American/Bermudan puts with per-date
+ * launches that give the one-launch pricing's bits and never wait on other
+ * blocks (for contexts that share the device with concurrent work; the
+ * one-launch form needs its whole grid resident), =2 with the plain
+ * per-date launches; MONTEPATH_QMC_STACK_BRIDGE=1 keeps FP32 Brownian
+ * bridges in LDS (same bits as the register form). */
 int mp_hip_open(int device_ordinal, mp_ctx** out);
 
 /* Release the context and every device buffer it owns. */
