@@ -136,7 +136,7 @@ struct mp_ctx {
     mp_timing last{};
     std::map<uint64_t, mp_tuning> tuning;
     // arenas
-    dbuf reqs, tile_req, order, partial, out, counter;
+    dbuf reqs, partial, counter;
     dbuf normals, uniforms, jumps, term, pay;
     dbuf plan, basket_par;
     dbuf lsm_paths, lsm_cash, lsm_ex, lsm_coeff, lsm_valid, lsm_gram, lsm_par, lsm_cnt,
@@ -153,6 +153,11 @@ struct mp_ctx {
     bool qmc_tree = true;
     dbuf draws;
     dbuf sampler;
+    // result row of one-request pricings: coherent pinned host memory that
+    // finalize_kernel writes directly and the host reads once the stream has
+    // synchronised (no copy command)
+    double* h_res = nullptr;
+    double* h_res_dev = nullptr;
 };
 
 namespace {
@@ -245,9 +250,12 @@ bool finite(double x) { return std::isfinite(x); }
 
 // Paths per work-queue tile: a pure function of the request, so a request's
 // tiles -- and hence its FP64 partial sums -- are the same alone and in a mix.
+// At most 2^17 path-steps per tile, and at least ~1,024 tiles (one wave
+// each) when the request has the paths, so a request priced alone spreads
+// over the chip; 64 to 1,024 paths.
 uint32_t tile_paths_for(uint64_t n_paths, uint64_t n_steps) {
     const uint64_t by_work = (1ull << 17) / std::max<uint64_t>(n_steps, 1);
-    const uint64_t by_count = n_paths / 256;
+    const uint64_t by_count = n_paths / 1024;
     uint64_t t = pow2_floor(std::max<uint64_t>(1, std::min(by_work, by_count)));
     t = std::min<uint64_t>(std::max<uint64_t>(t, 64), 1024);
     return (uint32_t)t;
@@ -882,10 +890,9 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     ENSURE(c->reqs, sizeof(mp_dreq));
     ENSURE(c->plan, std::max<size_t>(ops.size(), 1) * sizeof(mp_bbop));
     ENSURE(c->partial, (size_t)d.n_tiles * MP_PARTIAL_STRIDE * 8);
-    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
     if (term_host) ENSURE(c->term, d.n_paths * 8);
     if (pay_host) ENSURE(c->pay, d.n_paths * 8);
-    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(mp::launch_setup(d, c->reqs.as<mp_dreq>(), nullptr, 0, c->stream));
     if (!ops.empty())
         HIPCHK(hipMemcpyAsync(c->plan.p, ops.data(), ops.size() * sizeof(mp_bbop), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -896,13 +903,13 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
                           pay_host ? c->pay.as<double>() : nullptr, precision, grid, block, nib_lds,
                           c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->out.as<double>(), c->stream));
+    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->h_res_dev, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    double fin[MP_PARTIAL_STRIDE];
-    HIPCHK(hipMemcpyAsync(fin, c->out.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
     if (term_host) HIPCHK(hipMemcpyAsync(term_host, c->term.p, d.n_paths * 8, hipMemcpyDeviceToHost, c->stream));
     if (pay_host) HIPCHK(hipMemcpyAsync(pay_host, c->pay.p, d.n_paths * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    double fin[MP_PARTIAL_STRIDE];
+    std::memcpy(fin, c->h_res, sizeof(fin));
     float main_ms = 0.f, fin_ms = 0.f;
     (void)hipEventElapsedTime(&main_ms, c->ev[0], c->ev[1]);
     (void)hipEventElapsedTime(&fin_ms, c->ev[1], c->ev[2]);
@@ -958,7 +965,6 @@ int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const
     ENSURE(c->reqs, sizeof(mp_dreq));
     ENSURE(c->basket_par, par.size() * 8);
     ENSURE(c->partial, (size_t)d.n_tiles * MP_PARTIAL_STRIDE * 8);
-    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
     const size_t nn = (size_t)d.n_paths * d.n_steps * na;
     if (normals_host) ENSURE(c->normals, nn * 8);
     if (term_host) ENSURE(c->term, d.n_paths * 8);
@@ -970,7 +976,7 @@ int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const
         HIPCHK(hipMemcpyAsync(c->sampler.p, P.sampler.data(), P.sampler.size() * sizeof(mp_sampler_dim),
                               hipMemcpyHostToDevice, c->stream));
     }
-    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(mp::launch_setup(d, c->reqs.as<mp_dreq>(), nullptr, 0, c->stream));
     HIPCHK(hipMemcpyAsync(c->basket_par.p, par.data(), par.size() * 8, hipMemcpyHostToDevice, c->stream));
     if (normals_host) HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, nn * 8, hipMemcpyHostToDevice, c->stream));
     const double* bp = c->basket_par.as<double>();
@@ -983,13 +989,13 @@ int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const
                              term_host ? c->term.as<double>() : nullptr,
                              pay_host ? c->pay.as<double>() : nullptr, precision, grid, block, c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->out.as<double>(), c->stream));
+    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->h_res_dev, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    double fin[MP_PARTIAL_STRIDE];
-    HIPCHK(hipMemcpyAsync(fin, c->out.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
     if (term_host) HIPCHK(hipMemcpyAsync(term_host, c->term.p, d.n_paths * 8, hipMemcpyDeviceToHost, c->stream));
     if (pay_host) HIPCHK(hipMemcpyAsync(pay_host, c->pay.p, d.n_paths * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    double fin[MP_PARTIAL_STRIDE];
+    std::memcpy(fin, c->h_res, sizeof(fin));
     float main_ms = 0.f, fin_ms = 0.f;
     (void)hipEventElapsedTime(&main_ms, c->ev[0], c->ev[1]);
     (void)hipEventElapsedTime(&fin_ms, c->ev[1], c->ev[2]);
@@ -1208,9 +1214,7 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     const int F = d.model == MP_MODEL_HESTON ? 2 : 1;
     const size_t nn = (size_t)n * d.n_steps * F;
     ENSURE(c->reqs, sizeof(mp_dreq));
-    ENSURE(c->tile_req, (size_t)d.n_tiles * 4);
     ENSURE(c->partial, (size_t)d.n_tiles * MP_PARTIAL_STRIDE * 8);
-    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
     ENSURE(c->counter, 64);
     const bool want_paths = term_host != nullptr || pay_host != nullptr;
     if (want_paths) {
@@ -1224,7 +1228,6 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
             ENSURE(c->jumps, (size_t)n * d.n_steps * 8);
         }
     }
-    std::vector<uint32_t> tile_req(d.n_tiles, 0u);
     if (!buffer && !P.sampler.empty()) {
         d.sampler_off = 0;
         ENSURE(c->sampler, P.sampler.size() * sizeof(mp_sampler_dim));
@@ -1232,8 +1235,6 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
                               hipMemcpyHostToDevice, c->stream));
     }
     if (buffer) d.sampling = MP_SAMPLER_NONE;  // caller buffers replace every draw
-    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->tile_req.p, tile_req.data(), (size_t)d.n_tiles * 4, hipMemcpyHostToDevice, c->stream));
     if (buffer) {
         HIPCHK(hipMemcpyAsync(c->normals.p, normals, nn * 8, hipMemcpyHostToDevice, c->stream));
         if (d.model == MP_MODEL_MERTON) {
@@ -1244,7 +1245,7 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     const mp_tuning tu = tuning_for(c, d, precision);
     mp::tile_launch L{};
     L.reqs = c->reqs.as<mp_dreq>();
-    L.tile_req = c->tile_req.as<uint32_t>();
+    L.tile_req = nullptr;  // one request: every tile is request 0's
     L.order = nullptr;
     L.n_tiles = d.n_tiles;
     L.counter = c->counter.as<unsigned int>();
@@ -1260,17 +1261,17 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     L.block = tu.block_size > 0 ? tu.block_size : 256;
     L.ppl = tu.paths_per_lane > 0 && d.payoff != MP_PAY_ASIAN_MLMC ? tu.paths_per_lane : 1;
     L.grid = grid_for_tiles(c, d.n_tiles, L.block);
-    HIPCHK(mp::launch_reset(L.counter, 1, c->stream));
+    HIPCHK(mp::launch_setup(d, c->reqs.as<mp_dreq>(), L.counter, 1, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(mp::launch_tiles(L, precision, c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(mp::launch_finalize(L.reqs, 1, L.partial, c->out.as<double>(), c->stream));
+    HIPCHK(mp::launch_finalize(L.reqs, 1, L.partial, c->h_res_dev, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    double fin[MP_PARTIAL_STRIDE];
-    HIPCHK(hipMemcpyAsync(fin, c->out.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
     if (term_host) HIPCHK(hipMemcpyAsync(term_host, c->term.p, n * 8, hipMemcpyDeviceToHost, c->stream));
     if (pay_host) HIPCHK(hipMemcpyAsync(pay_host, c->pay.p, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    double fin[MP_PARTIAL_STRIDE];
+    std::memcpy(fin, c->h_res, sizeof(fin));
     float main_ms = 0.f, fin_ms = 0.f;
     (void)hipEventElapsedTime(&main_ms, c->ev[0], c->ev[1]);
     (void)hipEventElapsedTime(&fin_ms, c->ev[1], c->ev[2]);
@@ -1350,6 +1351,9 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
         return fail(MP_E_UNSUPPORTED, "Sobol direction numbers 0..15 have low bits set");
     HIPCHK(hipMalloc(&c->d_nib, nib.size() * 4));
     HIPCHK(hipMemcpy(c->d_nib, nib.data(), nib.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_res), MP_PARTIAL_STRIDE * 8,
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_res_dev), c->h_res, 0));
     *out = c.release();
     return MP_OK;
 }
@@ -1360,6 +1364,7 @@ MP_EXPORT void mp_hip_close(mp_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& b : c->bat) delete b;
     if (c->d_nib) (void)hipFree(c->d_nib);
+    if (c->h_res) (void)hipHostFree(c->h_res);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1539,24 +1544,23 @@ MP_EXPORT int mp_hip_price_plain_fp64(mp_ctx* c, const mp_request* req, mp_resul
     ENSURE(c->reqs, sizeof(mp_dreq));
     ENSURE(c->plain_pay, n * 8);
     ENSURE(c->partial, (size_t)rblocks * MP_PARTIAL_STRIDE * 8);
-    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
     // finalize sums payoffs that are already discounted
     mp_dreq fin_d = d;
     fin_d.tile0 = 0;
     fin_d.n_tiles = (uint32_t)rblocks;
     fin_d.n_strikes = 1;
     fin_d.discount = 1.0;
-    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(mp::launch_setup(d, c->reqs.as<mp_dreq>(), nullptr, 0, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(mp::launch_plain_fp64(c->reqs.as<mp_dreq>(), c->plain_pay.as<double>(), grid, block, c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     HIPCHK(mp::launch_plain_reduce(c->plain_pay.as<double>(), n, c->partial.as<double>(), rblocks, c->stream));
-    HIPCHK(hipMemcpyAsync(c->reqs.p, &fin_d, sizeof(fin_d), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->out.as<double>(), c->stream));
+    HIPCHK(mp::launch_setup(fin_d, c->reqs.as<mp_dreq>(), nullptr, 0, c->stream));
+    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->h_res_dev, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    double fin[MP_PARTIAL_STRIDE];
-    HIPCHK(hipMemcpyAsync(fin, c->out.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    double fin[MP_PARTIAL_STRIDE];
+    std::memcpy(fin, c->h_res, sizeof(fin));
     float main_ms = 0.f, fin_ms = 0.f;
     (void)hipEventElapsedTime(&main_ms, c->ev[0], c->ev[1]);
     (void)hipEventElapsedTime(&fin_ms, c->ev[1], c->ev[2]);
@@ -1812,18 +1816,16 @@ int greeks_single(mp_ctx* c, const mp_request& r, int estimator, const double* n
     d.technique = MP_TECH_RAW_MOMENTS;
     ENSURE(c->reqs, sizeof(mp_dreq));
     ENSURE(c->partial, (size_t)grid * MP_PARTIAL_STRIDE * 8);
-    ENSURE(c->out, MP_PARTIAL_STRIDE * 8);
     if (normals) ENSURE(c->normals, d.n_paths * 8);
-    HIPCHK(hipMemcpyAsync(c->reqs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(mp::launch_setup(d, c->reqs.as<mp_dreq>(), nullptr, 0, c->stream));
     if (normals) HIPCHK(hipMemcpyAsync(c->normals.p, normals, d.n_paths * 8, hipMemcpyHostToDevice, c->stream));
     const int est = estimator == MP_GREEKS_PATHWISE ? MP_GREEK_EST_PATHWISE : MP_GREEK_EST_LIKELIHOOD_RATIO;
     HIPCHK(mp::launch_greeks(c->reqs.as<mp_dreq>(), est, normals ? c->normals.as<double>() : nullptr,
                              c->partial.as<double>(), precision, grid, block, c->stream));
-    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->out.as<double>(),
-                               c->stream));
-    double tot[MP_PARTIAL_STRIDE];
-    HIPCHK(hipMemcpyAsync(tot, c->out.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(mp::launch_finalize(c->reqs.as<mp_dreq>(), 1, c->partial.as<double>(), c->h_res_dev, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    double tot[MP_PARTIAL_STRIDE];
+    std::memcpy(tot, c->h_res, sizeof(tot));
     static const int pathwise[4] = {MP_GREEK_DELTA, MP_GREEK_VEGA, MP_GREEK_RHO, MP_GREEK_THETA};
     const int count = est == MP_GREEK_EST_PATHWISE ? 4 : 3;
     const double nn = (double)d.n_paths;

@@ -116,7 +116,7 @@ namespace mp {
 
 struct tile_launch {
     const mp_dreq* reqs;
-    const uint32_t* tile_req;   // tile id -> request index
+    const uint32_t* tile_req;   // tile id -> request index (null: every tile is request 0's)
     const uint32_t* order;      // queue position -> tile id
     uint32_t n_tiles;
     unsigned int* counter;
@@ -132,9 +132,13 @@ struct tile_launch {
     int block;
     int ppl;                    // paths interleaved per lane
     int mlmc;                   // every request is a multilevel level (own kernel)
+    int claim;                  // queue positions per claim (launch_tiles sets it)
 };
 
 hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s);
+// Stores one request record (a kernel argument) at dst and clears
+// counter[0 .. n-1] (n <= 64): a launch in place of two copy commands.
+hipError_t launch_setup(const mp_dreq& d, mp_dreq* dst, unsigned int* counter, int n, hipStream_t s);
 hipError_t launch_tiles(const tile_launch& L, int precision, hipStream_t s);
 hipError_t launch_finalize(const mp_dreq* reqs, int n_reqs, const double* partial, double* out,
                            hipStream_t s);

@@ -228,8 +228,8 @@ __device__ __forceinline__ R payoff_of(const path_state<R>& st, const model_para
 // compiler's loop structurisation leaves at this point.
 __device__ __forceinline__ unsigned int next_queue_position(unsigned int* counter,
                                                             volatile unsigned int* slot,
-                                                            int lane) {
-    if (lane == 0) *slot = atomicAdd(counter, 1u);
+                                                            int lane, unsigned int k) {
+    if (lane == 0) *slot = atomicAdd(counter, k);
     __builtin_amdgcn_wave_barrier();
     return __builtin_amdgcn_readfirstlane(*slot);
 }
@@ -489,10 +489,14 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
             }
         }
     }
+    // columns past the last one the request uses hold zeros in every lane
+    const int used = tech == MP_TECH_CONTROL_VARIATE ? 3 : (ns > 1 ? ns : 1);
 #pragma unroll
     for (int j = 0; j < MP_NS_MAX; ++j) {
-        sum[j] = mp_wave_sum(sum[j]);
-        sq[j] = mp_wave_sum(sq[j]);
+        if (j < used) {
+            sum[j] = mp_wave_sum(sum[j]);
+            sq[j] = mp_wave_sum(sq[j]);
+        }
     }
     if (lane == 0) {
         double* out = L.partial + (uint64_t)tile * MP_PARTIAL_STRIDE;
@@ -504,21 +508,12 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
     }
 }
 
-// ML: a launch of multilevel levels only (their own kernel, so the coarse
-// path state does not raise the register count of the general kernel).
+// The tile at queue position qpos, with the body compiled for its request.
 template <class R, int SRC, int PPL, bool ML>
-__global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
-    __shared__ unsigned int queue_slot[256 / MP_WAVE];
-    const int lane = threadIdx.x & (MP_WAVE - 1);
-    volatile unsigned int* slot = &queue_slot[threadIdx.x >> 6];
-    // A wave claims at most n_tiles + 1 positions, so the loop ends even if the
-    // queue counter were left in a bad state.
-    unsigned int claims = 0;
-    for (unsigned int qpos = next_queue_position(L.counter, slot, lane);
-         qpos < L.n_tiles && claims <= L.n_tiles;
-         qpos = next_queue_position(L.counter, slot, lane), ++claims) {
+__device__ __forceinline__ void run_tile(const mp::tile_launch& L, unsigned int qpos, int lane) {
+    {
         const uint32_t tile = L.order ? L.order[qpos] : qpos;
-        const uint32_t ri = L.tile_req[tile];
+        const uint32_t ri = L.tile_req ? L.tile_req[tile] : 0u;
         const mp_dreq& q = L.reqs[ri];
         if constexpr (ML) {
             // multilevel levels, one path per lane
@@ -566,9 +561,38 @@ __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
     }
 }
 
+// ML: a launch of multilevel levels only (their own kernel, so the coarse
+// path state does not raise the register count of the general kernel).
+template <class R, int SRC, int PPL, bool ML>
+__global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
+    __shared__ unsigned int queue_slot[256 / MP_WAVE];
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    volatile unsigned int* slot = &queue_slot[threadIdx.x >> 6];
+    // A claim takes L.claim consecutive queue positions (one atomic for
+    // several tiles when the queue is long).  A wave makes at most
+    // n_tiles + 1 claims, so the loop ends even if the queue counter were left
+    // in a bad state.
+    const unsigned int k = (unsigned int)L.claim;
+    unsigned int claims = 0;
+    for (unsigned int q0 = next_queue_position(L.counter, slot, lane, k);
+         q0 < L.n_tiles && claims <= L.n_tiles;
+         q0 = next_queue_position(L.counter, slot, lane, k), ++claims) {
+        const unsigned int qend = min(q0 + k, L.n_tiles);
+        for (unsigned int qpos = q0; qpos < qend; ++qpos) run_tile<R, SRC, PPL, ML>(L, qpos, lane);
+    }
+}
+
 __global__ void reset_kernel(unsigned int* counter, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) counter[i] = 0u;
+}
+
+// Set-up of a one-request launch: the request record arrives as a kernel
+// argument and is stored where the kernels read it, and the first n
+// counters are cleared -- in place of a host-to-device copy and a reset.
+__global__ void setup_kernel(mp_dreq d, mp_dreq* dst, unsigned int* counter, int n) {
+    if (threadIdx.x == 0) *dst = d;
+    if ((int)threadIdx.x < n) counter[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -2100,6 +2124,12 @@ hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_setup(const mp_dreq& d, mp_dreq* dst, unsigned int* counter, int n, hipStream_t s) {
+    if (n < 0 || n > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(setup_kernel, dim3(1), dim3(64), 0, s, d, dst, counter, n);
+    return hipGetLastError();
+}
+
 template <class R, int SRC>
 static hipError_t tiles_ppl(const tile_launch& L, hipStream_t s) {
     if (L.mlmc) {
@@ -2119,7 +2149,13 @@ static hipError_t tiles_ppl(const tile_launch& L, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_tiles(const tile_launch& L, int precision, hipStream_t s) {
+hipError_t launch_tiles(const tile_launch& L0, int precision, hipStream_t s) {
+    // Positions per claim: 1 while the queue holds fewer than 16 tiles per
+    // wave (every wave should get work), up to 4 on long queues, where the
+    // one queue counter's atomic rate would otherwise bound the launch.
+    tile_launch L = L0;
+    const uint64_t waves = (uint64_t)L.grid * (uint64_t)(L.block / MP_WAVE);
+    L.claim = (int)std::min<uint64_t>(4, std::max<uint64_t>(1, L.n_tiles / std::max<uint64_t>(1, 16 * waves)));
     const bool buf = L.normals != nullptr;
     if (precision == 64) return buf ? tiles_ppl<double, 1>(L, s) : tiles_ppl<double, 0>(L, s);
     return buf ? tiles_ppl<float, 1>(L, s) : tiles_ppl<float, 0>(L, s);
