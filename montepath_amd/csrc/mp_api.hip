@@ -96,6 +96,41 @@ struct dbuf {
     }
 };
 
+// Pinned host memory, mapped for the device and coherent, grown on demand
+// (only between uses that the stream has completed).
+struct hbuf {
+    void* p = nullptr;
+    void* dev = nullptr;  // the device's address of p
+    size_t cap = 0;
+    hbuf() = default;
+    hbuf(const hbuf&) = delete;
+    hbuf& operator=(const hbuf&) = delete;
+    ~hbuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    int ensure(size_t bytes) {
+        if (bytes == 0) bytes = 8;
+        if (cap >= bytes) return MP_OK;
+        if (p) (void)hipHostFree(p);
+        p = dev = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+        if (hipHostMalloc(&p, want, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer(&dev, p, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            if (p) (void)hipHostFree(p);
+            p = dev = nullptr;
+            return fail(MP_E_OOM, "pinned host allocation of " + std::to_string(want) + " bytes failed");
+        }
+        cap = want;
+        return MP_OK;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
 #define ENSURE(buf, bytes)                      \
     do {                                        \
         int rc_ = (buf).ensure(bytes);          \
@@ -670,8 +705,9 @@ struct mp_batch {
     uint32_t n_tiles = 0;
     int grid = 1, block = 256, ppl = 1;
     bool mlmc = false;
-    dbuf d_reqs, d_tile_req, d_order, d_partial, d_out, d_counter, d_sampler;
-    std::vector<double> h_out;
+    dbuf d_reqs, d_tile_req, d_order, d_partial, d_counter, d_sampler;
+    hbuf h_stage;  // request records, tile map, queue order, sampler tables
+    hbuf h_out;    // result rows, written by finalize_kernel
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -717,17 +753,28 @@ int batch_build(mp_batch* b) {
     ENSURE(b->d_tile_req, (size_t)tiles * 4);
     ENSURE(b->d_order, (size_t)tiles * 4);
     ENSURE(b->d_partial, (size_t)tiles * MP_PARTIAL_STRIDE * 8);
-    ENSURE(b->d_out, n * MP_PARTIAL_STRIDE * 8);
     ENSURE(b->d_counter, 64);
-    HIPCHK(hipMemcpyAsync(b->d_reqs.p, dreqs.data(), n * sizeof(mp_dreq), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipMemcpyAsync(b->d_tile_req.p, tile_req.data(), (size_t)tiles * 4, hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipMemcpyAsync(b->d_order.p, order.data(), (size_t)tiles * 4, hipMemcpyHostToDevice, b->stream));
-    if (!sampler.empty()) {
-        ENSURE(b->d_sampler, sampler.size() * sizeof(mp_sampler_dim));
-        HIPCHK(hipMemcpyAsync(b->d_sampler.p, sampler.data(), sampler.size() * sizeof(mp_sampler_dim),
+    if (!sampler.empty()) ENSURE(b->d_sampler, sampler.size() * sizeof(mp_sampler_dim));
+    // Stage the inputs in pinned memory and copy them asynchronously: the
+    // staging outlives the copies (a batch is rebuilt only after its stream
+    // has completed), and the launches follow them on the same stream.
+    const size_t o_req = 0;
+    const size_t o_tile = (o_req + n * sizeof(mp_dreq) + 255) & ~(size_t)255;
+    const size_t o_order = (o_tile + (size_t)tiles * 4 + 255) & ~(size_t)255;
+    const size_t o_samp = (o_order + (size_t)tiles * 4 + 255) & ~(size_t)255;
+    ENSURE(b->h_stage, o_samp + sampler.size() * sizeof(mp_sampler_dim));
+    char* st = b->h_stage.as<char>();
+    std::memcpy(st + o_req, dreqs.data(), n * sizeof(mp_dreq));
+    std::memcpy(st + o_tile, tile_req.data(), (size_t)tiles * 4);
+    std::memcpy(st + o_order, order.data(), (size_t)tiles * 4);
+    if (!sampler.empty()) std::memcpy(st + o_samp, sampler.data(), sampler.size() * sizeof(mp_sampler_dim));
+    HIPCHK(hipMemcpyAsync(b->d_reqs.p, st + o_req, n * sizeof(mp_dreq), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_tile_req.p, st + o_tile, (size_t)tiles * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_order.p, st + o_order, (size_t)tiles * 4, hipMemcpyHostToDevice, b->stream));
+    if (!sampler.empty())
+        HIPCHK(hipMemcpyAsync(b->d_sampler.p, st + o_samp, sampler.size() * sizeof(mp_sampler_dim),
                               hipMemcpyHostToDevice, b->stream));
-    }
-    HIPCHK(hipStreamSynchronize(b->stream));
+    ENSURE(b->h_out, n * MP_PARTIAL_STRIDE * 8);
     // launch shape: tuned for the dominant request class
     mp_tuning tu = tuning_for(b->ctx, dreqs.empty() ? mp_dreq{} : dreqs[0], b->precision);
     b->block = tu.block_size > 0 ? tu.block_size : 256;
@@ -739,7 +786,6 @@ int batch_build(mp_batch* b) {
     b->mlmc = n_mlmc != 0;
     if (b->mlmc) b->ppl = 1;  // the multilevel kernel runs one path per lane
     b->grid = grid_for_tiles(b->ctx, tiles, b->block);
-    b->h_out.assign(n * MP_PARTIAL_STRIDE, 0.0);
     return MP_OK;
 }
 
@@ -766,7 +812,7 @@ int batch_enqueue(mp_batch* b) {
     HIPCHK(hipEventRecord(b->e0, b->stream));
     HIPCHK(mp::launch_tiles(L, b->precision, b->stream));
     HIPCHK(hipEventRecord(b->e1, b->stream));
-    HIPCHK(mp::launch_finalize(L.reqs, (int)b->reqs.size(), L.partial, b->d_out.as<double>(), b->stream));
+    HIPCHK(mp::launch_finalize(L.reqs, (int)b->reqs.size(), L.partial, static_cast<double*>(b->h_out.dev), b->stream));
     HIPCHK(hipEventRecord(b->e2, b->stream));
     return MP_OK;
 }
@@ -791,7 +837,7 @@ int batch_capture(mp_batch* b) {
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = mp::launch_reset(L.counter, 1, b->stream);
     hipError_t e2 = mp::launch_tiles(L, b->precision, b->stream);
-    hipError_t e3 = mp::launch_finalize(L.reqs, (int)b->reqs.size(), L.partial, b->d_out.as<double>(), b->stream);
+    hipError_t e3 = mp::launch_finalize(L.reqs, (int)b->reqs.size(), L.partial, static_cast<double*>(b->h_out.dev), b->stream);
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamEndCapture(b->stream, &g));
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -806,7 +852,6 @@ int batch_capture(mp_batch* b) {
 
 int batch_collect(mp_batch* b, mp_result* results, bool timed) {
     if (b->reqs.empty()) return MP_OK;
-    HIPCHK(hipMemcpyAsync(b->h_out.data(), b->d_out.p, b->h_out.size() * 8, hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     float main_ms = 0.f, fin_ms = 0.f;
     if (timed && !b->use_graph) {
@@ -817,7 +862,7 @@ int batch_collect(mp_batch* b, mp_result* results, bool timed) {
         b->ctx->last.main_launches = 1;
     }
     for (size_t i = 0; i < b->reqs.size(); ++i) {
-        fill_result(&results[b->req_index[i]], &b->h_out[i * MP_PARTIAL_STRIDE], b->reqs[i]);
+        fill_result(&results[b->req_index[i]], b->h_out.as<double>() + i * MP_PARTIAL_STRIDE, b->reqs[i]);
         results[b->req_index[i]].runtime_ms = main_ms + fin_ms;
     }
     return MP_OK;
@@ -1446,6 +1491,14 @@ MP_EXPORT int mp_hip_price_batch(mp_ctx* c, const mp_request* reqs, size_t n, mp
     }
     for (mp_batch* b : {&b32, &b64}) {
         if (b->reqs.empty()) continue;
+        if (b->reqs.size() == 1) {
+            // one request: the one-request path (the same tiles, hence the
+            // same bits, without staging the tile map and queue order)
+            rc = run_tiles_single(c, b->reqs[0], b->precision, &out[b->req_index[0]], nullptr, nullptr,
+                                  nullptr, nullptr, nullptr);
+            if (rc != MP_OK) return rc;
+            continue;
+        }
         rc = batch_build(b);
         if (rc != MP_OK) return rc;
         rc = batch_enqueue(b);
