@@ -1374,7 +1374,14 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
         (void)hipGetLastError();
         return fail(MP_E_UNKNOWN_DEVICE, "rocm:" + std::to_string(device_ordinal) + " is not available");
     }
-    std::unique_ptr<mp_ctx> c(new mp_ctx());
+    // a context that fails part-way is closed, which frees what it holds
+    struct open_guard {
+        mp_ctx* c;
+        ~open_guard() {
+            if (c) mp_hip_close(c);
+        }
+    } guard{new mp_ctx()};
+    mp_ctx* c = guard.c;
     c->device = device_ordinal;
     HIPCHK(hipSetDevice(device_ordinal));
     hipDeviceProp_t prop;
@@ -1399,7 +1406,8 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_res), MP_PARTIAL_STRIDE * 8,
                          hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_res_dev), c->h_res, 0));
-    *out = c.release();
+    guard.c = nullptr;
+    *out = c;
     return MP_OK;
 }
 
