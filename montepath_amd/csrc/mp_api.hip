@@ -131,6 +131,12 @@ struct hbuf {
     }
 };
 
+#define RETCHK(expr)                  \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != MP_OK) return rc_; \
+    } while (0)
+
 #define ENSURE(buf, bytes)                      \
     do {                                        \
         int rc_ = (buf).ensure(bytes);          \
@@ -175,7 +181,7 @@ struct mp_ctx {
     dbuf normals, uniforms, jumps, term, pay;
     dbuf plan, basket_par;
     dbuf lsm_paths, lsm_cash, lsm_ex, lsm_coeff, lsm_valid, lsm_gram, lsm_par, lsm_cnt,
-        lsm_part, lsm_out, lsm_dates, plain_pay;
+        lsm_part, lsm_dates, plain_pay;
     // one-launch LSM pricing when the request's grid fits on the device at
     // once; MONTEPATH_LSM_MULTI_LAUNCH at open selects per-date launches:
     // 1 matched to the one-launch pricing (same bits), 2 the plain grid-stride
@@ -191,11 +197,32 @@ struct mp_ctx {
     // result row of one-request pricings: coherent pinned host memory that
     // finalize_kernel writes directly and the host reads once the stream has
     // synchronised (no copy command)
-    double* h_res = nullptr;
+    double* h_res = nullptr;      // 32 doubles: the result row, then a wait-error word
     double* h_res_dev = nullptr;
+    // small host-to-device inputs of one call (request parameters, plans,
+    // tables) go through this pinned buffer as asynchronous copies; a call
+    // synchronises its stream before it returns, so the next one reuses it
+    // from the start
+    hbuf h_stage;
+    size_t stage_off = 0;
 };
 
 namespace {
+
+// Host-to-device copy of a call's small input through the context's pinned
+// staging buffer; inputs that do not fit are copied from where they are.
+int upload(mp_ctx* c, void* dst, const void* src, size_t bytes) {
+    const size_t at = (c->stage_off + 255) & ~(size_t)255;
+    if (at + bytes <= c->h_stage.cap) {
+        char* st = c->h_stage.as<char>() + at;
+        std::memcpy(st, src, bytes);
+        c->stage_off = at + bytes;
+        HIPCHK(hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    return MP_OK;
+}
 
 // 32-bit direction numbers from the Joe-Kuo primitive polynomials and
 // initial values (Bratley-Fox recurrence).
@@ -888,6 +915,7 @@ int batch_init(mp_ctx* ctx, mp_batch* b, hipStream_t stream, int precision) {
 // ---------------------------------------------------------------------------
 int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_result* out,
             double* term_host, double* pay_host) {
+    c->stage_off = 0;  // the previous call has synchronised
     mp_dreq d = P.d;
     std::vector<mp_bbop> ops;
     int n_slots = 1;
@@ -939,7 +967,7 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     if (pay_host) ENSURE(c->pay, d.n_paths * 8);
     HIPCHK(mp::launch_setup(d, c->reqs.as<mp_dreq>(), nullptr, 0, c->stream));
     if (!ops.empty())
-        HIPCHK(hipMemcpyAsync(c->plan.p, ops.data(), ops.size() * sizeof(mp_bbop), hipMemcpyHostToDevice, c->stream));
+        RETCHK(upload(c, c->plan.p, ops.data(), ops.size() * sizeof(mp_bbop)));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     // techniques run in the generic kernel (payoff -1 selects it)
     const int spec_payoff = d.technique == MP_TECH_STANDARD ? d.payoff : -1;
@@ -968,6 +996,7 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
 
 int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const double* normals_host,
                mp_result* out, double* term_host, double* pay_host) {
+    c->stage_off = 0;  // the previous call has synchronised
     const int na = P.n_assets;
     // Cholesky of the correlation matrix
     std::vector<double> corr(na * na), L(na * na, 0.0);
@@ -1018,11 +1047,10 @@ int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const
     if (d.sampling != MP_SAMPLER_NONE) {
         d.sampler_off = 0;
         ENSURE(c->sampler, P.sampler.size() * sizeof(mp_sampler_dim));
-        HIPCHK(hipMemcpyAsync(c->sampler.p, P.sampler.data(), P.sampler.size() * sizeof(mp_sampler_dim),
-                              hipMemcpyHostToDevice, c->stream));
+        RETCHK(upload(c, c->sampler.p, P.sampler.data(), P.sampler.size() * sizeof(mp_sampler_dim)));
     }
     HIPCHK(mp::launch_setup(d, c->reqs.as<mp_dreq>(), nullptr, 0, c->stream));
-    HIPCHK(hipMemcpyAsync(c->basket_par.p, par.data(), par.size() * 8, hipMemcpyHostToDevice, c->stream));
+    RETCHK(upload(c, c->basket_par.p, par.data(), par.size() * 8));
     if (normals_host) HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, nn * 8, hipMemcpyHostToDevice, c->stream));
     const double* bp = c->basket_par.as<double>();
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -1054,6 +1082,7 @@ int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const
 
 int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const double* normals_host,
             mp_result* out, double* coeff_host, uint32_t* ex_host) {
+    c->stage_off = 0;  // the previous call has synchronised
     const uint64_t n = r.n_paths;
     const int m = (int)r.n_steps;
     mp_lsm_params p{};
@@ -1137,13 +1166,10 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     // flag [m + 2]; one launch uses the final ticket [0] and the error flag [1]
     ENSURE(c->lsm_cnt, (size_t)(m + 3) * 4);
     ENSURE(c->lsm_part, (size_t)grid * 4 * 8);
-    ENSURE(c->lsm_out, 8 * 8);
     ENSURE(c->lsm_dates, (size_t)std::max(n_reg, 1) * 4);
     if (normals_host) ENSURE(c->normals, (size_t)m * n * 8);
-    HIPCHK(hipMemcpyAsync(c->lsm_par.p, blob.data(), blob_bytes, hipMemcpyHostToDevice, c->stream));
-    if (n_reg)
-        HIPCHK(hipMemcpyAsync(c->lsm_dates.p, reg_dates.data(), (size_t)n_reg * 4, hipMemcpyHostToDevice,
-                              c->stream));
+    RETCHK(upload(c, c->lsm_par.p, blob.data(), blob_bytes));
+    if (n_reg) RETCHK(upload(c, c->lsm_dates.p, reg_dates.data(), (size_t)n_reg * 4));
     if (normals_host)
         HIPCHK(hipMemcpyAsync(c->normals.p, normals_host, (size_t)m * n * 8, hipMemcpyHostToDevice, c->stream));
     const mp_lsm_params* dp = c->lsm_par.as<mp_lsm_params>();
@@ -1168,7 +1194,7 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
                                     c->lsm_dates.as<int>(), n_reg, c->lsm_coeff.as<double>(),
                                     c->lsm_valid.as<int>(), slots, slots + (size_t)n_reg * 16 * grid,
                                     slots + (size_t)n_reg * 16 * (grid + 32), c->lsm_part.as<double>(), cnt, c->lsm_ex.as<uint16_t>(),
-                                    c->lsm_out.as<double>(), cnt + 1, 1u << 20, fused_j, grid, precision,
+                                    c->h_res_dev, cnt + 1, 1u << 20, fused_j, grid, precision,
                                     c->stream));
         ++launches;
     } else {
@@ -1199,15 +1225,16 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
         ++launches;
     }
     HIPCHK(mp::launch_lsm_final(dp, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), c->lsm_valid.as<int>(),
-                                c->lsm_part.as<double>(), cnt + (m + 1), c->lsm_out.as<double>(), grid, block,
+                                c->lsm_part.as<double>(), cnt + (m + 1), c->h_res_dev, grid, block,
                                 c->stream));
     }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    double res[8];
-    unsigned int wait_err = 0;
-    HIPCHK(hipMemcpyAsync(res, c->lsm_out.p, 5 * 8, hipMemcpyDeviceToHost, c->stream));
-    if (fused_j) HIPCHK(hipMemcpyAsync(&wait_err, cnt + 1, 4, hipMemcpyDeviceToHost, c->stream));
-    if (matched) HIPCHK(hipMemcpyAsync(&wait_err, cnt + (m + 2), 4, hipMemcpyDeviceToHost, c->stream));
+    // the result row is written to pinned host memory by the last block; the
+    // wait-error word is copied next to it
+    unsigned int* h_err = reinterpret_cast<unsigned int*>(c->h_res + 16);
+    *h_err = 0u;
+    if (fused_j) HIPCHK(hipMemcpyAsync(h_err, cnt + 1, 4, hipMemcpyDeviceToHost, c->stream));
+    if (matched) HIPCHK(hipMemcpyAsync(h_err, cnt + (m + 2), 4, hipMemcpyDeviceToHost, c->stream));
     std::vector<double> coeff;
     if (coeff_host) {
         coeff.resize((size_t)(m + 1) * 4);
@@ -1219,7 +1246,7 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
         HIPCHK(hipMemcpyAsync(ex.data(), c->lsm_ex.p, n * 2, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (wait_err)
+    if (*h_err)
         return fail(MP_E_HIP, "LSM: a block waited too long for the other blocks of its launch "
                               "(the device was shared with other work); result discarded");
     float ms = 0.f;
@@ -1235,6 +1262,8 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     std::memset(out, 0, sizeof(*out));
     out->status = MP_OK;
     out->n_strikes = 1;
+    double res[8];
+    std::memcpy(res, c->h_res, 5 * 8);
     out->price = res[0];
     out->std_error = res[1];
     out->strike_price[0] = res[0];
@@ -1252,6 +1281,7 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
 int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, const double* normals,
                      const double* uniforms, const double* jumps, double* term_host,
                      double* pay_host) {
+    c->stage_off = 0;  // the previous call has synchronised
     mp_dreq d = P.d;
     d.tile0 = 0;
     const bool buffer = normals != nullptr;
@@ -1276,8 +1306,7 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     if (!buffer && !P.sampler.empty()) {
         d.sampler_off = 0;
         ENSURE(c->sampler, P.sampler.size() * sizeof(mp_sampler_dim));
-        HIPCHK(hipMemcpyAsync(c->sampler.p, P.sampler.data(), P.sampler.size() * sizeof(mp_sampler_dim),
-                              hipMemcpyHostToDevice, c->stream));
+        RETCHK(upload(c, c->sampler.p, P.sampler.data(), P.sampler.size() * sizeof(mp_sampler_dim)));
     }
     if (buffer) d.sampling = MP_SAMPLER_NONE;  // caller buffers replace every draw
     if (buffer) {
@@ -1403,8 +1432,9 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
         return fail(MP_E_UNSUPPORTED, "Sobol direction numbers 0..15 have low bits set");
     HIPCHK(hipMalloc(&c->d_nib, nib.size() * 4));
     HIPCHK(hipMemcpy(c->d_nib, nib.data(), nib.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_res), MP_PARTIAL_STRIDE * 8,
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_res), 32 * 8,
                          hipHostMallocMapped | hipHostMallocCoherent));
+    ENSURE(c->h_stage, 1u << 20);
     HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_res_dev), c->h_res, 0));
     guard.c = nullptr;
     *out = c;
