@@ -305,7 +305,8 @@ int mp_hip_price_batch(mp_ctx* ctx, const mp_request* reqs, size_t n, mp_result*
 /* Prepared batch: requests validated, packed and uploaded once (descriptors,
  * tile map and queue order resident in HBM), optionally captured as a
  * hipGraph (queue reset + persistent kernel + finalize) and replayed on its
- * own stream.  launch only enqueues; sync waits; results copies back.  GBM,
+ * own stream.  launch only enqueues; sync waits; results waits and reads the
+ * result rows (which the finalize kernel writes to pinned host memory).  GBM,
  * Heston and Merton pseudorandom requests of one precision.  No reference
  * counterpart (see mp_hip_price_batch). */
 typedef struct mp_batch mp_batch;
