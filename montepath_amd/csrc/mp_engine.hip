@@ -528,8 +528,9 @@ __device__ __forceinline__ void run_tile(const mp::tile_launch& L, unsigned int 
         } else if (SRC == 0 && q.sampling != MP_SAMPLER_NONE) {
             // randomized Halton / Latin hypercube requests
             if constexpr (SRC == 0) tile_body<R, 2, PPL, -1, -1, -1>(L, tile, q, lane);
-        } else if constexpr (sizeof(R) == 4 && SRC == 0) {
-            // FP32 Philox tiles (batches, serving): a body per model and payoff
+        } else if constexpr (SRC == 0 && (sizeof(R) == 4 || PPL == 1)) {
+            // Philox tiles (batches, serving; FP64 at one path per lane): a
+            // body per model and payoff
             const int mdl = q.model, pay = q.payoff;
             if (q.technique != MP_TECH_STANDARD) {
                 tile_body<R, SRC, PPL, -1, -1, -1>(L, tile, q, lane);
