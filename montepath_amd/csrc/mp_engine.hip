@@ -607,6 +607,7 @@ __global__ void setup_kernel(mp_dreq d, mp_dreq* dst, unsigned int* counter, int
 // are already discounted.
 __device__ void control_variate_summary(const double* tot, double n, double expectation,
                                         double* price, double* se) {
+    #pragma clang fp contract(off)  // the reference's summaries round every product (no FMA)
     const double payoff_mean = tot[0] / n;
     const double control_mean = tot[2] / n;
     const double control_var = (tot[3] / n) - (control_mean * control_mean);
@@ -681,6 +682,7 @@ __global__ __launch_bounds__(64 * MP_NS_MAX) void finalize_kernel(const mp_dreq*
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        #pragma clang fp contract(off)  // the reference's summaries round every product (no FMA)
         double* o = out + (uint64_t)blockIdx.x * MP_PARTIAL_STRIDE;
         for (int j = 0; j < MP_PARTIAL_STRIDE; ++j) o[j] = 0.0;
         if (moments) {
@@ -1768,6 +1770,7 @@ __global__ __launch_bounds__(256) void lsm_final_kernel(const mp_lsm_params* __r
     __shared__ double tot[4];
     reduce_partials<4>(partial, gridDim.x, 4, tot);
     if (threadIdx.x == 0) {
+        #pragma clang fp contract(off)  // the reference's summaries round every product (no FMA)
         int rs = 0;
         for (int s = 1; s <= P.n_steps; ++s) rs += valid_all[s] ? 1 : 0;
         const double n = (double)P.n_paths;
@@ -2008,6 +2011,7 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
     if (!publish_and_ticket<4>(v, fin_partial + (uint64_t)blockIdx.x * 4, ticket)) return;
     reduce_partials<4>(fin_partial, gridDim.x, 4, tot);
     if (threadIdx.x == 0) {
+        #pragma clang fp contract(off)  // the reference's summaries round every product (no FMA)
         int rs = 0;
         for (int s = 1; s <= m; ++s) rs += __hip_atomic_load(valid_all + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1 : 0;
         const double nn = (double)n;
