@@ -109,7 +109,9 @@ def test_ragged_lsm_on_reference_normals(engine, oracle, n, m):
 
 @pytest.mark.parametrize("bad", [dict(n_paths=0), dict(n_steps=0, kind="arithmetic_asian_call"),
                                  dict(sigma=-0.2), dict(sigma=0.0), dict(t=0.0), dict(s0=0.0),
-                                 dict(sigma=-0.1, kind="lookback_call")])
+                                 dict(sigma=-0.1, kind="lookback_call"),
+                                 # validate_american_put_config cpu.rs:5127-5135: n_steps >= 2
+                                 dict(n_steps=1, kind="american_put")])
 def test_invalid_requests_are_errors(engine, bad):
     kw = dict(bad)
     kind = kw.pop("kind", "european_call")
