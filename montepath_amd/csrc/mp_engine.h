@@ -132,7 +132,7 @@ struct tile_launch {
     int block;
     int ppl;                    // paths interleaved per lane
     int mlmc;                   // every request is a multilevel level (own kernel)
-    int claim;                  // queue positions per claim (launch_tiles sets it)
+    int claim;                  // queue positions per claim, 0 = one tile per wave (launch_tiles sets it)
 };
 
 hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s);

@@ -583,6 +583,20 @@ __global__ __launch_bounds__(256) void tile_kernel(mp::tile_launch L) {
     }
 }
 
+// The tiles of a launch by static assignment -- wave i takes positions i,
+// i + W, ... -- for a launch of one request (tiles of equal cost) or with a
+// wave per tile: the work queue's atomics, every wave hitting one counter at
+// the start, cost 10-25 us per launch.  A kernel of its own, so the queue
+// kernel's code is unchanged; one path per lane.
+template <class R, int SRC, bool ML>
+__global__ __launch_bounds__(256) void tile_static_kernel(mp::tile_launch L) {
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    const unsigned int waves = gridDim.x * (blockDim.x / MP_WAVE);
+    for (unsigned int qpos = blockIdx.x * (blockDim.x / MP_WAVE) + (threadIdx.x / MP_WAVE);
+         qpos < L.n_tiles; qpos += waves)
+        run_tile<R, SRC, 1, ML>(L, qpos, lane);
+}
+
 __global__ void reset_kernel(unsigned int* counter, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) counter[i] = 0u;
@@ -2137,6 +2151,13 @@ hipError_t launch_setup(const mp_dreq& d, mp_dreq* dst, unsigned int* counter, i
 
 template <class R, int SRC>
 static hipError_t tiles_ppl(const tile_launch& L, hipStream_t s) {
+    if (L.claim == 0) {
+        if (L.mlmc)
+            hipLaunchKernelGGL((tile_static_kernel<R, SRC, true>), dim3(L.grid), dim3(L.block), 0, s, L);
+        else
+            hipLaunchKernelGGL((tile_static_kernel<R, SRC, false>), dim3(L.grid), dim3(L.block), 0, s, L);
+        return hipGetLastError();
+    }
     if (L.mlmc) {
         hipLaunchKernelGGL((tile_kernel<R, SRC, 1, true>), dim3(L.grid), dim3(L.block), 0, s, L);
         return hipGetLastError();
@@ -2155,12 +2176,19 @@ static hipError_t tiles_ppl(const tile_launch& L, hipStream_t s) {
 }
 
 hipError_t launch_tiles(const tile_launch& L0, int precision, hipStream_t s) {
-    // Positions per claim: 1 while the queue holds fewer than 16 tiles per
-    // wave (every wave should get work), up to 4 on long queues, where the
-    // one queue counter's atomic rate would otherwise bound the launch.
+    // Positions per claim.  None -- tile_static_kernel, wave i taking
+    // positions i, i + W, ... -- when every tile has a wave of its own or all
+    // tiles belong to one request (equal cost), at one path per lane: the
+    // queue's atomics, all waves hitting one counter at the start, cost
+    // 10-25 us per launch.
+    // Otherwise 1 while the queue holds fewer than 16 tiles per wave (every
+    // wave should get work), up to 4 on long queues, where the counter's
+    // atomic rate would bound the launch.
     tile_launch L = L0;
     const uint64_t waves = (uint64_t)L.grid * (uint64_t)(L.block / MP_WAVE);
-    L.claim = (int)std::min<uint64_t>(4, std::max<uint64_t>(1, L.n_tiles / std::max<uint64_t>(1, 16 * waves)));
+    L.claim = (waves >= L.n_tiles || L.tile_req == nullptr) && (L.ppl == 1 || L.mlmc)
+                  ? 0
+                  : (int)std::min<uint64_t>(4, std::max<uint64_t>(1, L.n_tiles / std::max<uint64_t>(1, 16 * waves)));
     const bool buf = L.normals != nullptr;
     if (precision == 64) return buf ? tiles_ppl<double, 1>(L, s) : tiles_ppl<double, 0>(L, s);
     return buf ? tiles_ppl<float, 1>(L, s) : tiles_ppl<float, 0>(L, s);
