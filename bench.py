@@ -20,7 +20,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -30,10 +29,6 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-N_PATHS = 1 << 22
-N_STEPS = 256
-STRIKES = [80.0, 100.0, 120.0]
-HESTON = dict(s0=100.0, v0=0.04, kappa=2.0, theta=0.04, xi=0.5, rho=-0.7, r=0.03, t=1.0)
 # Algorithmic FP work per path-step of this workload: Heston full-truncation
 # step 20, plus per factor an Acklam inverse normal 12 and a bridge point 7.
 FLOP_PER_PATH_STEP = 20 + 2 * (12 + 7)
@@ -41,53 +36,70 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = "qmc_kernel_heston_sobol_bb_2p22x256_traffic.json"
 
+from montepath_amd.workloads import (HESTON, N_PATHS, N_STEPS, STRIKES,  # noqa: E402
+                                     batch_contracts, heston_request, serve_mix, serve_pilot)
+
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def heston_request(precision: int, n_paths: int = N_PATHS, n_steps: int = N_STEPS):
-    from montepath_amd.engine import make_request
+def host_cpu() -> dict:
+    """This host's core count and CPU model (as lscpu reports them)."""
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = None
+    try:
+        import subprocess
 
-    return make_request("heston_european_call", n_paths=n_paths, n_steps=n_steps, seed=42,
-                        precision=precision, heston_scheme="full_truncation",
-                        sampling="scrambled_sobol_brownian_bridge", strikes=STRIKES, **HESTON)
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, ValueError):
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return {"nproc": int(threads or 1), "os_cpu_count": os.cpu_count(), "model": model}
 
 
 def cpu_baseline(threads: int) -> dict:
     """The reference's CPU Heston path (xorshift64*/Box-Muller, floored Euler,
     std::thread chunks; runtime/cpu.rs:8632-8910) restated in C, timed on a
-    bounded sample of the workload's path-steps on this host's cores."""
+    bounded sample of the workload's path-steps on all of this host's cores
+    (one std::thread per core, as the reference's n_threads = 0), plus the
+    same sample's rate on one core."""
     from oracle import cpu as O
 
-    n, m = 1 << 21, N_STEPS
-    h = O.Heston(n_paths=n, n_steps=m, kappa=2.0, xi=0.5, rho=-0.7, n_threads=threads)
+    m = N_STEPS
+    # about 1 s of work on all cores: 2^21 paths for 16 cores, scaled by cores
+    n = max(1 << 16, min(1 << 24, (1 << 17) * threads))
     O.heston_price(O.Heston(n_paths=1 << 14, n_steps=m, n_threads=threads), 0)  # warm
+    runs = []
+    for _ in range(3):
+        h = O.Heston(n_paths=n, n_steps=m, kappa=2.0, xi=0.5, rho=-0.7, n_threads=threads)
+        t0 = time.perf_counter()
+        O.heston_price(h, O.HESTON_REFERENCE_FLOOR)
+        runs.append(time.perf_counter() - t0)
+    dt = float(np.median(runs))
+    n1 = 1 << 16
+    h1 = O.Heston(n_paths=n1, n_steps=m, kappa=2.0, xi=0.5, rho=-0.7, n_threads=1)
     t0 = time.perf_counter()
-    O.heston_price(h, O.HESTON_REFERENCE_FLOOR)
-    dt = time.perf_counter() - t0
+    O.heston_price(h1, O.HESTON_REFERENCE_FLOOR)
+    dt1 = time.perf_counter() - t0
+    host = host_cpu()
     return {"value": n * m / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
-            "sample": f"Heston floored Euler, pseudorandom, {n} paths x {m} steps "
-                      f"({dt:.2f} s wall on {threads} threads)"}
-
-
-def batch_contracts(n: int = 4096, seed: int = 7):
-    """4,096 contracts: half European, half arithmetic-average Asian calls under
-    GBM; K~U[80,120], sigma~U[0.1,0.6], T~U[0.1,3], r~U[0,0.05], paths
-    log-uniform in [2^12, 2^20], steps uniform in {16..365}; parameter draws
-    from one seeded host generator; Philox keyed by (seed 7, contract id)."""
-    from montepath_amd.engine import make_request
-
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for i in range(n):
-        reqs.append(make_request(
-            "european_call" if i < n // 2 else "arithmetic_asian_call",
-            n_paths=int(2 ** rng.uniform(12, 20)), n_steps=int(rng.integers(16, 366)),
-            k=float(rng.uniform(80, 120)), sigma=float(rng.uniform(0.1, 0.6)),
-            t=float(rng.uniform(0.1, 3.0)), r=float(rng.uniform(0.0, 0.05)),
-            seed=7, contract_id=i, precision=32))
-    return reqs
+            "per_core": n1 * m / dt1, "host": host,
+            "sample": f"Heston floored Euler, pseudorandom, {n} paths x {m} steps, median of 3 "
+                      f"({dt:.2f} s wall on {threads} threads, {host['model']}); per_core: "
+                      f"{n1} paths x {m} steps on one thread ({dt1:.2f} s)"}
 
 
 def bench_batch(eng, reps: int = 5) -> dict:
@@ -211,61 +223,6 @@ def bench_plain_c1(eng, threads: int, reps: int = 20) -> dict:
             "black_scholes": 9.413403383853016, "reference_m2_ms": 11.315}
 
 
-SERVE_KINDS = ("european_call", "arithmetic_asian_call", "heston_european_call",
-               "merton_jump_diffusion_call", "american_put")
-
-
-def serve_request(kind: str, k: float, paths: int, contract: int, precision: int = 32):
-    from montepath_amd.engine import make_request
-
-    common = dict(seed=11, contract_id=contract, precision=precision)
-    if kind in ("european_call", "arithmetic_asian_call"):
-        return make_request(kind, n_paths=paths, n_steps=64, k=k, **common)
-    if kind == "heston_european_call":
-        return make_request(kind, n_paths=paths, n_steps=64, k=k, heston_scheme="full_truncation",
-                            **HESTON, **common)
-    if kind == "merton_jump_diffusion_call":
-        return make_request(kind, n_paths=paths, n_steps=0, k=k, jump_intensity=1.0,
-                            jump_mean=-0.1, jump_volatility=0.15, sigma=0.2, **common)
-    return make_request("american_put", n_paths=paths, n_steps=50, k=k, basis_degree=3,
-                        r=0.05, sigma=0.25, **common)
-
-
-def serve_pilot(eng) -> dict:
-    """Coefficient of variation (payoff sd / price) per request kind and
-    strike, from 2^16-path pilots; interpolated in K."""
-    grid = np.linspace(80.0, 120.0, 9)
-    cv = {}
-    for kind in SERVE_KINDS:
-        vals = []
-        for k in grid:
-            r = eng.price(serve_request(kind, float(k), 1 << 16, 0))
-            vals.append(r.stderr * math.sqrt(1 << 16) / max(r.price, 1e-12))
-        cv[kind] = (grid, np.array(vals))
-    return cv
-
-
-def serve_mix(n: int, rate: float, cv: dict, seed: int = 11):
-    """Poisson arrivals of mixed requests -- 40% GBM European/Asian, 25% Heston,
-    20% Merton, 15% LSM American put; K~U[80,120] -- each with a target
-    standard error of 1e-3 to 1e-2 (log-uniform) relative to its price, which
-    the pilot's coefficient of variation turns into a path count clipped to
-    [1e4, 1e7]."""
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
-    reqs = []
-    for i in range(n):
-        u = rng.random()
-        kind = (SERVE_KINDS[0] if rng.random() < 0.5 else SERVE_KINDS[1]) if u < 0.40 else \
-            SERVE_KINDS[2] if u < 0.65 else SERVE_KINDS[3] if u < 0.85 else SERVE_KINDS[4]
-        k = float(rng.uniform(80, 120))
-        rel_se = 10 ** rng.uniform(-3, -2)
-        g, v = cv[kind]
-        paths = int(min(max((float(np.interp(k, g, v)) / rel_se) ** 2, 1e4), 1e7))
-        reqs.append(serve_request(kind, k, paths, i))
-    return reqs, arrivals
-
-
 def bench_serve(eng, seconds: float, rate: float) -> dict:
     n = int(seconds * rate)
     cv = serve_pilot(eng)
@@ -313,7 +270,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip batch and serving")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--only", choices=["c1", "batch", "lsm", "basket", "serve"], default=None,
                     help="run one secondary section by itself (for profiling) and print its JSON")
@@ -334,7 +291,8 @@ def main() -> int:
     info = eng.device_info()
     log(f"[bench] {info['device_id']} {info['name']} {info['arch']} CUs={info['compute_units']}")
     if args.only:
-        section = {"c1": lambda: bench_plain_c1(eng, 16), "batch": lambda: bench_batch(eng),
+        section = {"c1": lambda: bench_plain_c1(eng, host_cpu()["nproc"]),
+                   "batch": lambda: bench_batch(eng),
                    "lsm": lambda: bench_lsm(eng),
                    "basket": lambda: bench_basket(eng),
                    "serve": lambda: bench_serve(eng, args.serve_seconds, args.serve_rate)}[args.only]()
@@ -381,7 +339,7 @@ def main() -> int:
                    "n_paths": N_PATHS, "n_steps": N_STEPS, "sobol_dims": 2 * N_STEPS,
                    "strikes": STRIKES, "precision": "fp32_paths_fp64_kahan_sums",
                    **HESTON},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_source": f"profiles/{TRAFFIC_FILE}" if traffic else None,
@@ -402,15 +360,13 @@ def main() -> int:
         "characteristic_function": cf_prices,
         "error_in_se": [(p - c) / s for p, c, s in zip(res.strike_prices, cf_prices, res.strike_stderrs)],
     }
+    threads = host_cpu()["nproc"]
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = max(1, min(threads, 16))
         out["cpu_baseline"] = cpu_baseline(threads)
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3e} path-steps/s")
     if not args.no_extra:
         extra = {}
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        extra["plain_european_100k_x_64_fp64"] = bench_plain_c1(eng, max(1, min(threads, 16)))
+        extra["plain_european_100k_x_64_fp64"] = bench_plain_c1(eng, threads)
         log(f"[bench] c1 {extra['plain_european_100k_x_64_fp64']}")
         extra["batched_4096_contracts"] = bench_batch(eng)
         log(f"[bench] batch {extra['batched_4096_contracts']}")

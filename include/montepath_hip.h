@@ -429,7 +429,15 @@ int mp_hip_request_footprint(const mp_request* req, uint64_t* bytes);
  * Requests are admitted by HBM footprint as slots free up; admitted work is
  * packed into persistent launches on up to `max_inflight` concurrent
  * streams while later requests keep arriving.  No reference counterpart
- * (the reference is synchronous; production.py:440-481). */
+ * (the reference is synchronous; production.py:440-481).
+ *
+ * Admission: every request, the first of a launch included, is admitted
+ * only while the device bytes the workers hold in their arenas plus the
+ * reservations of the launches in flight (footprint + 25% + 1 MiB each) stay
+ * within hbm_budget_bytes.  When the oldest pending request does not fit,
+ * idle workers free their arenas and the request waits for running launches
+ * to finish; a request whose reservation alone exceeds the budget is
+ * rejected by mp_hip_submit with MP_E_OOM. */
 typedef struct mp_serve_config {
     uint64_t hbm_budget_bytes;   /* admission budget (0: 75% of free memory) */
     int32_t max_inflight;        /* concurrent launches in flight (streams), 1..8 */
@@ -466,6 +474,12 @@ typedef struct mp_serve_stats {
 int mp_hip_serve_trace(mp_ctx* ctx, const mp_serve_config* cfg, const mp_request* reqs,
                        const double* arrival_s, size_t n, mp_result* results,
                        double* latency_ms, mp_serve_stats* stats);
+
+/* Counters of a live server: requests completed, launches, and the largest
+ * number of device bytes held or reserved by its workers at any admission
+ * or completion (max_admitted_bytes; never above hbm_budget_bytes).  The
+ * latency and rate fields are left 0. */
+int mp_hip_server_stats(mp_server* srv, mp_serve_stats* stats);
 
 /* ---- tuning --------------------------------------------------------------
  * Kernel launch parameters: block size, paths interleaved per lane, paths

@@ -241,6 +241,7 @@ SIGNATURES = {
     "mp_hip_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(Result), C.POINTER(C.c_int)]),
     "mp_hip_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(Result)]),
     "mp_hip_cancel": (C.c_int, [_P, C.c_uint64]),
+    "mp_hip_server_stats": (C.c_int, [_P, C.POINTER(ServeStats)]),
     "mp_hip_serve_trace": (C.c_int, [_P, C.POINTER(ServeConfig), C.POINTER(Request), _D,
                                      C.c_size_t, C.POINTER(Result), _D, C.POINTER(ServeStats)]),
     "mp_hip_autotune": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Tuning), _D]),

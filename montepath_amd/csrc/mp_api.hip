@@ -90,6 +90,11 @@ struct dbuf {
         cap = want;
         return MP_OK;
     }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
     template <class T>
     T* as() const {
         return static_cast<T*>(p);
@@ -1441,6 +1446,44 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
     return MP_OK;
 }
 
+namespace {
+
+// Device bytes a context's arenas hold (its reusable batches included; the
+// Sobol index tables and pinned host buffers are not counted).
+uint64_t ctx_device_bytes(const mp_ctx* c) {
+    const dbuf* bufs[] = {&c->reqs,      &c->partial,   &c->counter,  &c->normals,   &c->uniforms,
+                          &c->jumps,     &c->term,      &c->pay,      &c->plan,      &c->basket_par,
+                          &c->lsm_paths, &c->lsm_cash,  &c->lsm_ex,   &c->lsm_coeff, &c->lsm_valid,
+                          &c->lsm_gram,  &c->lsm_par,   &c->lsm_cnt,  &c->lsm_part,  &c->lsm_dates,
+                          &c->plain_pay, &c->draws,     &c->sampler};
+    uint64_t b = 0;
+    for (const dbuf* d : bufs) b += d->cap;
+    for (const mp_batch* bt : c->bat)
+        if (bt)
+            b += bt->d_reqs.cap + bt->d_tile_req.cap + bt->d_order.cap + bt->d_partial.cap +
+                 bt->d_counter.cap + bt->d_sampler.cap;
+    return b;
+}
+
+// Frees a context's arenas and reusable batches (after its stream has
+// drained); the next pricing allocates what it needs again.
+void ctx_trim(mp_ctx* c) {
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dbuf* bufs[] = {&c->reqs,      &c->partial,  &c->counter,  &c->normals,   &c->uniforms,
+                    &c->jumps,     &c->term,     &c->pay,      &c->plan,      &c->basket_par,
+                    &c->lsm_paths, &c->lsm_cash, &c->lsm_ex,   &c->lsm_coeff, &c->lsm_valid,
+                    &c->lsm_gram,  &c->lsm_par,  &c->lsm_cnt,  &c->lsm_part,  &c->lsm_dates,
+                    &c->plain_pay, &c->draws,    &c->sampler};
+    for (dbuf* d : bufs) d->release();
+    for (auto& b : c->bat) {
+        delete b;
+        b = nullptr;
+    }
+}
+
+}  // namespace
+
 MP_EXPORT void mp_hip_close(mp_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
@@ -2508,54 +2551,133 @@ struct mp_server {
     mp_ctx* ctx = nullptr;
     mp_serve_config cfg{};
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_start;
     std::deque<uint64_t> pending;
     std::unordered_map<uint64_t, std::unique_ptr<ticket_state>> tickets;
     uint64_t next_ticket = 1;
-    uint64_t admitted_bytes = 0;
+    // per worker: device bytes its context's arenas hold after its last
+    // pricing, and the bytes reserved for the batch it is pricing now
+    struct slot_bytes {
+        uint64_t held = 0, reserved = 0;
+    };
+    std::vector<slot_bytes> slots;
     uint64_t max_admitted = 0;
     uint64_t launches = 0;
+    uint64_t completed = 0;
+    uint64_t trims = 0;
+    bool trim_wanted = false;  // a request waits for memory that idle workers hold
     bool stop = false;
+    int started = 0, start_failed = 0;
     std::vector<std::thread> workers;
 };
 
 namespace {
 
-// A worker owns one stream (one in-flight launch slot).  It takes every
-// pending work-queue request that fits the HBM budget, packs them into one
-// persistent launch, and completes them; requests that arrive meanwhile wait
-// for the next free slot.
+// Bytes a request may add to a worker's arenas: its footprint, the arenas'
+// 25% growth slack and the small per-call buffers.
+uint64_t reserve_bytes(uint64_t footprint) { return footprint + footprint / 4 + (1u << 20); }
+
+// Device bytes the server's workers hold or have reserved, except worker
+// `skip` (-1: all of them).
+uint64_t server_bytes(const mp_server* s, int skip) {
+    uint64_t b = 0;
+    for (size_t v = 0; v < s->slots.size(); ++v)
+        if ((int)v != skip) b += s->slots[v].held + s->slots[v].reserved;
+    return b;
+}
+
+// A worker owns one context and stream (one in-flight launch slot).  It takes
+// pending requests in arrival order while the server's device bytes -- every
+// worker's arenas plus the reservations of the batches in flight -- stay
+// within the HBM budget, the first request of a batch included, packs the
+// work-queue requests into one persistent launch and prices the others one by
+// one; requests that arrive meanwhile wait for the next free worker.  When
+// the head request does not fit, the worker frees its own arenas if that
+// makes room, and otherwise asks idle workers to free theirs and waits for
+// running batches to finish.  A request over the budget on its own is
+// rejected at submission, so the head request always fits eventually.
 void server_worker(mp_server* s, int slot) {
     (void)hipSetDevice(s->ctx->device);
     mp_ctx* own = nullptr;
-    if (mp_hip_open(s->ctx->device, &own) != MP_OK) return;
+    const int orc = mp_hip_open(s->ctx->device, &own);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (orc == MP_OK) {
+            s->started += 1;
+            s->slots[slot].held = ctx_device_bytes(own);
+        } else {
+            s->start_failed += 1;
+        }
+    }
+    s->cv_start.notify_all();
+    if (orc != MP_OK) return;
     // other workers' launches share the device, so a one-launch LSM pricing
     // (which needs its whole grid resident) could wait on them: price LSM
     // requests with the matched per-date launches, which give the same bits
     own->lsm_fused = false;
+    const uint64_t budget = s->cfg.hbm_budget_bytes;
     for (;;) {
         std::vector<uint64_t> take;
         uint64_t bytes = 0;
+        bool trim_self = false, quit = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
-            if (s->stop && s->pending.empty()) break;
-            while (!s->pending.empty() && (int)take.size() < s->cfg.max_batch) {
-                const uint64_t id = s->pending.front();
-                auto it = s->tickets.find(id);
-                if (it == s->tickets.end()) {
-                    s->pending.pop_front();
-                    continue;
+            for (;;) {
+                if (s->stop && s->pending.empty()) {
+                    quit = true;
+                    break;
                 }
-                const uint64_t fp = it->second->footprint;
-                if (!take.empty() && s->admitted_bytes + fp > s->cfg.hbm_budget_bytes) break;
-                s->pending.pop_front();
-                s->admitted_bytes += fp;
-                bytes += fp;
-                take.push_back(id);
+                if (s->trim_wanted && s->slots[slot].held > 0) {
+                    trim_self = true;  // an idle worker gives its arenas back
+                    break;
+                }
+                uint64_t head_fp = 0;
+                while (!s->pending.empty() && (int)take.size() < s->cfg.max_batch) {
+                    const uint64_t id = s->pending.front();
+                    auto it = s->tickets.find(id);
+                    if (it == s->tickets.end()) {
+                        s->pending.pop_front();
+                        continue;
+                    }
+                    const uint64_t rb = reserve_bytes(it->second->footprint);
+                    if (server_bytes(s, slot) + s->slots[slot].held + bytes + rb > budget) {
+                        head_fp = rb;
+                        break;
+                    }
+                    s->pending.pop_front();
+                    bytes += rb;
+                    take.push_back(id);
+                }
+                if (!take.empty()) break;
+                if (head_fp) {
+                    if (s->slots[slot].held > 0 && server_bytes(s, slot) + head_fp <= budget) {
+                        trim_self = true;  // this worker's own arenas are in the way
+                        break;
+                    }
+                    s->trim_wanted = true;
+                    s->cv_work.notify_all();
+                }
+                s->cv_work.wait(lk);
             }
-            s->max_admitted = std::max(s->max_admitted, s->admitted_bytes);
-            s->launches += 1;
+            if (!take.empty()) {
+                s->slots[slot].reserved = bytes;
+                s->max_admitted = std::max(s->max_admitted, server_bytes(s, -1));
+                s->launches += 1;
+            }
+        }
+        if (quit) break;
+        if (trim_self) {
+            ctx_trim(own);
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->slots[slot].held = ctx_device_bytes(own);
+                s->trims += 1;
+                bool idle_holding = false;
+                for (const auto& v : s->slots) idle_holding |= (v.reserved == 0 && v.held > 0);
+                if (!idle_holding) s->trim_wanted = false;
+            }
+            s->cv_work.notify_all();
+            continue;
         }
         // split into work-queue requests (one persistent launch) and the rest
         std::vector<mp_request> wq;
@@ -2585,6 +2707,7 @@ void server_worker(mp_server* s, int slot) {
             }
             rrc[i] = mp_hip_price(own, &rq, &rres[i]);
         }
+        const uint64_t held = ctx_device_bytes(own);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (size_t i = 0; i < wq_ids.size(); ++i) {
@@ -2599,11 +2722,14 @@ void server_worker(mp_server* s, int slot) {
                 if (rrc[i] != MP_OK) t->res.status = rrc[i];
                 t->done = 1;
             }
-            s->admitted_bytes -= bytes;
+            s->completed += take.size();
+            // the batch's allocations are now part of the arenas
+            s->slots[slot].held = held;
+            s->slots[slot].reserved = 0;
+            s->max_admitted = std::max(s->max_admitted, server_bytes(s, -1));
         }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
-        (void)slot;
     }
     mp_hip_close(own);
 }
@@ -2624,8 +2750,31 @@ MP_EXPORT int mp_hip_server_open(mp_ctx* c, const mp_serve_config* cfg, mp_serve
         HIPCHK(hipMemGetInfo(&fr, &tot));
         s->cfg.hbm_budget_bytes = (uint64_t)(fr * 0.75);
     }
+    s->slots.resize(s->cfg.max_inflight);
     for (int i = 0; i < s->cfg.max_inflight; ++i) s->workers.emplace_back(server_worker, s.get(), i);
+    int failed = 0;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_start.wait(lk, [&] { return s->started + s->start_failed == s->cfg.max_inflight; });
+        failed = s->start_failed;
+        if (failed) s->stop = true;
+    }
+    if (failed) {
+        s->cv_work.notify_all();
+        for (auto& t : s->workers) t.join();
+        return fail(MP_E_HIP, "a server worker could not open its device context");
+    }
     *out = s.release();
+    return MP_OK;
+}
+
+MP_EXPORT int mp_hip_server_stats(mp_server* s, mp_serve_stats* stats) {
+    if (!s || !stats) return fail(MP_E_INVALID_CONFIG, "null argument");
+    std::memset(stats, 0, sizeof(*stats));
+    std::lock_guard<std::mutex> lk(s->mu);
+    stats->completed = s->completed;
+    stats->launches = s->launches;
+    stats->max_admitted_bytes = s->max_admitted;
     return MP_OK;
 }
 
@@ -2650,7 +2799,7 @@ MP_EXPORT int mp_hip_submit(mp_server* s, const mp_request* req, uint64_t* ticke
     }
     int rc = mp_hip_request_footprint(&t->req, &t->footprint);
     if (rc != MP_OK) return rc;
-    if (t->footprint > s->cfg.hbm_budget_bytes)
+    if (reserve_bytes(t->footprint) > s->cfg.hbm_budget_bytes)
         return fail(MP_E_OOM, "request footprint exceeds the server's HBM budget");
     std::memset(&t->res, 0, sizeof(t->res));
     {
@@ -2712,7 +2861,7 @@ MP_EXPORT int mp_hip_serve_trace(mp_ctx* c, const mp_serve_config* cfg, const mp
     if (rc != MP_OK) return rc;
     std::vector<uint64_t> tickets(n);
     std::vector<std::chrono::steady_clock::time_point> t_arr(n);
-    std::vector<int> got(n, 0);
+    std::vector<size_t> live;  // submitted, not yet completed (polled in submission order)
     const auto start = std::chrono::steady_clock::now();
     size_t next = 0, finished = 0;
     uint64_t path_steps = 0;
@@ -2726,10 +2875,12 @@ MP_EXPORT int mp_hip_serve_trace(mp_ctx* c, const mp_serve_config* cfg, const mp
                 mp_hip_server_close(s);
                 return rc;
             }
+            live.push_back(next);
             ++next;
         }
-        for (size_t i = 0; i < next; ++i) {
-            if (got[i]) continue;
+        size_t keep = 0;
+        for (size_t j = 0; j < live.size(); ++j) {
+            const size_t i = live[j];
             int done = 0;
             rc = mp_hip_poll(s, tickets[i], &results[i], &done);
             if (rc != MP_OK) {
@@ -2737,13 +2888,15 @@ MP_EXPORT int mp_hip_serve_trace(mp_ctx* c, const mp_serve_config* cfg, const mp
                 return rc;
             }
             if (done) {
-                got[i] = 1;
                 ++finished;
                 latency_ms[i] = std::chrono::duration<double, std::milli>(
                                     std::chrono::steady_clock::now() - t_arr[i]).count();
                 path_steps += reqs[i].n_paths * std::max<uint64_t>(results[i].n_steps, 1);
+            } else {
+                live[keep++] = i;
             }
         }
+        live.resize(keep);
         if (finished < n) std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();

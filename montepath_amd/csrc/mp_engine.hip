@@ -1889,6 +1889,11 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
     __shared__ double tot[15], gsum[15];
     __shared__ double beta_s[4];
     __shared__ int valid_s;
+    // Lanes past the last path take no part in the regression.  (A padded
+    // price of R(k) is not enough: in FP32, k - float(k) can be positive.)
+    bool live[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) live[j] = p0 + (uint64_t)j * T < n;
     // prefetch the prices of the first regression date
     R sd[J];
     if (n_reg > 0) {
@@ -1896,7 +1901,7 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const uint64_t p = p0 + (uint64_t)j * T;
-            sd[j] = p < n ? pd[p] : R(k);
+            sd[j] = live[j] ? pd[p] : R(0);
         }
     }
     for (int i = 0; i < n_reg; ++i) {
@@ -1909,7 +1914,7 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
         for (int j = 0; j < J; ++j) {
             const double s = (double)sd[j];
             const double imm = k - s > 0.0 ? k - s : 0.0;
-            if (imm > 0.0) {  // false for p >= n (sd = k)
+            if (live[j] && imm > 0.0) {
                 double bb[4];
                 laguerre(s * inv_k, deg, bb);
                 const double cont = cash[j] * grow;
@@ -1974,7 +1979,7 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
 #pragma unroll
             for (int j = 0; j < J; ++j) {
                 const uint64_t p = p0 + (uint64_t)j * T;
-                sn[j] = p < n ? pn[p] : R(k);
+                sn[j] = live[j] ? pn[p] : R(0);
             }
         }
         __syncthreads();
@@ -1987,7 +1992,7 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
             for (int j = 0; j < J; ++j) {
                 const double s = (double)sd[j];
                 const double imm = k - s > 0.0 ? k - s : 0.0;
-                if (imm > 0.0) {
+                if (live[j] && imm > 0.0) {
                     double bb[4];
                     laguerre(s * inv_k, deg, bb);
                     double cont = 0.0;

@@ -278,6 +278,12 @@ class Engine:
         A.check(self.lib.mp_hip_price(self._ctx, C.byref(req), C.byref(res)))
         return PriceResult.of(res)
 
+    def footprint(self, req: A.Request) -> int:
+        """HBM bytes the request needs while in flight."""
+        b = C.c_uint64()
+        A.check(self.lib.mp_hip_request_footprint(C.byref(req), C.byref(b)))
+        return b.value
+
     def price_batch(self, reqs: list) -> list:
         n = len(reqs)
         arr = (A.Request * n)(*reqs)
@@ -443,6 +449,47 @@ class Engine:
                                             _pd(lat), C.byref(st)))
         stats = {f: getattr(st, f) for f, _ in A.ServeStats._fields_}
         return [PriceResult.of(r) for r in res], lat, stats
+
+    def server(self, hbm_budget_bytes: int = 0, max_inflight: int = 2,
+               max_batch: int = 4096) -> "Server":
+        return Server(self, hbm_budget_bytes, max_inflight, max_batch)
+
+
+class Server:
+    """A live continuous-batching server on the engine's device."""
+
+    def __init__(self, eng: Engine, hbm_budget_bytes: int, max_inflight: int, max_batch: int):
+        self.lib = eng.lib
+        self._s = C.c_void_p()
+        cfg = A.ServeConfig(hbm_budget_bytes, max_inflight, max_batch)
+        A.check(self.lib.mp_hip_server_open(eng._ctx, C.byref(cfg), C.byref(self._s)))
+
+    def submit(self, req: A.Request) -> int:
+        t = C.c_uint64()
+        A.check(self.lib.mp_hip_submit(self._s, C.byref(req), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> "PriceResult":
+        out = A.Result()
+        A.check(self.lib.mp_hip_wait(self._s, ticket, C.byref(out)))
+        return PriceResult.of(out)
+
+    def stats(self) -> dict:
+        st = A.ServeStats()
+        A.check(self.lib.mp_hip_server_stats(self._s, C.byref(st)))
+        return {"completed": st.completed, "launches": st.launches,
+                "max_admitted_bytes": st.max_admitted_bytes}
+
+    def close(self) -> None:
+        if self._s:
+            self.lib.mp_hip_server_close(self._s)
+            self._s = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class PreparedBatch:

@@ -76,6 +76,13 @@ ROWS = {
     # scrambled-Sobol rows: sobol_burley 0.5 bits, not reproducible without the crate
     "mc_cpu_european_call_rust_scrambled_sobol": ("gbm_qmc", {"bridge": False}),  # :3356
     "mc_cpu_european_call_rust_scrambled_sobol_brownian_bridge": ("gbm_qmc", {"bridge": True}),
+    "mc_cpu_arithmetic_asian_call_rust_scrambled_sobol_brownian_bridge": (
+        "gbm_qmc", {"bridge": True, "payoff": "arithmetic_asian_call"}),  # :218-223
+    "mc_cpu_down_and_out_call_rust_scrambled_sobol_brownian_bridge": (
+        "gbm_qmc", {"bridge": True, "payoff": "down_and_out_call", "barrier": 80.0}),  # :265-270
+    "mc_cpu_basket_call_rust_scrambled_sobol": (
+        "basket2_structured", {"s01": 100.0, "s02": 95.0, "sigma1": 0.2, "sigma2": 0.25, "rho": 0.35,
+                               "weight1": 0.5, "weight2": 0.5, "sampling": "scrambled_sobol"}),  # :310-315
 }
 
 # stderr(technique) / stderr(standard) on one seed (harness.rs:2712-2891, :3607-3641)

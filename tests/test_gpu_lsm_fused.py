@@ -54,6 +54,12 @@ CASES = [
     ("american_put", 300_000, 50, 64, {}),
     ("american_put", 5_000, 16, 32, {}),
     ("bermudan_put", 100_000, 64, 64, {"exercise_steps": [8, 16, 24, 32, 40, 48, 56, 64]}),
+    # FP32 with path counts that leave idle lanes and strikes that FP32 does
+    # not represent (float(k) < k): idle lanes must stay out of the regression
+    ("american_put", 10_000, 50, 32, {"k": 116.10130597591788}),
+    ("american_put", 22_725, 50, 32, {"k": 119.84186}),
+    ("american_put", 1_189_060, 50, 32, {"k": 84.8445}),
+    ("american_put", 153_242, 50, 32, {"k": 97.218}),
 ]
 
 
@@ -61,7 +67,7 @@ CASES = [
 def test_one_launch_equals_matched_per_date_launches(engine, matched_engine, plain_engine, kind, n, m,
                                                      prec, extra):
     print(f"[lsm] {kind} n={n} m={m} fp{prec}", flush=True)
-    req = make_request(kind, n_paths=n, n_steps=m, precision=prec, seed=5, **LSM, **extra)
+    req = make_request(kind, n_paths=n, n_steps=m, precision=prec, seed=5, **{**LSM, **extra})
     a = engine.price(req)
     launches_a = engine.last_timing()["main_launches"]
     b = matched_engine.price(req)

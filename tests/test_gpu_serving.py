@@ -52,6 +52,33 @@ def test_served_results_equal_results_alone(engine):
     assert np.all(lat > 0)
 
 
+def test_admission_keeps_device_bytes_within_budget(engine):
+    """Two workers, a budget B and American puts whose reservations are about
+    0.6 B each: no two of them may be on the device together, the bytes the
+    workers hold or reserve never exceed B, and every result equals the same
+    request priced alone."""
+    fp = engine.footprint(make_request("american_put", n_paths=1 << 18, n_steps=50))
+    budget = int((fp + fp // 4 + (1 << 20)) / 0.6)
+    reqs = []
+    for i in range(12):
+        if i % 3 == 2:
+            reqs.append(make_request("european_call", n_paths=1 << 14, n_steps=32, seed=5,
+                                     contract_id=i))
+        else:
+            n = (1 << 18) - 4096 * (i % 4)
+            reqs.append(make_request("american_put", n_paths=n, n_steps=50, r=0.05, sigma=0.25,
+                                     basis_degree=3, seed=5, contract_id=i))
+    alone = [engine.price(r) for r in reqs]
+    with engine.server(hbm_budget_bytes=budget, max_inflight=2, max_batch=4) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        served = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(reqs)
+    assert 0 < st["max_admitted_bytes"] <= budget, (st, budget)
+    for a, b in zip(alone, served):
+        assert a.price == b.price and a.stderr == b.stderr
+
+
 def test_budget_admission_and_errors():
     from montepath_amd.engine import Engine
 
@@ -60,7 +87,7 @@ def test_budget_admission_and_errors():
     eng = Engine(0)
     lib = eng.lib
     srv = C.c_void_p()
-    cfg = A.ServeConfig(1 << 20, 1, 4)  # 1 MiB budget
+    cfg = A.ServeConfig(4 << 20, 1, 4)  # 4 MiB budget
     A.check(lib.mp_hip_server_open(eng._ctx, C.byref(cfg), C.byref(srv)))
     try:
         big = make_request("american_put", n_paths=1 << 20, n_steps=50)
