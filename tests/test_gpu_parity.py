@@ -110,10 +110,8 @@ def test_heston_paths_from_reference_normals(engine, oracle, scheme, precision):
     if precision == 64:
         assert float(err.max()) <= 1e-12
     else:
-        # FP32 variance recursion near zero variance is path dependent: bound
-        # the bulk at 1e-5 and the worst path at 1e-4
-        assert float(np.quantile(err, 0.999)) <= 1e-5
-        assert float(err.max()) <= 1e-4
+        # every path within 1e-5 relative (measured worst: 3.1e-7)
+        assert float(err.max()) <= 1e-5, float(err.max())
 
 
 # ------------------------------------------------------------- Merton
@@ -218,8 +216,9 @@ def test_heston_qmc_bridge_terminals_match_oracle(engine, oracle, precision):
     if precision == 64:
         assert float(err.max()) <= 1e-12
     else:
-        assert float(np.quantile(err, 0.999)) <= 1e-5
-        assert float(err.max()) <= 1e-4
+        # every path within 1e-5 relative (measured worst: 1.3e-6; the FP32
+        # run also draws its normals in FP32)
+        assert float(err.max()) <= 1e-5, float(err.max())
         req.precision = 64
         ref64 = engine.price(req)
         assert abs(res.price - ref64.price) <= 0.05 * ref64.stderr
