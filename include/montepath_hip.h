@@ -296,6 +296,24 @@ int mp_hip_device_info(mp_ctx* ctx, mp_device_info* out);
  * the CPU). */
 int mp_hip_price(mp_ctx* ctx, const mp_request* req, mp_result* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code that must not call hipMalloc, such as work recorded under HIP
+ * graph capture or tight serving loops): mp_hip_price for one request of the
+ * GBM family (European, Asian, down-and-out, lookback; stepwise or terminal),
+ * Heston or Merton with pseudorandom (Philox) draws, any technique, with the
+ * same result bits.  Its device buffers -- the request record, the queue
+ * counter and the per-tile FP64 partial sums -- live in a static device
+ * array of MP_NOALLOC_SCRATCH_BYTES per device, set up by mp_hip_open, so the
+ * call allocates nothing (the context's pinned result row receives the
+ * result).  A request needing more than MP_NOALLOC_SCRATCH_BYTES (under 1 KiB
+ * plus 64 bytes per path tile: about 131,000 tiles, e.g. 3e7 paths x 365
+ * steps) fails with MP_E_OOM; Sobol, Halton/LHS, basket, American/Bermudan and
+ * multilevel requests fail with MP_E_UNSUPPORTED.  Every call on a device
+ * uses that device's one array: calls on one device from several threads of
+ * the process are serialized by the library. */
+#define MP_NOALLOC_SCRATCH_BYTES (8u << 20)
+int mp_hip_price_noalloc(mp_ctx* ctx, const mp_request* req, mp_result* out);
+
 /* Price n independent requests in one persistent launch (work queue of
  * (request, path-tile) items); results are bit-identical to pricing each
  * request alone.  No reference counterpart: the reference executes one

@@ -206,6 +206,7 @@ SIGNATURES = {
     "mp_hip_close": (None, [_P]),
     "mp_hip_device_info": (C.c_int, [_P, C.POINTER(DeviceInfo)]),
     "mp_hip_price": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Result)]),
+    "mp_hip_price_noalloc": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Result)]),
     "mp_hip_price_batch": (C.c_int, [_P, C.POINTER(Request), C.c_size_t, C.POINTER(Result)]),
     "mp_hip_batch_prepare": (C.c_int, [_P, C.POINTER(Request), C.c_size_t, C.c_int,
                                        C.POINTER(_P)]),

@@ -33,6 +33,14 @@
 
 #define MP_EXPORT extern "C" __attribute__((visibility("default")))
 
+// Device scratch of the allocation-free one-request pricing
+// (mp_hip_price_noalloc): the request record, the queue counter and the
+// per-tile FP64 partial sums.  Each device holds its own copy of the array
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (g_scratch_ptr), never
+// through the array's name.
+static __device__ __align__(256) unsigned char scratch[MP_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -48,6 +56,27 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                            \
             return fail(MP_E_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
     } while (0)
+
+// Device address of `scratch` per device ordinal (0..63), resolved once per
+// device by scratch_setup with that device current; calls that price into a
+// device's scratch hold that device's g_scratch_call_mu.
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+std::mutex g_scratch_call_mu[64];
+
+int scratch_setup(int device) {
+    if (device < 0 || device >= 64) return fail(MP_E_UNSUPPORTED, "device ordinals above 63 are not supported");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MP_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        return fail(MP_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = p;
+    return MP_OK;
+}
 
 // compiler-rt __powidf2: the lowering of Rust's f64::powi used by the reference
 double powi_ref(double a, int b) {
@@ -1283,9 +1312,17 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     return MP_OK;
 }
 
+// Device buffers of a one-request launch given by the caller instead of the
+// context's arenas (the allocation-free pricing's scratch).
+struct tile_bufs {
+    mp_dreq* reqs;
+    unsigned int* counter;
+    double* partial;
+};
+
 int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, const double* normals,
                      const double* uniforms, const double* jumps, double* term_host,
-                     double* pay_host) {
+                     double* pay_host, const tile_bufs* fixed = nullptr) {
     c->stage_off = 0;  // the previous call has synchronised
     mp_dreq d = P.d;
     d.tile0 = 0;
@@ -1293,9 +1330,14 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     const uint64_t n = d.n_paths;
     const int F = d.model == MP_MODEL_HESTON ? 2 : 1;
     const size_t nn = (size_t)n * d.n_steps * F;
-    ENSURE(c->reqs, sizeof(mp_dreq));
-    ENSURE(c->partial, (size_t)d.n_tiles * MP_PARTIAL_STRIDE * 8);
-    ENSURE(c->counter, 64);
+    if (!fixed) {
+        ENSURE(c->reqs, sizeof(mp_dreq));
+        ENSURE(c->partial, (size_t)d.n_tiles * MP_PARTIAL_STRIDE * 8);
+        ENSURE(c->counter, 64);
+    }
+    mp_dreq* d_reqs = fixed ? fixed->reqs : c->reqs.as<mp_dreq>();
+    unsigned int* d_counter = fixed ? fixed->counter : c->counter.as<unsigned int>();
+    double* d_partial = fixed ? fixed->partial : c->partial.as<double>();
     const bool want_paths = term_host != nullptr || pay_host != nullptr;
     if (want_paths) {
         ENSURE(c->term, n * 8);
@@ -1323,12 +1365,12 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     }
     const mp_tuning tu = tuning_for(c, d, precision);
     mp::tile_launch L{};
-    L.reqs = c->reqs.as<mp_dreq>();
+    L.reqs = d_reqs;
     L.tile_req = nullptr;  // one request: every tile is request 0's
     L.order = nullptr;
     L.n_tiles = d.n_tiles;
-    L.counter = c->counter.as<unsigned int>();
-    L.partial = c->partial.as<double>();
+    L.counter = d_counter;
+    L.partial = d_partial;
     L.normals = buffer ? c->normals.as<double>() : nullptr;
     L.uniforms = buffer && d.model == MP_MODEL_MERTON ? c->uniforms.as<double>() : nullptr;
     L.jump_normals = buffer && d.model == MP_MODEL_MERTON ? c->jumps.as<double>() : nullptr;
@@ -1340,7 +1382,7 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     L.block = tu.block_size > 0 ? tu.block_size : 256;
     L.ppl = tu.paths_per_lane > 0 && d.payoff != MP_PAY_ASIAN_MLMC ? tu.paths_per_lane : 1;
     L.grid = grid_for_tiles(c, d.n_tiles, L.block);
-    HIPCHK(mp::launch_setup(d, c->reqs.as<mp_dreq>(), L.counter, 1, c->stream));
+    HIPCHK(mp::launch_setup(d, d_reqs, L.counter, 1, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(mp::launch_tiles(L, precision, c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
@@ -1418,6 +1460,7 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
     mp_ctx* c = guard.c;
     c->device = device_ordinal;
     HIPCHK(hipSetDevice(device_ordinal));
+    RETCHK(scratch_setup(device_ordinal));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device_ordinal));
     c->n_cu = prop.multiProcessorCount;
@@ -1529,6 +1572,57 @@ MP_EXPORT int mp_hip_price(mp_ctx* c, const mp_request* req, mp_result* out) {
         mp_result ref;
         const mp_timing keep = c->last;
         rc = price_one(c, *req, 64, &ref);
+        c->last = keep;
+        if (rc != MP_OK) return rc;
+        out->fp64_ref_error = std::fabs(out->price - ref.price);
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    c->last.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return MP_OK;
+}
+
+// mp_hip_price for one work-queue request with Philox draws, priced into the
+// device's static scratch: the request record at offset 0, the queue counter
+// and then the tile partials after it (256-byte aligned).  The set-up kernel writes the record
+// and clears the counter, and every tile writes its partial slot before the
+// finalize kernel reads it, so nothing a previous call left in the scratch is
+// read.  Calls on one device are serialized by g_scratch_call_mu.
+MP_EXPORT int mp_hip_price_noalloc(mp_ctx* c, const mp_request* req, mp_result* out) {
+    if (!c || !req || !out) return fail(MP_E_INVALID_CONFIG, "null argument");
+    if (c->device < 0 || c->device >= 64 || g_scratch_ptr[c->device] == nullptr)
+        return fail(MP_E_HIP, "the device scratch of this context's device is not set up");
+    HIPCHK(hipSetDevice(c->device));
+    prepared P;
+    int rc = prepare_request(*req, P);
+    if (rc != MP_OK) return rc;
+    if (P.kind != KIND_TILE || P.d.payoff == MP_PAY_ASIAN_MLMC)
+        return fail(MP_E_UNSUPPORTED,
+                    "mp_hip_price_noalloc prices GBM European/Asian/down-and-out/lookback, Heston and "
+                    "Merton requests; price Sobol, basket and American/Bermudan requests with mp_hip_price");
+    if (!P.sampler.empty() || P.d.sampling != MP_SAMPLER_NONE)
+        return fail(MP_E_UNSUPPORTED, "mp_hip_price_noalloc takes pseudorandom (Philox) draws only");
+    // request record, then the counter, then the partials, each 256-byte aligned
+    constexpr uint64_t rec_bytes = (sizeof(mp_dreq) + 255) / 256 * 256;
+    const uint64_t need = rec_bytes + 256 + (uint64_t)P.d.n_tiles * MP_PARTIAL_STRIDE * 8;
+    if (need > MP_NOALLOC_SCRATCH_BYTES)
+        return fail(MP_E_OOM, "request needs " + std::to_string(need) + " bytes of device scratch (" +
+                                  std::to_string(P.d.n_tiles) + " tiles); mp_hip_price_noalloc has " +
+                                  std::to_string(MP_NOALLOC_SCRATCH_BYTES) + " per device");
+    unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[c->device]);
+    const tile_bufs fixed{reinterpret_cast<mp_dreq*>(base), reinterpret_cast<unsigned int*>(base + rec_bytes),
+                          reinterpret_cast<double*>(base + rec_bytes + 256)};
+    std::lock_guard<std::mutex> lk(g_scratch_call_mu[c->device]);
+    const int prec = req->precision == 64 ? 64 : 32;
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = run_tiles_single(c, P, prec, out, nullptr, nullptr, nullptr, nullptr, nullptr, &fixed);
+    if (rc != MP_OK) {
+        out->status = rc;
+        return rc;
+    }
+    if (req->fp64_check && prec == 32) {
+        mp_result ref;
+        const mp_timing keep = c->last;
+        rc = run_tiles_single(c, P, 64, &ref, nullptr, nullptr, nullptr, nullptr, nullptr, &fixed);
         c->last = keep;
         if (rc != MP_OK) return rc;
         out->fp64_ref_error = std::fabs(out->price - ref.price);
