@@ -33,6 +33,55 @@ __device__ __forceinline__ mp_u4 mp_philox(uint32_t c0, uint32_t c1, uint32_t c2
     return mp_u4{c0, c1, c2, c3};
 }
 
+// mp_philox for a counter whose words c1..c3 and key are the same in every
+// lane of the wave (step block, stream, seed high word; seed low word,
+// contract), with only c0 (the path) per lane: the first round's c2 product
+// and the second round's c0 product are then wave-uniform and, when the
+// caller holds c1..c3 and the key in scalar registers (values the compiler
+// knows to be uniform), are formed on the scalar unit.  Same output as
+// mp_philox.
+__device__ __forceinline__ mp_u4 mp_philox_wave(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                uint32_t k0, uint32_t k1) {
+    // round 1: M1 * c2 is uniform
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint32_t p1_hi = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t p1_lo = 0xCD9E8D57u * c2;
+    uint32_t x0 = p1_hi ^ c1 ^ k0;  // uniform
+    uint32_t x1 = p1_lo;            // uniform
+    uint32_t x2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
+    uint32_t x3 = (uint32_t)p0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+    // round 2: M0 * x0 is uniform
+    {
+        const uint32_t q0_hi = __umulhi(0xD2511F53u, x0);
+        const uint32_t q0_lo = 0xD2511F53u * x0;
+        const uint64_t q1 = (uint64_t)0xCD9E8D57u * x2;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(q1 >> 32), x1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32(q0_hi, x3, k1, 0x96);
+        x0 = n0;
+        x1 = (uint32_t)q1;
+        x2 = n2;
+        x3 = q0_lo;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+#pragma unroll
+    for (int round = 2; round < 10; ++round) {
+        const uint64_t p0r = (uint64_t)0xD2511F53u * x0;
+        const uint64_t p1r = (uint64_t)0xCD9E8D57u * x2;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1r >> 32), x1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0r >> 32), x3, k1, 0x96);
+        x0 = n0;
+        x1 = (uint32_t)p1r;
+        x2 = n2;
+        x3 = (uint32_t)p0r;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return mp_u4{x0, x1, x2, x3};
+}
+
 // Uniform in [2^-24, 1 - 2^-24] with 23 significant bits plus a half step:
 // exact in both float and double, so FP32 and FP64 runs see the same draws.
 __device__ __forceinline__ float mp_u01f(uint32_t x) {

@@ -508,6 +508,152 @@ __device__ __forceinline__ void tile_body(const mp::tile_launch& L, uint32_t til
     }
 }
 
+// One Box-Muller pair of the FP32 Philox stream, scaled for a GBM step: the
+// volatility terms vol * z0 = vr * c and vol * z1 = vr * s of the normals
+// z0 = r cos(2 pi u2), z1 = r sin(2 pi u2), r = sqrt(-2 ln u1) =
+// sqrt(2 ln 2) sqrt(-log2 u1).  vscale is vol * sqrt(2 ln 2) (times log2(e)
+// for log2-unit paths), so the radius, the volatility and the unit change
+// cost one multiply per pair; the caller adds vr * c and vr * s with FMAs.
+__device__ __forceinline__ void gbm_pair_f32(uint32_t a, uint32_t b, float vscale, float& vr, float& c,
+                                             float& s) {
+    const float u1 = mp_u01f(a);
+    const float u2 = mp_u01f(b);
+    vr = vscale * __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
+    c = __builtin_amdgcn_cosf(u2);
+    s = __builtin_amdgcn_sinf(u2);
+}
+
+// FP32 GBM tile with Philox draws and the standard technique (the batching
+// and serving workload).  The same draws as tile_body; the arithmetic is
+// arranged for the vector unit:
+// * European: only S_T matters, so the log-return is the sum of the
+//   volatility terms plus n_steps * drift_dt, added once at the end;
+// * Asian: the path is carried in log2 units, so S/S0 is one v_exp_f32 per
+//   step; each 4-step block's S/S0 values are added pairwise and the block
+//   sum enters a Kahan-compensated running sum;
+// * down-and-out / lookback: natural-log path, barrier test / running max;
+// * full 4-step blocks run without per-step predicates; the tail block of
+//   n_steps % 4 steps is predicated.
+// Payoffs, discounting and sums then follow tile_body.
+template <int PPL, int PAY>
+__device__ __forceinline__ void gbm_philox_f32_tile(const mp::tile_launch& L, uint32_t tile,
+                                                    const mp_dreq& q, int lane) {
+    model_params<float> m = load_params<float>(q);
+    m.model = MP_MODEL_GBM;
+    m.payoff = PAY;
+    constexpr float kLog2e = 1.4426950408889634f;
+    constexpr bool kLog2 = PAY == MP_PAY_ASIAN;
+    const int n_steps = q.n_steps;
+    const int ns = q.n_strikes;
+    const float s0 = float(q.s0);
+    float kk[MP_NS_MAX];
+#pragma unroll
+    for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = float(q.strikes[j]);
+    const float vscale = float(q.vol_dt * (kLog2 ? 1.1774100225154747 * 1.4426950408889634 : 1.1774100225154747));
+    const float drift = kLog2 ? float(q.drift_dt * 1.4426950408889634) : m.drift_dt;
+    const float xb = m.xb;
+    const uint64_t pbeg = (uint64_t)(tile - q.tile0) * q.tile_paths;
+    const uint64_t pend = min(pbeg + (uint64_t)q.tile_paths, q.n_paths);
+    const int nfull = n_steps >> 2, ntail = n_steps & 3;
+    // the Philox words shared by the wave, in scalar registers once per tile
+    const uint32_t seed_hi = __builtin_amdgcn_readfirstlane(q.seed_hi);
+    const uint32_t key0 = __builtin_amdgcn_readfirstlane(q.key0);
+    const uint32_t key1 = __builtin_amdgcn_readfirstlane(q.key1);
+
+    double sum[MP_NS_MAX], sq[MP_NS_MAX];
+#pragma unroll
+    for (int j = 0; j < MP_NS_MAX; ++j) sum[j] = sq[j] = 0.0;
+
+    for (uint64_t base = pbeg; base < pend; base += (uint64_t)MP_WAVE * PPL) {
+        float x[PPL], xmax[PPL];
+        mp_kahan<float> acc[PPL];
+        bool knocked[PPL];
+        uint32_t pid32[PPL];
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            pid32[j] = (uint32_t)(base + lane + (uint64_t)MP_WAVE * j);
+            x[j] = 0.0f;
+            acc[j].init();
+            xmax[j] = 0.0f;
+            knocked[j] = q.knocked_at_start != 0;
+        }
+        // one 4-step block; nsub < 4 only for the tail block
+        auto block = [&](uint32_t blk, int nsub) {
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                const mp_u4 r = mp_philox_wave(pid32[j], blk, 0u, seed_hi, key0, key1);
+                // step e's volatility term is vr[e / 2] * tr[e]
+                float vr[2], tr[4];
+                gbm_pair_f32(r.x, r.y, vscale, vr[0], tr[0], tr[1]);
+                gbm_pair_f32(r.z, r.w, vscale, vr[1], tr[2], tr[3]);
+                if constexpr (PAY == MP_PAY_EUROPEAN) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (e < nsub) x[j] = __builtin_fmaf(vr[e >> 1], tr[e], x[j]);
+                } else if constexpr (PAY == MP_PAY_ASIAN) {
+                    float g[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        if (e < nsub) x[j] = __builtin_fmaf(vr[e >> 1], tr[e], x[j] + drift);
+                        g[e] = e < nsub ? __builtin_amdgcn_exp2f(x[j]) : 0.0f;
+                    }
+                    acc[j].add((g[0] + g[1]) + (g[2] + g[3]));
+                } else if constexpr (PAY == MP_PAY_DOWN_AND_OUT) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        if (e < nsub) {
+                            x[j] = __builtin_fmaf(vr[e >> 1], tr[e], x[j] + drift);
+                            knocked[j] = knocked[j] || (x[j] <= xb);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        if (e < nsub) {
+                            x[j] = __builtin_fmaf(vr[e >> 1], tr[e], x[j] + drift);
+                            xmax[j] = x[j] > xmax[j] ? x[j] : xmax[j];
+                        }
+                    }
+                }
+            }
+        };
+        for (int b = 0; b < nfull; ++b) block((uint32_t)b, 4);
+        if (ntail) block((uint32_t)nfull, ntail);
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            const uint64_t p = base + lane + (uint64_t)MP_WAVE * j;
+            if (p >= pend) continue;
+            path_state<float> st, sa;
+            st.x = PAY == MP_PAY_EUROPEAN ? x[j] + float(n_steps) * drift
+                                          : (kLog2 ? x[j] * 0.69314718055994531f : x[j]);
+            st.v = 0.0f;
+            st.acc = acc[j];
+            st.xmax = xmax[j];
+            st.knocked = knocked[j];
+            sa = st;
+            accumulate_unit(st, sa, m, q, MP_TECH_STANDARD, n_steps, s0, kk, ns, sum, sq);
+            if (L.terminal_out) L.terminal_out[p] = (double)(s0 * mp_exp(st.x));
+            if (L.payoff_out) L.payoff_out[p] = (double)payoff_of(st, m, q, n_steps, s0, kk[0]);
+        }
+    }
+    const int used = ns > 1 ? ns : 1;
+#pragma unroll
+    for (int j = 0; j < MP_NS_MAX; ++j) {
+        if (j < used) {
+            sum[j] = mp_wave_sum(sum[j]);
+            sq[j] = mp_wave_sum(sq[j]);
+        }
+    }
+    if (lane == 0) {
+        double* out = L.partial + (uint64_t)tile * MP_PARTIAL_STRIDE;
+#pragma unroll
+        for (int j = 0; j < MP_NS_MAX; ++j) {
+            out[2 * j] = sum[j];
+            out[2 * j + 1] = sq[j];
+        }
+    }
+}
+
 // The tile at queue position qpos, with the body compiled for its request.
 template <class R, int SRC, int PPL, bool ML>
 __device__ __forceinline__ void run_tile(const mp::tile_launch& L, unsigned int qpos, int lane) {
@@ -534,6 +680,15 @@ __device__ __forceinline__ void run_tile(const mp::tile_launch& L, unsigned int 
             const int mdl = q.model, pay = q.payoff;
             if (q.technique != MP_TECH_STANDARD) {
                 tile_body<R, SRC, PPL, -1, -1, -1>(L, tile, q, lane);
+            } else if (mdl == MP_MODEL_GBM && sizeof(R) == 4) {
+                if (pay == MP_PAY_EUROPEAN)
+                    gbm_philox_f32_tile<PPL, MP_PAY_EUROPEAN>(L, tile, q, lane);
+                else if (pay == MP_PAY_ASIAN)
+                    gbm_philox_f32_tile<PPL, MP_PAY_ASIAN>(L, tile, q, lane);
+                else if (pay == MP_PAY_DOWN_AND_OUT)
+                    gbm_philox_f32_tile<PPL, MP_PAY_DOWN_AND_OUT>(L, tile, q, lane);
+                else
+                    gbm_philox_f32_tile<PPL, MP_PAY_LOOKBACK>(L, tile, q, lane);
             } else if (mdl == MP_MODEL_GBM) {
                 if (pay == MP_PAY_EUROPEAN)
                     tile_body<R, SRC, PPL, MP_MODEL_GBM, MP_PAY_EUROPEAN, -1>(L, tile, q, lane);
