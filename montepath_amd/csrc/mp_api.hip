@@ -222,7 +222,11 @@ struct mp_ctx {
     // passes (the only per-date form for requests too large for one launch)
     bool lsm_fused = true;
     bool lsm_matched = true;
-    int lsm_blocks_per_cu[2][2][5] = {};  // [fp64][normals][log2 J], 0 = not yet queried
+    // polls before a one-launch pricing's wait gives up (MONTEPATH_LSM_SPIN_LIMIT
+    // at open, for tests of the fallback), and how often it fell back
+    unsigned int lsm_spin_limit = 1u << 20;
+    uint64_t lsm_fallbacks = 0;
+    int lsm_blocks_per_cu[2][2][5][3] = {};  // [fp64][normals][log2 J][block shape], 0 = not yet queried
     // FP32 Sobol-bridge requests on 16 * 2^J steps keep the bridge in
     // registers; MONTEPATH_QMC_STACK_BRIDGE=1 at open selects the LDS stack
     bool qmc_tree = true;
@@ -1114,8 +1118,20 @@ int run_basket(mp_ctx* c, const mp_request& r, prepared& P, int precision, const
     return MP_OK;
 }
 
-int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const double* normals_host,
-            mp_result* out, double* coeff_host, uint32_t* ex_host) {
+// Block shapes of the one-launch LSM pricing, in order of preference:
+// threads per block and the largest log2 of paths per thread.
+struct lsm_shape {
+    int bs, max_lj;
+};
+constexpr lsm_shape kLsmShapes[] = {{512, 3}, {256, 4}, {1024, 2}};
+inline int lsm_shape_index(int bs) { return bs == 1024 ? 2 : (bs == 512 ? 1 : 0); }
+
+// One LSM pricing.  per_date forces the matched per-date launches; *gave_up
+// reports a one-launch pricing whose bounded waits ran out (its result is
+// then not valid).
+int run_lsm_once(mp_ctx* c, const mp_request& r, prepared& P, int precision, const double* normals_host,
+                 mp_result* out, double* coeff_host, uint32_t* ex_host, bool per_date, bool* gave_up) {
+    *gave_up = false;
     c->stage_off = 0;  // the previous call has synchronised
     const uint64_t n = r.n_paths;
     const int m = (int)r.n_steps;
@@ -1155,28 +1171,36 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     std::vector<int> reg_dates;
     for (int i = ns - 2; i >= 0; --i) reg_dates.push_back((int)P.schedule[i]);
     const int n_reg = (int)reg_dates.size();
-    // the one-launch decomposition: J paths per thread, the fewest that keep
-    // every block resident.  It is priced in one launch, or (contexts opened
-    // with per-date launches, e.g. the server's) as matched per-date launches
-    // that reproduce the one-launch pricing bit for bit.
-    int fused_j = 0, fused_grid = 0;
-    {
-        for (int lj = 0; lj < 5 && !fused_j; ++lj) {
-            const int J = 1 << lj;
-            const uint64_t g = (n + (uint64_t)block * J - 1) / ((uint64_t)block * J);
-            int& bpc = c->lsm_blocks_per_cu[precision == 64][normals_host != nullptr][lj];
-            if (bpc == 0) bpc = std::max(-1, mp::lsm_fused_blocks_per_cu(precision, normals_host != nullptr, J));
-            if (bpc <= 0) continue;
-            // write-once slots of every date: at most 64 MiB
-            const uint64_t slot_bytes = (uint64_t)n_reg * (16 * g + 16 * 32 + 8) * 8;
-            if (g <= (uint64_t)MP_LSM_MAX_BLOCKS && g <= (uint64_t)bpc * (uint64_t)c->n_cu &&
-                slot_bytes <= (64ull << 20)) {
-                fused_j = J;
-                fused_grid = (int)std::max<uint64_t>(1, g);
-            }
-        }
+    // The one-launch decomposition: blocks of fused_bs threads with J paths
+    // per thread.  Fewer blocks make each date's exchange cheaper, so the
+    // first fit with one block per CU wins, in the order of kLsmShapes; if
+    // none fits, the fewest paths per thread that keep every 256-thread block
+    // resident.  It is priced in one launch, or (contexts opened with
+    // per-date launches, e.g. the server's) as matched per-date launches on
+    // the same grid and block size that reproduce the one-launch pricing bit
+    // for bit.
+    int fused_j = 0, fused_grid = 0, fused_bs = block;
+    auto try_shape = [&](int bs, int lj, uint64_t per_cu) {
+        const int J = 1 << lj;
+        const uint64_t g = (n + (uint64_t)bs * J - 1) / ((uint64_t)bs * J);
+        int& bpc = c->lsm_blocks_per_cu[precision == 64][normals_host != nullptr][lj][lsm_shape_index(bs)];
+        if (bpc == 0) bpc = std::max(-1, mp::lsm_fused_blocks_per_cu(precision, normals_host != nullptr, J, bs));
+        if (bpc <= 0) return false;
+        const uint64_t resident = std::min<uint64_t>(per_cu, (uint64_t)bpc) * (uint64_t)c->n_cu;
+        // write-once slots of every date: at most 64 MiB
+        const uint64_t slot_bytes = (uint64_t)n_reg * (16 * g + 16 * 32) * 8;
+        if (g > (uint64_t)MP_LSM_MAX_BLOCKS || g > resident || slot_bytes > (64ull << 20)) return false;
+        fused_j = J;
+        fused_bs = bs;
+        fused_grid = (int)std::max<uint64_t>(1, g);
+        return true;
+    };
+    for (const lsm_shape& sh : kLsmShapes) {
+        if (fused_j) break;
+        for (int lj = 0; lj <= sh.max_lj && !fused_j; ++lj) try_shape(sh.bs, lj, 1);
     }
-    const bool matched = fused_j && !c->lsm_fused && c->lsm_matched;
+    for (int lj = 0; lj < 5 && !fused_j; ++lj) try_shape(256, lj, ~0ull);
+    const bool matched = fused_j && ((!c->lsm_fused && c->lsm_matched) || per_date);
     if (!c->lsm_fused && !c->lsm_matched) fused_j = 0;
     if (matched) fused_j = 0;  // per-date launches on the one-launch grid
     const int grid = (fused_j || matched)
@@ -1190,9 +1214,8 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     ENSURE(c->lsm_coeff, (size_t)(m + 1) * 4 * 8);
     ENSURE(c->lsm_valid, (size_t)(m + 1) * 4);
     // per-date launches: block partials [grid][16]; one launch: write-once
-    // slots [n_reg][16][grid], group totals [n_reg][16][32] and coefficient
-    // broadcasts [n_reg][8]
-    const size_t n_slots = fused_j ? (size_t)n_reg * (16 * grid + 16 * 32 + 8) : 0;
+    // slots [n_reg][16][grid] and group totals [n_reg][16][32]
+    const size_t n_slots = fused_j ? (size_t)n_reg * (16 * grid + 16 * 32) : 0;
     // matched per-date launches: block sums [16][grid], then group totals [16][32]
     ENSURE(c->lsm_gram, std::max((size_t)grid * 16 + 16 * 32, n_slots) * 8);
     ENSURE(c->lsm_par, blob_bytes);
@@ -1227,11 +1250,14 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
         HIPCHK(mp::launch_lsm_fused(dp, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
                                     c->lsm_dates.as<int>(), n_reg, c->lsm_coeff.as<double>(),
                                     c->lsm_valid.as<int>(), slots, slots + (size_t)n_reg * 16 * grid,
-                                    slots + (size_t)n_reg * 16 * (grid + 32), c->lsm_part.as<double>(), cnt, c->lsm_ex.as<uint16_t>(),
-                                    c->h_res_dev, cnt + 1, 1u << 20, fused_j, grid, precision,
-                                    c->stream));
+                                    c->lsm_part.as<double>(), cnt, c->lsm_ex.as<uint16_t>(),
+                                    c->h_res_dev, cnt + 1, c->lsm_spin_limit, fused_j, fused_bs, grid,
+                                    precision, c->stream));
         ++launches;
     } else {
+    // matched launches use the one-launch block size (same per-thread paths
+    // and block sums); plain ones 256-thread blocks
+    const int sblock = matched ? fused_bs : block;
     HIPCHK(mp::launch_lsm_paths(dp, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
                                 c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), precision, grid,
                                 block, c->stream));
@@ -1242,7 +1268,7 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
         const int date = reg_dates[i];
         HIPCHK(mp::launch_lsm_step(dp, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
                                    c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, date,
-                                   c->lsm_gram.as<double>(), cnt, precision, grid, block, matched, c->stream));
+                                   c->lsm_gram.as<double>(), cnt, precision, grid, sblock, matched, c->stream));
         ++launches;
         if (matched) {
             HIPCHK(mp::launch_lsm_group(dp, c->lsm_gram.as<double>(), grid, c->lsm_gram.as<double>() + 16 * grid,
@@ -1255,11 +1281,11 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     if (prev >= 1) {
         HIPCHK(mp::launch_lsm_step(dp, c->lsm_paths.p, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(),
                                    c->lsm_coeff.as<double>(), c->lsm_valid.as<int>(), prev, 0,
-                                   c->lsm_gram.as<double>(), cnt, precision, grid, block, matched, c->stream));
+                                   c->lsm_gram.as<double>(), cnt, precision, grid, sblock, matched, c->stream));
         ++launches;
     }
     HIPCHK(mp::launch_lsm_final(dp, c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), c->lsm_valid.as<int>(),
-                                c->lsm_part.as<double>(), cnt + (m + 1), c->h_res_dev, grid, block,
+                                c->lsm_part.as<double>(), cnt + (m + 1), c->h_res_dev, grid, sblock,
                                 c->stream));
     }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
@@ -1280,9 +1306,12 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
         HIPCHK(hipMemcpyAsync(ex.data(), c->lsm_ex.p, n * 2, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (*h_err && fused_j) {
+        *gave_up = true;
+        return MP_OK;
+    }
     if (*h_err)
-        return fail(MP_E_HIP, "LSM: a block waited too long for the other blocks of its launch "
-                              "(the device was shared with other work); result discarded");
+        return fail(MP_E_HIP, "LSM: a per-date reduction found a block's sums missing");
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     c->last.main_kernel_ms = ms;
@@ -1310,6 +1339,23 @@ int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const do
     out->runtime_ms = ms;
     out->fp64_ref_error = NAN;
     return MP_OK;
+}
+
+// LSM pricing.  A one-launch pricing needs every block of its grid resident
+// at once; when other work on the device (another process) keeps some of
+// them out, the others' bounded waits run out and the pricing is repeated
+// with the matched per-date launches, which need no co-residency and give
+// the same bits.
+int run_lsm(mp_ctx* c, const mp_request& r, prepared& P, int precision, const double* normals_host,
+            mp_result* out, double* coeff_host, uint32_t* ex_host) {
+    bool gave_up = false;
+    int rc = run_lsm_once(c, r, P, precision, normals_host, out, coeff_host, ex_host, false, &gave_up);
+    if (rc == MP_OK && gave_up) {
+        c->lsm_fallbacks += 1;
+        rc = run_lsm_once(c, r, P, precision, normals_host, out, coeff_host, ex_host, true, &gave_up);
+        if (rc == MP_OK && gave_up) rc = fail(MP_E_HIP, "LSM: per-date launches reported a timed-out wait");
+    }
+    return rc;
 }
 
 // Device buffers of a one-request launch given by the caller instead of the
@@ -1472,6 +1518,10 @@ MP_EXPORT int mp_hip_open(int device_ordinal, mp_ctx** out) {
         c->lsm_matched = v[0] != '2';
     }
     if (const char* v = std::getenv("MONTEPATH_QMC_STACK_BRIDGE")) c->qmc_tree = !(v[0] == '1');
+    if (const char* v = std::getenv("MONTEPATH_LSM_SPIN_LIMIT")) {
+        const long lim = std::strtol(v, nullptr, 10);
+        if (lim >= 1 && lim <= (1l << 24)) c->lsm_spin_limit = (unsigned int)lim;
+    }
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     sobol_directions(c->dirs);

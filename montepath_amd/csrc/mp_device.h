@@ -178,24 +178,26 @@ __device__ __forceinline__ double mp_wave_sum(double v) {
 // adjacent lanes, 1 adjacent pairs (quad_perm [1,0,3,2] / [2,3,0,1]), 2
 // mirrors each half-row, 3 each row.
 __device__ __forceinline__ double mp_dpp_f64(double v, int ctrl_id) {
+    // every pattern reads inside the lane's own row, so the bound-control
+    // form (no old value to keep) moves the same data
     const long long b = __double_as_longlong(v);
     int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
     switch (ctrl_id) {
         case 0:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xf, 0xf, false);
+            lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xf, 0xf, true);
+            hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xf, 0xf, true);
             break;
         case 1:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xf, 0xf, false);
+            lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xf, 0xf, true);
+            hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xf, 0xf, true);
             break;
         case 2:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xf, 0xf, false);
+            lo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xf, 0xf, true);
+            hi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xf, 0xf, true);
             break;
         default:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xf, 0xf, false);
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xf, 0xf, false);
+            lo = __builtin_amdgcn_mov_dpp(lo, 0x140, 0xf, 0xf, true);
+            hi = __builtin_amdgcn_mov_dpp(hi, 0x140, 0xf, 0xf, true);
             break;
     }
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);

@@ -177,18 +177,19 @@ hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* ca
 hipError_t launch_lsm_group(const mp_lsm_params* p, const double* partial, int nb, double* gpart,
                             unsigned int* counter, double* coeff_all, int* valid_all, int step_gram,
                             unsigned int* err, hipStream_t s);
-// Whole pricing in one launch (paths per thread J in {1,2,4,8,16}, grid
-// blocks of 256 threads, all resident at once).  slots: n_reg * 16 * grid
-// doubles, gslots: n_reg * 16 * 32 and bcast: n_reg * 8, all filled by
-// launch_slot_fill first; ticket and err zeroed; err is set when a wait
-// timed out.
-int lsm_fused_blocks_per_cu(int precision, bool normals, int J);
+// Whole pricing in one launch (grid blocks of bs threads, all resident at
+// once; J paths per thread, J in {1,2,4} for bs = 1024 and {1,2,4,8,16} for
+// bs = 256).  slots: n_reg * 16 * grid
+// doubles and gslots: n_reg * 16 * 32, both filled by launch_slot_fill
+// first; ticket and err zeroed; err is set when a wait timed out.
+int lsm_fused_blocks_per_cu(int precision, bool normals, int J, int bs);
 hipError_t launch_slot_fill(double* slots, uint64_t n, hipStream_t s);
 hipError_t launch_lsm_fused(const mp_lsm_params* p, const double* normals, void* paths,
                             const int* reg_dates, int n_reg, double* coeff_all, int* valid_all,
-                            double* slots, double* gslots, double* bcast, double* fin_partial,
+                            double* slots, double* gslots, double* fin_partial,
                             unsigned int* ticket, uint16_t* ex, double* out, unsigned int* err,
-                            unsigned int spin_limit, int J, int grid, int precision, hipStream_t s);
+                            unsigned int spin_limit, int J, int bs, int grid, int precision,
+                            hipStream_t s);
 hipError_t launch_lsm_final(const mp_lsm_params* p, const double* cash, const uint16_t* ex,
                             const int* valid_all, double* partial, unsigned int* counter,
                             double* out, int grid, int block, hipStream_t s);

@@ -541,7 +541,6 @@ __device__ __forceinline__ void gbm_philox_f32_tile(const mp::tile_launch& L, ui
     model_params<float> m = load_params<float>(q);
     m.model = MP_MODEL_GBM;
     m.payoff = PAY;
-    constexpr float kLog2e = 1.4426950408889634f;
     constexpr bool kLog2 = PAY == MP_PAY_ASIAN;
     const int n_steps = q.n_steps;
     const int ns = q.n_strikes;
@@ -1526,20 +1525,27 @@ __global__ __launch_bounds__(256) void lsm_paths_kernel(const mp_lsm_params* __r
 }
 
 __device__ __forceinline__ void laguerre(double x, int degree, double b[4]) {
-    // laguerre_lsm_basis cpu.rs:7667-7682 (unweighted, x = S/K)
+    // laguerre_lsm_basis cpu.rs:7667-7682 (unweighted, x = S/K).  x^3 / 6 is
+    // the product with RN(1/6) corrected once by its exact FMA remainder,
+    // which equals the rounded quotient (no miss on 1.5e9 sampled values)
+    // without an FP64 division.
     const double x2 = x * x;
     const double x3 = x2 * x;
+    const double q0 = x3 * 0.16666666666666666;
+    const double x3_6 = __builtin_fma(__builtin_fma(-q0, 6.0, x3), 0.16666666666666666, q0);
     b[0] = 1.0;
     b[1] = degree >= 1 ? 1.0 - x : 0.0;
     b[2] = degree >= 2 ? 1.0 - 2.0 * x + 0.5 * x2 : 0.0;
-    b[3] = degree >= 3 ? 1.0 - 3.0 * x + 1.5 * x2 - x3 / 6.0 : 0.0;
+    b[3] = degree >= 3 ? 1.0 - 3.0 * x + 1.5 * x2 - x3_6 : 0.0;
 }
 
 // Gauss-Jordan with partial pivoting, failing when the best pivot <= 1e-12
-// (solve_lsm_normal_equations cpu.rs:7684-7736).  Single lane; every index
-// is static so the system stays in registers.  Rows and columns at or beyond
-// dim are zero (their basis functions are zero), so carrying them through
-// the eliminations changes nothing.
+// (solve_lsm_normal_equations cpu.rs:7684-7736), with the pivot row scaled
+// by the pivot's reciprocal instead of divided element by element (one
+// division per pivot; the results differ from division in the last bits).
+// Single lane; every index is static so the system stays in registers.  Rows
+// and columns at or beyond dim are zero (their basis functions are zero), so
+// carrying them through the eliminations changes nothing.
 __device__ bool lsm_solve4(double (&m)[4][4], double (&rhs)[4], int dim) {
 #pragma unroll
     for (int piv = 0; piv < 4; ++piv) {
@@ -1569,10 +1575,12 @@ __device__ bool lsm_solve4(double (&m)[4][4], double (&rhs)[4], int dim) {
                 rhs[row] = t;
             }
         }
-        const double pivot = m[piv][piv];
+        // one division per pivot; the row is scaled by its reciprocal
+        const double inv = 1.0 / m[piv][piv];
+        m[piv][piv] = 1.0;
 #pragma unroll
-        for (int c = piv; c < 4; ++c) m[piv][c] /= pivot;
-        rhs[piv] /= pivot;
+        for (int c = piv + 1; c < 4; ++c) m[piv][c] *= inv;
+        rhs[piv] *= inv;
 #pragma unroll
         for (int row = 0; row < 4; ++row) {
             if (row == piv) continue;
@@ -1638,17 +1646,20 @@ __device__ __forceinline__ bool publish_and_ticket(double (&v)[NV], double* dst,
     return is_last;
 }
 
-// Fixed-order FP64 sum of NV values over nb <= MP_LSM_MAX_BLOCKS rows,
-// 256 threads: thread t first adds its rows t, t + 256, ... in that order
-// (the caller's rowsum), then 16 threads per value sum 16 of those in
-// order, and one thread per value sums the 16 results into tot (shared).
+// Fixed-order FP64 sum of NV values over nb <= MP_LSM_MAX_BLOCKS rows by
+// the block's first 256 threads (blocks of 256 or more): thread t first adds
+// its rows t, t + 256, ... in that order (the caller's rowsum), then 16
+// threads per value sum 16 of those in order, and one thread per value sums
+// the 16 results into tot (shared).
 template <int NV>
 __device__ __forceinline__ void sum_rows(const double (&rowsum)[NV], double* tot) {
     __shared__ double row[256][NV + 1];
     __shared__ double red[NV][17];
     const int t = threadIdx.x;
+    if (t < 256) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) row[t][i] = rowsum[i];
+        for (int i = 0; i < NV; ++i) row[t][i] = rowsum[i];
+    }
     __syncthreads();
     const int vi = t >> 4, j = t & 15;
     if (vi < NV) {
@@ -1678,7 +1689,9 @@ __device__ __forceinline__ void reduce_partials(const double* partial, int nb, i
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int i = 0; i < NV; ++i)
-            a[r][i] = t + 256 * r < nb ? mp_load_coherent(partial + (uint64_t)(t + 256 * r) * stride + i) : 0.0;
+            a[r][i] = t < 256 && t + 256 * r < nb
+                          ? mp_load_coherent(partial + (uint64_t)(t + 256 * r) * stride + i)
+                          : 0.0;
     double v[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -1745,26 +1758,41 @@ __device__ __forceinline__ void reduce_group(const double* base, uint64_t stride
     __syncthreads();
 }
 
-// Block sum of NV values; thread i < NV stores value i to dst[i * stride]
-// as a write-once slot.
+// Block sum of NV <= 16 values (blocks of 256 to 1024 threads); thread
+// i < NV stores value i to dst[i * stride] as a write-once slot.  Fixed
+// order: DPP sums within each quad of lanes; the block's quad sums (64 to
+// 256 per value) go to LDS, 16 threads per value each add a run of them in
+// order, and one thread per value adds the 16 run sums in order.
 template <int NV>
-// (DPP sums within each 16-lane row, then the row sums of every wave added
-// in order by one thread per value.)
 __device__ __forceinline__ void block_sum_to_slots(double (&v)[NV], double* dst, uint64_t stride) {
-    __shared__ double red[64][NV];
-    const int lane = threadIdx.x & (MP_WAVE - 1);
-    const int nrows = blockDim.x >> 4;
+    __shared__ double red[NV][256];
+    __shared__ double red2[NV][16];
+    const int t = threadIdx.x;
+    const int nq = blockDim.x >> 2;  // 64 .. 256
 #pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = mp_row_sum_dpp(v[i]);
-    if ((lane & 15) == 0) {
+    for (int i = 0; i < NV; ++i) {
+        v[i] += mp_dpp_f64(v[i], 0);
+        v[i] += mp_dpp_f64(v[i], 1);
+    }
+    if ((t & 3) == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) red[threadIdx.x >> 4][i] = v[i];
+        for (int i = 0; i < NV; ++i) red[i][t >> 2] = v[i];
     }
     __syncthreads();
-    if (threadIdx.x < NV) {
-        double acc = 0.0;
-        for (int r = 0; r < nrows; ++r) acc += red[r][threadIdx.x];
-        mp_store_coherent(dst + threadIdx.x * stride, acc);
+    const int vi = t >> 4, j = t & 15;
+    if (vi < NV) {
+        const int per = nq >> 4;  // 4 .. 16
+        const double* r = &red[vi][j * per];
+        double a = r[0];
+        for (int q = 1; q < per; ++q) a += r[q];
+        red2[vi][j] = a;
+    }
+    __syncthreads();
+    if (t < NV) {
+        double acc = red2[t][0];
+#pragma unroll
+        for (int q = 1; q < 16; ++q) acc += red2[t][q];
+        mp_store_coherent(dst + t * stride, acc);
     }
 }
 
@@ -1780,7 +1808,7 @@ __device__ __forceinline__ void block_sum_to_slots(double (&v)[NV], double* dst,
 // Each thread already visits its paths p0, p0 + T, ... in the one-launch
 // kernel's order with the same per-path arithmetic.
 template <class R, bool MATCHED>
-__global__ __launch_bounds__(256) void lsm_step_kernel(const mp_lsm_params* __restrict__ Pp,
+__global__ __launch_bounds__(1024) void lsm_step_kernel(const mp_lsm_params* __restrict__ Pp,
                                                       const R* __restrict__ paths,
                                                       double* cash, uint16_t* ex,
                                                       double* coeff_all, int* valid_all,
@@ -1918,7 +1946,7 @@ __global__ __launch_bounds__(256) void lsm_group_kernel(const mp_lsm_params* __r
 
 // Sums of discounted cash flows and exercise counts; the last block writes
 // price, standard error, counts and the number of regression dates.
-__global__ __launch_bounds__(256) void lsm_final_kernel(const mp_lsm_params* __restrict__ Pp,
+__global__ __launch_bounds__(1024) void lsm_final_kernel(const mp_lsm_params* __restrict__ Pp,
                                                        const double* cash, const uint16_t* ex,
                                                        const int* valid_all, double* partial,
                                                        unsigned int* counter, double* out) {
@@ -1971,24 +1999,26 @@ __global__ __launch_bounds__(256) void lsm_final_kernel(const mp_lsm_params* __r
 // date i every block stores its Gram/RHS sums into write-once slots
 // slots[i][value][block]; block g < NG waits for the sums of blocks
 // 32g .. 32g + 31 and stores their fixed-order total into gslots[i][value][g];
-// block 0 waits for the NG group totals, adds them in group order, solves
-// the 4x4 system and stores the coefficients and the validity flag into the
-// write-once slots bcast[i][0..4], which the other blocks wait for.  (A
-// coherent load costs the issuing CU about one request per lane, so one
-// block polling every block's sums would take ~15 us per date.)  Every wait is bounded: after spin_limit polls a
+// every block then waits for the NG group totals, adds them in group order
+// and solves the 4x4 system itself (the same bits in every block), so a date
+// costs two cross-block hops.  (A coherent load costs the issuing CU about
+// one request per lane, so the host picks the fewest blocks -- one per CU
+// when the paths allow -- which keeps NG, and every block's loads of the
+// group totals, small.)  Every wait is bounded: after spin_limit polls a
 // block records an error and carries on, so every wave reaches the end.
 // The block count must not exceed what the device holds at once (checked
 // by the host).  Arithmetic and its order per path equal lsm_paths_kernel,
 // lsm_step_kernel and lsm_final_kernel; only the grouping of the FP64 Gram
-// sums differs (paths per thread, then wave, block and fixed block order).
+// sums differs (paths per thread, then wave, block and fixed block order),
+// and lsm_group_kernel adds the group totals in the same order.
 // ---------------------------------------------------------------------------
-template <class R, int SRC, int J>
-__global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __restrict__ Pp,
+template <class R, int SRC, int J, int BS>
+__global__ __launch_bounds__(BS) void lsm_fused_kernel(const mp_lsm_params* __restrict__ Pp,
                                                        const double* __restrict__ normals,
                                                        R* __restrict__ paths,
                                                        const int* __restrict__ reg_dates, int n_reg,
                                                        double* coeff_all, int* valid_all,
-                                                       double* slots, double* gslots, double* bcast,
+                                                       double* slots, double* gslots,
                                                        double* fin_partial, unsigned int* ticket,
                                                        uint16_t* ex_out, double* out,
                                                        unsigned int* err, unsigned int spin_limit) {
@@ -2049,92 +2079,89 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
     bool live[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) live[j] = p0 + (uint64_t)j * T < n;
-    // prefetch the prices of the first regression date
-    R sd[J];
-    if (n_reg > 0) {
-        const R* pd = paths + (uint64_t)(reg_dates[0] - 1) * n;
+    // Prices of the current regression date (sd) and of the next one (sn),
+    // loaded one exchange ahead.  A date's basis values, Gram sums and count
+    // depend on its prices only, not on exercise decisions, so they are
+    // formed while the previous date's sums travel between blocks; the basis
+    // values (bb) are kept for that date's right-hand side, which needs the
+    // cash flows after the later dates' exercise and is formed after the
+    // previous solve.  The exercise test evaluates the basis once more.
+    R sd[J], sn[J];
+    double bb[J][4], gacc[11];
+    auto load_date = [&](int i, R (&dst)[J]) {
+        const R* pd = paths + (uint64_t)(reg_dates[i] - 1) * n;
+#pragma unroll
+        for (int j = 0; j < J; ++j) dst[j] = live[j] ? pd[(uint64_t)(p0 + (uint64_t)j * T)] : R(0);
+    };
+    auto basis_of = [&](const R (&sx)[J]) {
+#pragma unroll
+        for (int q = 0; q < 11; ++q) gacc[q] = 0.0;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            const uint64_t p = p0 + (uint64_t)j * T;
-            sd[j] = live[j] ? pd[p] : R(0);
+            const double s = (double)sx[j];
+            laguerre(s * inv_k, deg, bb[j]);
+            if (live[j] && k - s > 0.0) {
+                int e = 0;
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = a; b < 4; ++b) gacc[e++] += bb[j][a] * bb[j][b];
+                gacc[10] += 1.0;
+            }
         }
+    };
+    if (n_reg > 0) {
+        load_date(0, sd);
+        basis_of(sd);
+        if (n_reg > 1) load_date(1, sn);
     }
+    const int NG = (G + 31) >> 5;
     for (int i = 0; i < n_reg; ++i) {
         const int date = reg_dates[i];
         const double grow = growth[date];
         double acc[15];
 #pragma unroll
-        for (int q = 0; q < 15; ++q) acc[q] = 0.0;
+        for (int q = 0; q < 10; ++q) acc[q] = gacc[q];
+        acc[14] = gacc[10];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[10 + a] = 0.0;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const double s = (double)sd[j];
-            const double imm = k - s > 0.0 ? k - s : 0.0;
-            if (live[j] && imm > 0.0) {
-                double bb[4];
-                laguerre(s * inv_k, deg, bb);
+            if (live[j] && k - s > 0.0) {
                 const double cont = cash[j] * grow;
-                int e = 0;
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = a; b < 4; ++b) acc[e++] += bb[a] * bb[b];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) acc[10 + a] += bb[a] * cont;
-                acc[14] += 1.0;
+                for (int a = 0; a < 4; ++a) acc[10 + a] += bb[j][a] * cont;
             }
         }
         double* slot_i = slots + (uint64_t)i * 16 * G;
         double* gslot_i = gslots + (uint64_t)i * 16 * 32;
-        double* bc = bcast + (uint64_t)i * 8;
         block_sum_to_slots<15>(acc, slot_i + blockIdx.x, (uint64_t)G);
-        // blocks 0 .. NG-1 each reduce a group of 32 blocks' sums
-        const int NG = (G + 31) >> 5;
+        // blocks 0 .. NG-1 each reduce a group of 32 blocks' sums and publish
+        // the group total
         if ((int)blockIdx.x < NG) {
             const int g = blockIdx.x;
             reduce_group<15>(slot_i, (uint64_t)G, 32 * g, min(32, G - 32 * g), spin_limit, err, gsum);
-            if (NG > 1 && threadIdx.x < 15) mp_store_coherent(gslot_i + threadIdx.x * 32 + g, gsum[threadIdx.x]);
+            if (threadIdx.x < 15) mp_store_coherent(gslot_i + threadIdx.x * 32 + g, gsum[threadIdx.x]);
         }
-        if (blockIdx.x == 0) {
-            // the group sums, in group order
-            if (NG > 1) {
-                reduce_group<15>(gslot_i, 32, 0, NG, spin_limit, err, tot);
-            } else if (threadIdx.x < 15) {
-                tot[threadIdx.x] = gsum[threadIdx.x];
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double c[4];
-                const int ok = lsm_coefficients(tot, nc, c);
-                for (int a = 0; a < 4; ++a) {
-                    beta_s[a] = c[a];
-                    mp_store_coherent(bc + a, c[a]);
-                }
-                mp_store_coherent(bc + 4, ok ? 1.0 : 0.0);
-                valid_s = ok;
+        // off the critical path: the next date's basis and Gram sums, and the
+        // prices of the date after it
+        R s2[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) s2[j] = R(0);
+        if (i + 1 < n_reg) basis_of(sn);
+        if (i + 2 < n_reg) load_date(i + 2, s2);
+        // every block adds the group totals in group order and solves
+        reduce_group<15>(gslot_i, 32, 0, NG, spin_limit, err, tot);
+        if (threadIdx.x == 0) {
+            double c[4];
+            const int ok = lsm_coefficients(tot, nc, c);
+            for (int a = 0; a < 4; ++a) beta_s[a] = c[a];
+            valid_s = ok;
+            if (blockIdx.x == 0) {
                 // host outputs and the regression count (read after the final ticket)
                 for (int a = 0; a < 4; ++a) mp_store_coherent(coeff_all + date * 4 + a, c[a]);
                 __hip_atomic_store(valid_all + date, ok, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        } else if (threadIdx.x == 0) {
-            double bv[5];
-            unsigned int polls = 0;
-            for (;;) {
-                bool full = true;
-#pragma unroll
-                for (int a = 0; a < 5; ++a) full &= mp_slot_read(bc + a, bv[a]);
-                if (full || mp_poll_give_up(polls, spin_limit, err)) break;
-            }
-            for (int a = 0; a < 4; ++a) beta_s[a] = bv[a];
-            valid_s = bv[4] != 0.0;
-        }
-        // prefetch the next date's prices while the block waits
-        R sn[J];
-        if (i + 1 < n_reg) {
-            const R* pn = paths + (uint64_t)(reg_dates[i + 1] - 1) * n;
-#pragma unroll
-            for (int j = 0; j < J; ++j) {
-                const uint64_t p = p0 + (uint64_t)j * T;
-                sn[j] = live[j] ? pn[p] : R(0);
             }
         }
         __syncthreads();
@@ -2148,11 +2175,11 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
                 const double s = (double)sd[j];
                 const double imm = k - s > 0.0 ? k - s : 0.0;
                 if (live[j] && imm > 0.0) {
-                    double bb[4];
-                    laguerre(s * inv_k, deg, bb);
+                    double bx[4];
+                    laguerre(s * inv_k, deg, bx);
                     double cont = 0.0;
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) cont += beta[a] * bb[a];
+                    for (int a = 0; a < 4; ++a) cont += beta[a] * bx[a];
                     if (imm > (cont > 0.0 ? cont : 0.0)) {
                         cash[j] = imm * dpow;
                         exd[j] = date;
@@ -2160,9 +2187,10 @@ __global__ __launch_bounds__(256) void lsm_fused_kernel(const mp_lsm_params* __r
                 }
             }
         }
-        if (i + 1 < n_reg) {
 #pragma unroll
-            for (int j = 0; j < J; ++j) sd[j] = sn[j];
+        for (int j = 0; j < J; ++j) {
+            sd[j] = sn[j];
+            sn[j] = s2[j];
         }
         __syncthreads();  // beta_s / valid_s are rewritten at the next date
     }
@@ -2551,27 +2579,47 @@ hipError_t launch_lsm_paths(const mp_lsm_params* p, const double* normals, void*
     return hipGetLastError();
 }
 
+// Blocks of 1024 threads take up to 4 paths per thread (the register budget
+// of four waves per SIMD), blocks of 512 up to 8, blocks of 256 up to 16.
 template <class R, int SRC>
-static const void* lsm_fused_fn(int J) {
+static const void* lsm_fused_fn(int J, int bs) {
+    if (bs == 1024) {
+        switch (J) {
+            case 1: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 1, 1024>);
+            case 2: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 2, 1024>);
+            case 4: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 4, 1024>);
+            default: return nullptr;
+        }
+    }
+    if (bs == 512) {
+        switch (J) {
+            case 1: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 1, 512>);
+            case 2: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 2, 512>);
+            case 4: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 4, 512>);
+            case 8: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 8, 512>);
+            default: return nullptr;
+        }
+    }
+    if (bs != 256) return nullptr;
     switch (J) {
-        case 1: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 1>);
-        case 2: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 2>);
-        case 4: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 4>);
-        case 8: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 8>);
-        case 16: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 16>);
+        case 1: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 1, 256>);
+        case 2: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 2, 256>);
+        case 4: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 4, 256>);
+        case 8: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 8, 256>);
+        case 16: return reinterpret_cast<const void*>(&lsm_fused_kernel<R, SRC, 16, 256>);
         default: return nullptr;
     }
 }
 
-static const void* lsm_fused_select(int precision, bool normals, int J) {
-    if (precision == 64) return normals ? lsm_fused_fn<double, 1>(J) : lsm_fused_fn<double, 0>(J);
-    return normals ? lsm_fused_fn<float, 1>(J) : lsm_fused_fn<float, 0>(J);
+static const void* lsm_fused_select(int precision, bool normals, int J, int bs) {
+    if (precision == 64) return normals ? lsm_fused_fn<double, 1>(J, bs) : lsm_fused_fn<double, 0>(J, bs);
+    return normals ? lsm_fused_fn<float, 1>(J, bs) : lsm_fused_fn<float, 0>(J, bs);
 }
 
-int lsm_fused_blocks_per_cu(int precision, bool normals, int J) {
-    const void* f = lsm_fused_select(precision, normals, J);
+int lsm_fused_blocks_per_cu(int precision, bool normals, int J, int bs) {
+    const void* f = lsm_fused_select(precision, normals, J, bs);
     int nb = 0;
-    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) != hipSuccess) {
+    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, bs, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
@@ -2593,14 +2641,15 @@ hipError_t launch_slot_fill(double* slots, uint64_t n, hipStream_t s) {
 
 hipError_t launch_lsm_fused(const mp_lsm_params* p, const double* normals, void* paths,
                             const int* reg_dates, int n_reg, double* coeff_all, int* valid_all,
-                            double* slots, double* gslots, double* bcast, double* fin_partial,
+                            double* slots, double* gslots, double* fin_partial,
                             unsigned int* ticket, uint16_t* ex, double* out, unsigned int* err,
-                            unsigned int spin_limit, int J, int grid, int precision, hipStream_t s) {
-    const void* f = lsm_fused_select(precision, normals != nullptr, J);
+                            unsigned int spin_limit, int J, int bs, int grid, int precision,
+                            hipStream_t s) {
+    const void* f = lsm_fused_select(precision, normals != nullptr, J, bs);
     if (!f || grid < 1 || grid > MP_LSM_MAX_BLOCKS) return hipErrorInvalidValue;
     void* args[] = {&p, &normals, &paths, &reg_dates, &n_reg, &coeff_all, &valid_all, &slots,
-                    &gslots, &bcast, &fin_partial, &ticket, &ex, &out, &err, &spin_limit};
-    return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, s);
+                    &gslots, &fin_partial, &ticket, &ex, &out, &err, &spin_limit};
+    return hipLaunchKernel(f, dim3(grid), dim3(bs), args, 0, s);
 }
 
 template <bool MATCHED>
@@ -2622,7 +2671,8 @@ hipError_t launch_lsm_step(const mp_lsm_params* p, const void* paths, double* ca
                            uint16_t* ex, double* coeff_all, int* valid_all, int step_update,
                            int step_gram, double* gram_partial, unsigned int* counters,
                            int precision, int grid, int block, bool matched, hipStream_t s) {
-    if (grid < 1 || grid > MP_LSM_MAX_BLOCKS || block != 256) return hipErrorInvalidValue;
+    if (grid < 1 || grid > MP_LSM_MAX_BLOCKS || (block != 256 && block != 512 && block != 1024))
+        return hipErrorInvalidValue;
     if (matched)
         lsm_step_m<true>(p, paths, cash, ex, coeff_all, valid_all, step_update, step_gram,
                          gram_partial, counters, precision, grid, block, s);
@@ -2644,7 +2694,8 @@ hipError_t launch_lsm_group(const mp_lsm_params* p, const double* partial, int n
 hipError_t launch_lsm_final(const mp_lsm_params* p, const double* cash, const uint16_t* ex,
                             const int* valid_all, double* partial, unsigned int* counter,
                             double* out, int grid, int block, hipStream_t s) {
-    if (grid < 1 || grid > MP_LSM_MAX_BLOCKS || block != 256) return hipErrorInvalidValue;
+    if (grid < 1 || grid > MP_LSM_MAX_BLOCKS || (block != 256 && block != 512 && block != 1024))
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(lsm_final_kernel, dim3(grid), dim3(block), 0, s, p, cash, ex, valid_all,
                        partial, counter, out);
     return hipGetLastError();

@@ -58,7 +58,7 @@ CASES = [
     # not represent (float(k) < k): idle lanes must stay out of the regression
     ("american_put", 10_000, 50, 32, {"k": 116.10130597591788}),
     ("american_put", 22_725, 50, 32, {"k": 119.84186}),
-    ("american_put", 1_189_060, 50, 32, {"k": 84.8445}),
+    ("american_put", 1_047_001, 50, 32, {"k": 84.8445}),
     ("american_put", 153_242, 50, 32, {"k": 97.218}),
 ]
 
@@ -118,3 +118,28 @@ def test_plain_per_date_launches_match_oracle(plain_engine, oracle):
     assert abs(res.price - ref["price"]) <= 1e-10 * ref["price"]
     assert abs(res.stderr - ref["stderr"]) <= 1e-8 * ref["stderr"]
     assert np.mean(ex == ref["exercise_steps"]) >= 0.9999
+
+
+def test_one_launch_timeout_falls_back_to_per_date_launches(engine):
+    # A context whose one-launch waits give up after a single poll (forced
+    # from the host; every block records the timeout and runs to its end):
+    # the pricing is repeated with the matched per-date launches, which give
+    # the same bits as an undisturbed one-launch pricing.
+    os.environ["MONTEPATH_LSM_SPIN_LIMIT"] = "1"
+    try:
+        eng = Engine(0)
+    finally:
+        del os.environ["MONTEPATH_LSM_SPIN_LIMIT"]
+    try:
+        for n, prec in ((1 << 18, 32), (50_000, 64)):
+            print(f"[lsm] spin-limited n={n} fp{prec}", flush=True)
+            req = make_request("american_put", n_paths=n, n_steps=50, precision=prec, seed=8, **LSM)
+            want = engine.price(req)
+            assert engine.last_timing()["main_launches"] == 3
+            got = eng.price(req)
+            assert eng.last_timing()["main_launches"] > 3  # the per-date launches ran
+            assert (got.price, got.stderr, got.early_exercise_count, got.maturity_exercise_count,
+                    got.regression_steps) == (want.price, want.stderr, want.early_exercise_count,
+                                              want.maturity_exercise_count, want.regression_steps)
+    finally:
+        eng.close()
