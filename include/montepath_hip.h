@@ -280,8 +280,8 @@ int mp_hip_device_count(int* count);
  * launches that give the one-launch pricing's bits and never wait on other
  * blocks (for contexts that share the device with concurrent work; the
  * one-launch form needs its whole grid resident), =2 with the plain
- * per-date launches; MONTEPATH_QMC_STACK_BRIDGE=1 keeps FP32 Brownian
- * bridges in LDS (same bits as the register form). */
+ * per-date launches; MONTEPATH_QMC_STACK_BRIDGE=1 keeps Brownian bridges
+ * on 16 * 2^J steps in LDS (same bits as the register form). */
 int mp_hip_open(int device_ordinal, mp_ctx** out);
 
 /* Release the context and every device buffer it owns. */

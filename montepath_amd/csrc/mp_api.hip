@@ -966,9 +966,9 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     if (P.bridge) bridge_plan((uint64_t)d.n_steps, r.t, ops, n_slots);
     qp.n_ops = (int)ops.size();
     qp.n_slots = n_slots;
-    // FP32 bridges on 16 * 2^J steps run in registers when the Sobol tables
-    // sit in LDS (no LDS stack)
-    const int tree = P.bridge && precision == 32 && c->qmc_tree ? mp::qmc_tree_levels((uint64_t)d.n_steps) : -1;
+    // bridges on 16 * 2^J steps run in registers when the Sobol tables sit
+    // in LDS (no LDS stack)
+    const int tree = P.bridge && c->qmc_tree ? mp::qmc_tree_levels((uint64_t)d.n_steps) : -1;
     qp.tree_levels = -1;
     // block size: largest power of two <= 1024 whose LDS fits
     // block size: largest power of two <= 1024 whose LDS (plus 2 KB of static
@@ -976,12 +976,14 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     // per-block LDS cannot hold them
     const size_t reserve = 2048;
     bool nib_lds = true;
-    int block = 1024;
+    // FP64 kernels are compiled for blocks of at most 512 threads
+    const int max_block = precision == 64 ? 512 : 1024;
+    int block = max_block;
     while (block > 64 && mp::qmc_lds_bytes(precision, P.factors, n_slots, block, true) + reserve > c->max_lds)
         block /= 2;
     if (mp::qmc_lds_bytes(precision, P.factors, n_slots, block, true) + reserve > c->max_lds) {
         nib_lds = false;
-        block = 1024;
+        block = max_block;
         while (block > 64 && mp::qmc_lds_bytes(precision, P.factors, n_slots, block, false) + reserve > c->max_lds)
             block /= 2;
         if (mp::qmc_lds_bytes(precision, P.factors, n_slots, block, false) + reserve > c->max_lds)
@@ -990,7 +992,7 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     if (nib_lds && tree >= 0) {
         qp.tree_levels = tree;
         n_slots = 0;
-        block = 1024;
+        block = max_block;
     }
     const size_t lds = mp::qmc_lds_bytes(precision, P.factors, n_slots, block, nib_lds) + reserve;
     const int per_cu = std::max<int>(1, (int)(c->max_lds / std::max<size_t>(lds, 1)));

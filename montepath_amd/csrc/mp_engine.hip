@@ -1280,8 +1280,11 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
     }
 }
 
+// FP64 blocks hold at most 512 threads (the bridge stack's LDS and the
+// registers of FP64 paths), so the FP64 instantiations are compiled for
+// 512-thread blocks and get twice the registers.
 template <class R, int F, bool NIB_LDS, int PAY, int SCH>
-__global__ __launch_bounds__(1024) void qmc_kernel(const mp_dreq* __restrict__ reqp, mp_qmc_params qp,
+__global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void qmc_kernel(const mp_dreq* __restrict__ reqp, mp_qmc_params qp,
                                                   const uint32_t* __restrict__ nib_g,
                                                   const mp_bbop* __restrict__ plan, double* partial,
                                                   double* terminal_out, double* payoff_out) {
@@ -1297,6 +1300,15 @@ __global__ __launch_bounds__(1024) void qmc_tree_kernel(const mp_dreq* __restric
                      const uint32_t* __restrict__ nib_g, const mp_bbop* __restrict__ plan,
                      double* partial, double* terminal_out, double* payoff_out) {
     qmc_body<float, F, true, PAY, SCH, true>(reqp, qp, nib_g, plan, partial, terminal_out, payoff_out);
+}
+
+// The FP64 register bridge, 512-thread blocks: the same bits as the FP64
+// LDS-stack kernel without its stack traffic.
+template <int F, int PAY, int SCH>
+__global__ __launch_bounds__(512) void qmc_tree64_kernel(const mp_dreq* __restrict__ reqp, mp_qmc_params qp,
+                     const uint32_t* __restrict__ nib_g, const mp_bbop* __restrict__ plan,
+                     double* partial, double* terminal_out, double* payoff_out) {
+    qmc_body<double, F, true, PAY, SCH, true>(reqp, qp, nib_g, plan, partial, terminal_out, payoff_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -2403,12 +2415,14 @@ template <class R, int F, int PAY, int SCH>
 static void qmc_launch_nl(bool nib_lds, dim3 g, dim3 b, size_t lds, hipStream_t s,
                           const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,
                           const mp_bbop* plan, double* partial, double* term, double* pay) {
-    if constexpr (sizeof(R) == 4) {
-        if (nib_lds && qp.bridge && qp.tree_levels >= 0) {
+    if (nib_lds && qp.bridge && qp.tree_levels >= 0) {
+        if constexpr (sizeof(R) == 4)
             hipLaunchKernelGGL((qmc_tree_kernel<F, PAY, SCH>), g, b, lds, s, req, qp, nib, plan, partial,
                                term, pay);
-            return;
-        }
+        else
+            hipLaunchKernelGGL((qmc_tree64_kernel<F, PAY, SCH>), g, b, lds, s, req, qp, nib, plan, partial,
+                               term, pay);
+        return;
     }
     if (nib_lds)
         hipLaunchKernelGGL((qmc_kernel<R, F, true, PAY, SCH>), g, b, lds, s, req, qp, nib, plan,

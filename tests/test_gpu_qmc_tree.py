@@ -1,5 +1,5 @@
-"""FP32 Sobol + Brownian-bridge pricing with the bridge in registers (step
-counts 16 * 2^J) against the LDS-stack kernel.
+"""Sobol + Brownian-bridge pricing (FP32 and FP64) with the bridge in
+registers (step counts 16 * 2^J) against the LDS-stack kernel.
 
 Both consume the bridge plan in the same preorder with the same arithmetic,
 so the bar is bit-identical per-path terminal values and prices.  The stack
@@ -43,11 +43,12 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("precision", [32, 64])
 @pytest.mark.parametrize("kind,m,extra", CASES)
-def test_register_bridge_equals_stack_bridge(engine, stack_engine, kind, m, extra):
+def test_register_bridge_equals_stack_bridge(engine, stack_engine, kind, m, extra, precision):
     n = 5000
-    print(f"[qmc-tree] {kind} m={m} {sorted(extra)}", flush=True)
-    req = make_request(kind, n_paths=n, n_steps=m, precision=32, seed=9,
+    print(f"[qmc-tree] {kind} m={m} fp{precision} {sorted(extra)}", flush=True)
+    req = make_request(kind, n_paths=n, n_steps=m, precision=precision, seed=9,
                        sampling="scrambled_sobol_brownian_bridge", **extra)
     a, term_a, pay_a = engine.price_with_paths(req)
     b, term_b, pay_b = stack_engine.price_with_paths(req)
