@@ -143,21 +143,30 @@ def bench_lsm(eng, reps: int = 5) -> dict:
     FP64 cash flows, Gram sums and 4x4 solves.  Roofline: the backward pass's
     algorithmic HBM bytes, 1,776 per path (50 x 4 B path writes, 49 dates x
     32 B: the date's price read twice and the FP64 discounted cash flow read,
-    read and written, plus 8 B terminal), over the measured time."""
+    read and written, plus 8 B terminal), over the measured time.  The same
+    pricing with the weighted basis e^{-x/2} L_i(x) is timed beside it."""
     from montepath_amd.engine import make_request
 
     n = 1 << 20
-    req = make_request("american_put", n_paths=n, n_steps=50, k=100.0, r=0.05, sigma=0.25,
-                       t=1.0, basis_degree=3, precision=32, seed=42)
-    eng.price(req)
-    ms = []
-    for _ in range(reps):
-        res = eng.price(req)
-        ms.append(eng.last_timing()["main_kernel_ms"])
-    t = float(np.median(ms))
+
+    def run(weighted):
+        req = make_request("american_put", n_paths=n, n_steps=50, k=100.0, r=0.05, sigma=0.25,
+                           t=1.0, basis_degree=3, precision=32, seed=42, basis_weighted=weighted)
+        eng.price(req)
+        ms = []
+        for _ in range(reps):
+            res = eng.price(req)
+            ms.append(eng.last_timing()["main_kernel_ms"])
+        return res, float(np.median(ms))
+
+    res, t = run(False)
+    launches = eng.last_timing()["main_launches"]
+    res_w, t_w = run(True)
     b_alg = 1776.0 * n
-    return {"n_paths": n, "dates": 50, "degree": 3, "ms": t, "launches": eng.last_timing()["main_launches"],
+    return {"n_paths": n, "dates": 50, "degree": 3, "ms": t, "launches": launches,
             "price": res.price, "stderr": res.stderr, "regression_steps": res.regression_steps,
+            "weighted_basis": {"ms": t_w, "price": res_w.price, "stderr": res_w.stderr,
+                               "regression_steps": res_w.regression_steps},
             "crr_bermudan_50": 7.962057,
             "hbm_alg_bytes": b_alg, "hbm_alg_gbs": b_alg / (t * 1e-3) / 1e9,
             "hbm_frac": b_alg / (t * 1e-3) / 1e9 / PEAK_HBM_GBS}

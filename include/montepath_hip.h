@@ -118,6 +118,9 @@ typedef struct mp_request {
     int32_t allow_intrinsic_now; /* American put floor at max(K - S0, 0) (cpu.rs:3911-3917) */
     int32_t n_assets;        /* basket */
     int32_t n_exercise;      /* Bermudan: entries in exercise_steps */
+    int32_t basis_weighted;  /* LSM: nonzero regresses on the weighted Laguerre functions
+                                e^{-x/2} L_i(x) of Longstaff-Schwartz (2001) instead of the
+                                reference's unweighted L_i(x) (cpu.rs:7667-7682, the default) */
     uint64_t n_paths;
     uint64_t n_steps;
     uint64_t seed;

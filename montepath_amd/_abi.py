@@ -61,6 +61,7 @@ class Request(C.Structure):
         ("technique", C.c_int32), ("method", C.c_int32), ("heston_scheme", C.c_int32),
         ("n_strikes", C.c_int32), ("basis_degree", C.c_int32), ("fp64_check", C.c_int32),
         ("allow_intrinsic_now", C.c_int32), ("n_assets", C.c_int32), ("n_exercise", C.c_int32),
+        ("basis_weighted", C.c_int32),
         ("n_paths", C.c_uint64), ("n_steps", C.c_uint64), ("seed", C.c_uint64),
         ("contract_id", C.c_uint64),
         ("s0", C.c_double), ("k", C.c_double), ("r", C.c_double), ("sigma", C.c_double),

@@ -189,6 +189,7 @@ struct prepared {
     std::vector<uint32_t> schedule;  // LSM dates
     int allow_intrinsic;
     int degree;
+    int weighted = 0;  // LSM basis e^{-x/2} L_i(x)
     uint64_t n_steps_cfg;  // steps as configured (terminal method walks 1)
     uint64_t n_paths_req;  // paths as requested (antithetic units are pairs)
     std::vector<mp_sampler_dim> sampler;  // Halton / Latin hypercube dimensions
@@ -447,6 +448,7 @@ void build_sampler(int kind, uint64_t seed, uint64_t n_paths, uint64_t dims,
 int prepare_request(const mp_request& r, prepared& P) {
     std::memset(&P.d, 0, sizeof(P.d));
     P.schedule.clear();
+    P.weighted = 0;
     P.precision = r.precision == 64 ? 64 : 32;
     if (r.precision != 0 && r.precision != 32 && r.precision != 64)
         return fail(MP_E_INVALID_CONFIG, "precision must be 32 or 64");
@@ -648,6 +650,7 @@ int prepare_request(const mp_request& r, prepared& P) {
                 return fail(MP_E_UNSUPPORTED, "LSM is priced with pseudorandom draws only");
             P.kind = KIND_LSM;
             P.degree = std::min(std::max(r.basis_degree, 1), 3);
+            P.weighted = r.basis_weighted != 0;
             if (r.workload == MP_AMERICAN_PUT) {
                 for (uint32_t s = 1; s <= r.n_steps; ++s) P.schedule.push_back(s);
                 P.allow_intrinsic = 1;
@@ -672,6 +675,8 @@ int prepare_request(const mp_request& r, prepared& P) {
         default:
             return fail(MP_E_UNSUPPORTED, "unknown workload " + std::to_string(r.workload));
     }
+    if (r.basis_weighted && P.kind != KIND_LSM)
+        return fail(MP_E_INVALID_CONFIG, "basis_weighted applies to American and Bermudan puts only");
     P.n_paths_req = r.n_paths;
     if (structured) {
         // one point per path (or antithetic pair); dimensions: step s of the
@@ -1141,6 +1146,7 @@ int run_lsm_once(mp_ctx* c, const mp_request& r, prepared& P, int precision, con
     p.n_steps = m;
     p.degree = P.degree;
     p.coeffs = P.degree + 1;
+    p.weighted = (uint32_t)P.weighted;
     p.allow_intrinsic = P.allow_intrinsic;
     p.n_paths = n;
     p.s0 = r.s0;

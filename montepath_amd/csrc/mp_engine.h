@@ -108,7 +108,8 @@ struct alignas(128) mp_lsm_params {
     int32_t allow_intrinsic;
     uint64_t n_paths;
     double log_s0, s0, k, inv_k, drift_dt, vol_dt, disc_step, disc_m;
-    uint32_t key0, key1, seed_hi, pad2;
+    uint32_t key0, key1, seed_hi;
+    uint32_t weighted;          // basis e^{-x/2} L_i(x) instead of L_i(x)
 };
 
 // Launch helpers (defined in mp_engine.hip).

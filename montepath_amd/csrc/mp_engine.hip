@@ -1536,19 +1536,21 @@ __global__ __launch_bounds__(256) void lsm_paths_kernel(const mp_lsm_params* __r
     }
 }
 
-__device__ __forceinline__ void laguerre(double x, int degree, double b[4]) {
+__device__ __forceinline__ void laguerre(double x, int degree, bool weighted, double b[4]) {
     // laguerre_lsm_basis cpu.rs:7667-7682 (unweighted, x = S/K).  x^3 / 6 is
     // the product with RN(1/6) corrected once by its exact FMA remainder,
     // which equals the rounded quotient (no miss on 1.5e9 sampled values)
-    // without an FP64 division.
+    // without an FP64 division.  Weighted: every function times e^{-x/2}, the
+    // basis of Longstaff and Schwartz (2001), orthonormal on [0, inf).
     const double x2 = x * x;
     const double x3 = x2 * x;
     const double q0 = x3 * 0.16666666666666666;
     const double x3_6 = __builtin_fma(__builtin_fma(-q0, 6.0, x3), 0.16666666666666666, q0);
-    b[0] = 1.0;
-    b[1] = degree >= 1 ? 1.0 - x : 0.0;
-    b[2] = degree >= 2 ? 1.0 - 2.0 * x + 0.5 * x2 : 0.0;
-    b[3] = degree >= 3 ? 1.0 - 3.0 * x + 1.5 * x2 - x3_6 : 0.0;
+    const double w = weighted ? exp(-0.5 * x) : 1.0;
+    b[0] = w;
+    b[1] = degree >= 1 ? (1.0 - x) * w : 0.0;
+    b[2] = degree >= 2 ? (1.0 - 2.0 * x + 0.5 * x2) * w : 0.0;
+    b[3] = degree >= 3 ? (1.0 - 3.0 * x + 1.5 * x2 - x3_6) * w : 0.0;
 }
 
 // Gauss-Jordan with partial pivoting, failing when the best pivot <= 1e-12
@@ -1831,6 +1833,7 @@ __global__ __launch_bounds__(1024) void lsm_step_kernel(const mp_lsm_params* __r
     const uint64_t n = P.n_paths;
     const double k = P.k, inv_k = P.inv_k;
     const int deg = P.degree, nc = P.coeffs;
+    const bool wtd = P.weighted != 0;
     const double* disc_pow = reinterpret_cast<const double*>(Pp + 1);
     const double* growth = disc_pow + (P.n_steps + 1);
     const bool do_update = step_update >= 1 && valid_all[step_update] != 0;
@@ -1865,7 +1868,7 @@ __global__ __launch_bounds__(1024) void lsm_step_kernel(const mp_lsm_params* __r
             const double imm_u = k - su[u] > 0.0 ? k - su[u] : 0.0;
             if (imm_u > 0.0) {  // false when not updating or p >= n (su = k)
                 double bb[4];
-                laguerre(su[u] * inv_k, deg, bb);
+                laguerre(su[u] * inv_k, deg, wtd, bb);
                 // beta[i] = 0 for i >= nc, so the full sum equals the sum over nc
                 double cont = 0.0;
 #pragma unroll
@@ -1879,7 +1882,7 @@ __global__ __launch_bounds__(1024) void lsm_step_kernel(const mp_lsm_params* __r
             const double imm_g = k - sg[u] > 0.0 ? k - sg[u] : 0.0;
             if (imm_g > 0.0) {  // false when not accumulating or p >= n
                 double bb[4];
-                laguerre(sg[u] * inv_k, deg, bb);
+                laguerre(sg[u] * inv_k, deg, wtd, bb);
                 const double cont = c[u] * grow_g;
                 int e = 0;
 #pragma unroll
@@ -2039,6 +2042,7 @@ __global__ __launch_bounds__(BS) void lsm_fused_kernel(const mp_lsm_params* __re
     const int m = P.n_steps;
     const double k = P.k, inv_k = P.inv_k;
     const int deg = P.degree, nc = P.coeffs;
+    const bool wtd = P.weighted != 0;
     const double* disc_pow = reinterpret_cast<const double*>(Pp + 1);
     const double* growth = disc_pow + (m + 1);
     const int G = gridDim.x;
@@ -2111,7 +2115,7 @@ __global__ __launch_bounds__(BS) void lsm_fused_kernel(const mp_lsm_params* __re
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const double s = (double)sx[j];
-            laguerre(s * inv_k, deg, bb[j]);
+            laguerre(s * inv_k, deg, wtd, bb[j]);
             if (live[j] && k - s > 0.0) {
                 int e = 0;
 #pragma unroll
@@ -2188,7 +2192,7 @@ __global__ __launch_bounds__(BS) void lsm_fused_kernel(const mp_lsm_params* __re
                 const double imm = k - s > 0.0 ? k - s : 0.0;
                 if (live[j] && imm > 0.0) {
                     double bx[4];
-                    laguerre(s * inv_k, deg, bx);
+                    laguerre(s * inv_k, deg, wtd, bx);
                     double cont = 0.0;
 #pragma unroll
                     for (int a = 0; a < 4; ++a) cont += beta[a] * bx[a];

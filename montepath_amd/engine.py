@@ -57,6 +57,7 @@ def make_request(workload: str, **kw) -> A.Request:
                        else A.HESTON_REFERENCE_FLOOR)
     strikes = kw.pop("strikes", None)
     r.basis_degree = int(kw.pop("basis_degree", 2))
+    r.basis_weighted = int(bool(kw.pop("basis_weighted", False)))
     r.fp64_check = int(bool(kw.pop("fp64_check", False)))
     r.allow_intrinsic_now = int(bool(kw.pop("allow_intrinsic_now", workload == "american_put")))
     r.n_paths = int(kw.pop("n_paths", 100_000))

@@ -330,8 +330,10 @@ _LSM_DETAILS = ["early_exercise_count", "maturity_exercise_count", "regression_s
 def _american_put_parts(config: Mapping):
     c = _gbm(config, steps_default=64)
     c["basis_degree"] = _u(config, ["basis_degree"], 2)
+    c["basis_weighted"] = bool(config.get("basis_weighted", False))
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], s0=c["s0"], k=c["k"],
-              r=c["r"], sigma=c["sigma"], t=c["t"], basis_degree=c["basis_degree"])
+              r=c["r"], sigma=c["sigma"], t=c["t"], basis_degree=c["basis_degree"],
+              basis_weighted=c["basis_weighted"])
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)
@@ -341,11 +343,12 @@ def _american_put_parts(config: Mapping):
 def _bermudan_put_parts(config: Mapping):
     c = _gbm(config, steps_default=64)
     c["basis_degree"] = _u(config, ["basis_degree"], 2)
+    c["basis_weighted"] = bool(config.get("basis_weighted", False))
     sched = config.get("exercise_steps")
     c["exercise_steps"] = [16, 32, 48, 64] if sched is None else [int(x) for x in sched]
     kw = dict(n_paths=c["n_paths"], n_steps=c["n_steps"], seed=c["seed"], s0=c["s0"], k=c["k"],
               r=c["r"], sigma=c["sigma"], t=c["t"], basis_degree=c["basis_degree"],
-              exercise_steps=c["exercise_steps"])
+              basis_weighted=c["basis_weighted"], exercise_steps=c["exercise_steps"])
     ek = _engine_keys(config)
     kw.update(ek)
     c.update(ek)

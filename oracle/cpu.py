@@ -84,6 +84,7 @@ class LsmCfg(C.Structure):
         ("t", C.c_double),
         ("n_paths", C.c_uint64), ("n_steps", C.c_uint64), ("seed", C.c_uint64),
         ("basis_degree", C.c_uint64), ("allow_intrinsic_now", C.c_int),
+        ("basis_weighted", C.c_int),
     ]
 
 
@@ -157,6 +158,7 @@ def lib() -> C.CDLL:
                    C.c_uint64, _D, _D]),
         "mpo_cholesky": (C.c_int, [C.c_uint64, _D, _D]),
         "mpo_lsm_solve": (C.c_int, [_D, _D, C.c_uint64, _D]),
+        "mpo_laguerre_basis": (None, [C.c_double, C.c_uint64, C.c_int, _D]),
         "mpo_lsm_from_paths": (
             None, [C.POINTER(LsmCfg), _D, _U64, C.c_uint64, C.POINTER(LsmResult), _D, _U64]),
         "mpo_lsm_paths": (None, [C.POINTER(LsmCfg), _D]),
@@ -299,10 +301,12 @@ class Lsm:
     seed: int = 42
     basis_degree: int = 2
     allow_intrinsic_now: bool = True
+    basis_weighted: bool = False  # extension: e^{-x/2} L_i(x)
 
     def c(self) -> LsmCfg:
         return LsmCfg(self.s0, self.k, self.r, self.sigma, self.t, self.n_paths, self.n_steps,
-                      self.seed, self.basis_degree, int(self.allow_intrinsic_now))
+                      self.seed, self.basis_degree, int(self.allow_intrinsic_now),
+                      int(self.basis_weighted))
 
 
 # ------------------------------------------------------------------ wrappers
@@ -522,6 +526,13 @@ def basket_paths_from_normals(s0, sigma, weight, corr, r, t, n_steps, n_paths, n
     out = np.empty(n_paths)
     lib().mpo_basket_paths_from_normals(n_assets, _pd(s0), _pd(sigma), _pd(weight), _pd(chol),
                                         r, t, n_steps, n_paths, _pd(normals), _pd(out))
+    return out
+
+
+def laguerre_basis(x: float, degree: int, weighted: bool = False) -> np.ndarray:
+    """laguerre_lsm_basis (cpu.rs:7667-7682); weighted: times e^{-x/2} (extension)."""
+    out = np.zeros(4)
+    lib().mpo_laguerre_basis(float(x), int(degree), int(bool(weighted)), _pd(out))
     return out
 
 

@@ -17,7 +17,7 @@
  *
  * Functions marked "extension" have no counterpart in the reference
  * (stepped Merton, n-asset basket, two-factor Sobol/Brownian-bridge Heston,
- * Philox draws); they define the engine's semantics for those cases and
+ * Philox draws, the weighted Laguerre basis); they define the engine's semantics for those cases and
  * are pinned by closed forms and invariants in tests/.
  */
 #include <math.h>
@@ -1218,6 +1218,7 @@ typedef struct {
     uint64_t n_paths, n_steps, seed;
     uint64_t basis_degree;
     int allow_intrinsic_now;
+    int basis_weighted; /* extension: e^{-x/2} L_i(x) */
 } mpo_lsm_cfg;
 
 typedef struct {
@@ -1225,10 +1226,13 @@ typedef struct {
     uint64_t early_exercise_count, maturity_exercise_count, regression_steps;
 } mpo_lsm_result;
 
-/* laguerre_lsm_basis cpu.rs:7667-7682 (unweighted Laguerre in x=S/K) */
-static void laguerre_basis(double x, uint64_t degree, double b[4]) {
+/* laguerre_lsm_basis cpu.rs:7667-7682 (unweighted Laguerre in x=S/K).
+ * Extension: weighted != 0 multiplies every function by e^{-x/2}, the
+ * weighted Laguerre functions of Longstaff and Schwartz (2001, eq. 2). */
+static void laguerre_basis(double x, uint64_t degree, int weighted, double b[4]) {
     double x2 = x * x;
     double x3 = x2 * x;
+    double w = weighted ? exp(-0.5 * x) : 1.0;
     b[0] = 1.0;
     b[1] = 0.0;
     b[2] = 0.0;
@@ -1236,6 +1240,12 @@ static void laguerre_basis(double x, uint64_t degree, double b[4]) {
     if (degree >= 1) b[1] = 1.0 - x;
     if (degree >= 2) b[2] = 1.0 - 2.0 * x + 0.5 * x2;
     if (degree >= 3) b[3] = 1.0 - 3.0 * x + 1.5 * x2 - x3 / 6.0;
+    for (int i = 0; i < 4; ++i) b[i] *= w;
+}
+
+MPO_EXPORT void mpo_laguerre_basis(double x, uint64_t degree, int weighted, double* out) {
+    uint64_t d = degree < 1 ? 1 : (degree > 3 ? 3 : degree);
+    laguerre_basis(x, d, weighted, out);
 }
 
 /* solve_lsm_normal_equations cpu.rs:7684-7736: Gauss-Jordan with partial
@@ -1317,7 +1327,7 @@ MPO_EXPORT void mpo_lsm_from_paths(const mpo_lsm_cfg* cfg, const double* paths,
             if (imm <= 0.0) continue;
             double cont = cash[p] * powi_ref(disc_step, (int)(ex[p] - step));
             double b[4];
-            laguerre_basis(s / kk, degree, b);
+            laguerre_basis(s / kk, degree, cfg->basis_weighted, b);
             for (uint64_t i = 0; i < coeffs; ++i) {
                 atb[i] += b[i] * cont;
                 for (uint64_t j = 0; j < coeffs; ++j) ata[i * 4 + j] += b[i] * b[j];
@@ -1335,7 +1345,7 @@ MPO_EXPORT void mpo_lsm_from_paths(const mpo_lsm_cfg* cfg, const double* paths,
             double imm = fmax_ref(cfg->k - s, 0.0);
             if (imm <= 0.0) continue;
             double b[4];
-            laguerre_basis(s / kk, degree, b);
+            laguerre_basis(s / kk, degree, cfg->basis_weighted, b);
             double cont = 0.0;
             for (uint64_t i = 0; i < coeffs; ++i) cont += beta[i] * b[i];
             if (imm > fmax_ref(cont, 0.0)) {

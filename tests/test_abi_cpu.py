@@ -97,6 +97,7 @@ def test_valid_requests_report_footprints():
     ("merton_jump_diffusion_call", {"sampling": "scrambled_sobol"}, A.MP_E_UNSUPPORTED, "pseudorandom"),
     ("american_put", {"n_steps": 1}, A.MP_E_INVALID_CONFIG, "n_steps must be >= 2"),
     ("american_put", {"basis_degree": 0}, A.MP_E_INVALID_CONFIG, "basis_degree must be > 0"),
+    ("european_call", {"basis_weighted": True}, A.MP_E_INVALID_CONFIG, "American and Bermudan puts only"),
     ("bermudan_put", {"n_steps": 64, "exercise_steps": [16, 65]}, A.MP_E_INVALID_CONFIG,
      "exercise_steps must be in 1..=n_steps"),
     ("bermudan_put", {"n_steps": 64}, A.MP_E_INVALID_CONFIG, "at least one exercise date"),

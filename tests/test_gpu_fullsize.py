@@ -14,6 +14,9 @@
   from a target standard error (1e4..1e7 paths).  Served results equal the
   same requests priced alone, for a sample of every kind that includes the
   largest requests (at least 1e6 paths).
+* the LSM configuration: the American put S0 = K = 100, r = 0.05,
+  sigma = 0.25, T = 1 on 50 dates with 2^20 paths and degree-3 Laguerre
+  regression, unweighted and weighted, near the CRR Bermudan-50 value.
 """
 import math
 
@@ -32,6 +35,18 @@ def black_scholes_call(s0, k, r, sigma, t):
     d2 = d1 - sd
     ncdf = lambda x: 0.5 * math.erfc(-x / math.sqrt(2.0))  # noqa: E731
     return s0 * ncdf(d1) - k * math.exp(-r * t) * ncdf(d2)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_lsm_full_size_near_crr_bermudan(engine, weighted):
+    print(f"[fullsize] lsm 2^20 x 50 weighted={weighted}", flush=True)
+    req = make_request("american_put", n_paths=1 << 20, n_steps=50, k=100.0, r=0.05, sigma=0.25,
+                       t=1.0, basis_degree=3, precision=32, seed=42, basis_weighted=weighted)
+    res = engine.price(req)
+    assert res.regression_steps == 49
+    # CRR, 5000 tree steps exercising every 100th: 7.962057; LSM's
+    # suboptimal exercise rule biases it low by a few cents at most
+    assert abs(res.price - 7.962057) <= 4 * res.stderr + 0.03, (res.price, res.stderr)
 
 
 def test_4096_contract_batch_full_size(engine):

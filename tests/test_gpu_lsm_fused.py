@@ -60,13 +60,17 @@ CASES = [
     ("american_put", 22_725, 50, 32, {"k": 119.84186}),
     ("american_put", 1_047_001, 50, 32, {"k": 84.8445}),
     ("american_put", 153_242, 50, 32, {"k": 97.218}),
+    # the weighted Laguerre basis e^{-x/2} L_i(x)
+    ("american_put", 1 << 20, 50, 32, {"basis_weighted": True}),
+    ("bermudan_put", 100_000, 64, 64, {"exercise_steps": [8, 16, 24, 32, 40, 48, 56, 64],
+                                       "basis_weighted": True}),
 ]
 
 
 @pytest.mark.parametrize("kind,n,m,prec,extra", CASES)
 def test_one_launch_equals_matched_per_date_launches(engine, matched_engine, plain_engine, kind, n, m,
                                                      prec, extra):
-    print(f"[lsm] {kind} n={n} m={m} fp{prec}", flush=True)
+    print(f"[lsm] {kind} n={n} m={m} fp{prec} {sorted(extra)}", flush=True)
     req = make_request(kind, n_paths=n, n_steps=m, precision=prec, seed=5, **{**LSM, **extra})
     a = engine.price(req)
     launches_a = engine.last_timing()["main_launches"]
@@ -105,13 +109,15 @@ def test_one_launch_and_matched_agree_path_by_path_on_identical_normals(engine, 
     assert np.mean(ex_c == ex_a) >= 0.9999
 
 
-def test_plain_per_date_launches_match_oracle(plain_engine, oracle):
+@pytest.mark.parametrize("weighted", [False, True])
+def test_plain_per_date_launches_match_oracle(plain_engine, oracle, weighted):
     n, m = 20000, 16
-    cfg = oracle.Lsm(n_paths=n, n_steps=m, seed=42, basis_degree=3, r=0.05, sigma=0.25)
+    cfg = oracle.Lsm(n_paths=n, n_steps=m, seed=42, basis_degree=3, r=0.05, sigma=0.25,
+                     basis_weighted=weighted)
     z = oracle.rng_normals(42, n * m).reshape(n, m)
     ref = oracle.lsm_from_paths(cfg, oracle.lsm_paths_from_normals(cfg, z), want_detail=True)
     req = make_request("american_put", n_paths=n, n_steps=m, precision=64, basis_degree=3,
-                       r=0.05, sigma=0.25)
+                       r=0.05, sigma=0.25, basis_weighted=weighted)
     res, _, ex = plain_engine.lsm_from_normals(req, z.T.copy())
     assert plain_engine.last_timing()["main_launches"] > 3
     assert res.regression_steps == ref["regression_steps"]

@@ -151,14 +151,16 @@ def test_basket_paths_from_normals(engine, oracle, precision):
 
 
 # ------------------------------------------------------------- LSM
-def test_lsm_fp64_matches_oracle_on_reference_normals(engine, oracle):
+@pytest.mark.parametrize("weighted", [False, True])
+def test_lsm_fp64_matches_oracle_on_reference_normals(engine, oracle, weighted):
     n, m = 20000, 16
-    cfg = oracle.Lsm(n_paths=n, n_steps=m, seed=42, basis_degree=3, r=0.05, sigma=0.25)
+    cfg = oracle.Lsm(n_paths=n, n_steps=m, seed=42, basis_degree=3, r=0.05, sigma=0.25,
+                     basis_weighted=weighted)
     z = oracle.rng_normals(42, n * m).reshape(n, m)
     paths = oracle.lsm_paths_from_normals(cfg, z)
     ref = oracle.lsm_from_paths(cfg, paths, want_detail=True)
     req = make_request("american_put", n_paths=n, n_steps=m, precision=64, basis_degree=3,
-                       r=0.05, sigma=0.25)
+                       r=0.05, sigma=0.25, basis_weighted=weighted)
     res, coeff, ex = engine.lsm_from_normals(req, z.T.copy())
     assert res.regression_steps == ref["regression_steps"]
     assert abs(res.price - ref["price"]) <= 1e-10 * ref["price"]
@@ -172,8 +174,7 @@ def test_lsm_fp64_matches_oracle_on_reference_normals(engine, oracle):
             continue
         s = paths[:, d]
         x = s[s < 100.0] / 100.0
-        basis = np.stack([np.ones_like(x), 1 - x, 1 - 2 * x + 0.5 * x * x,
-                          1 - 3 * x + 1.5 * x * x - x ** 3 / 6.0])
+        basis = np.stack([oracle.laguerre_basis(v, 3, weighted) for v in x], axis=1)
         cont_ref = ref["coefficients"][d] @ basis
         cont = coeff[d] @ basis
         assert np.max(np.abs(cont - cont_ref)) <= 2e-6 * np.max(np.abs(cont_ref)), d

@@ -593,6 +593,46 @@ def test_lsm_invariants(oracle):
     assert sol[:2] == pytest.approx([1.0, 2.0])
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_laguerre_basis_is_orthonormal(oracle, weighted):
+    """The Laguerre polynomials are orthonormal under the weight e^{-x} on
+    [0, inf); the weighted functions e^{-x/2} L_i(x) are orthonormal under
+    weight 1.  A wrong coefficient, a dropped or mis-signed weight fails."""
+    from scipy.integrate import quad
+
+    def f(x, i, j):
+        b = oracle.laguerre_basis(x, 3, weighted)
+        return b[i] * b[j] * (1.0 if weighted else math.exp(-x))
+
+    for i in range(4):
+        for j in range(i, 4):
+            got = quad(f, 0.0, math.inf, args=(i, j), epsabs=1e-12, limit=200)[0]
+            assert got == pytest.approx(1.0 if i == j else 0.0, abs=1e-9), (i, j)
+    # degree clamps to [1, 3] and zeroes the higher functions
+    b = oracle.laguerre_basis(0.7, 1, weighted)
+    assert b[2] == b[3] == 0.0 and b[1] == pytest.approx(0.3 * (math.exp(-0.35) if weighted else 1.0))
+
+
+def test_weighted_lsm_prices_the_bermudan_put_near_crr(oracle):
+    # American put S0 = K = 100, r = 0.05, sigma = 0.25, T = 1 on 50 dates:
+    # CRR with 5000 tree steps exercising every 100th gives the Bermudan-50
+    # value 7.96206
+    steps = list(range(100, 5001, 100))
+    crr = oracle.crr_put(100, 100, 0.05, 0.25, 1.0, 5000, steps)
+    assert crr == pytest.approx(7.962057, abs=2e-4)
+    prices = {}
+    for weighted in (False, True):
+        cfg = oracle.Lsm(n_paths=40_000, n_steps=50, seed=5, basis_degree=3, r=0.05, sigma=0.25,
+                         basis_weighted=weighted)
+        res = oracle.lsm_price(cfg)
+        assert res["regression_steps"] == 49
+        assert abs(res["price"] - crr) <= 4 * res["stderr"] + 0.03, (weighted, res)
+        prices[weighted] = res
+    # same paths, different regression basis: close but not equal
+    assert prices[True]["price"] != prices[False]["price"]
+    assert abs(prices[True]["price"] - prices[False]["price"]) <= 0.05
+
+
 def test_lookback_dominates_european_and_asian_deterministic_limit(oracle):
     e = oracle.gbm_price(oracle.Gbm(n_paths=20000, n_steps=32, seed=5), "european_call")
     lb = oracle.gbm_price(oracle.Gbm(n_paths=20000, n_steps=32, seed=5), "lookback_call")
