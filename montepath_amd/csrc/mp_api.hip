@@ -775,6 +775,7 @@ struct mp_batch {
     uint32_t n_tiles = 0;
     int grid = 1, block = 256, ppl = 1;
     bool mlmc = false;
+    bool gbm_f32 = false;  // every request takes the FP32 GBM-family Philox body
     dbuf d_reqs, d_tile_req, d_order, d_partial, d_counter, d_sampler;
     hbuf h_stage;  // request records, tile map, queue order, sampler tables
     hbuf h_out;    // result rows, written by finalize_kernel
@@ -793,6 +794,12 @@ struct mp_batch {
 };
 
 namespace {
+
+// A tile the FP32 GBM-family Philox body prices (tile_gbm_kernel).
+bool gbm_philox_tile(const mp_dreq& d) {
+    return d.model == MP_MODEL_GBM && d.technique == MP_TECH_STANDARD && d.sampling == MP_SAMPLER_NONE &&
+           d.payoff >= MP_PAY_EUROPEAN && d.payoff <= MP_PAY_LOOKBACK;
+}
 
 int batch_build(mp_batch* b) {
     const size_t n = b->reqs.size();
@@ -855,6 +862,7 @@ int batch_build(mp_batch* b) {
         return fail(MP_E_INVALID_CONFIG, "multilevel levels run in batches of their own");
     b->mlmc = n_mlmc != 0;
     if (b->mlmc) b->ppl = 1;  // the multilevel kernel runs one path per lane
+    b->gbm_f32 = b->precision == 32 && std::all_of(dreqs.begin(), dreqs.end(), gbm_philox_tile);
     b->grid = grid_for_tiles(b->ctx, tiles, b->block);
     return MP_OK;
 }
@@ -875,6 +883,7 @@ int batch_enqueue(mp_batch* b) {
     L.sampler = b->d_sampler.as<mp_sampler_dim>();
     L.nib = b->ctx->d_nib;
     L.mlmc = b->mlmc ? 1 : 0;
+    L.gbm_f32 = b->gbm_f32 ? 1 : 0;
     L.grid = b->grid;
     L.block = b->block;
     L.ppl = b->ppl;
@@ -901,6 +910,7 @@ int batch_capture(mp_batch* b) {
     L.sampler = b->d_sampler.as<mp_sampler_dim>();
     L.nib = b->ctx->d_nib;
     L.mlmc = b->mlmc ? 1 : 0;
+    L.gbm_f32 = b->gbm_f32 ? 1 : 0;
     L.grid = b->grid;
     L.block = b->block;
     L.ppl = b->ppl;
@@ -1431,6 +1441,7 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     L.sampler = c->sampler.as<mp_sampler_dim>();
     L.nib = c->d_nib;
     L.mlmc = d.payoff == MP_PAY_ASIAN_MLMC ? 1 : 0;
+    L.gbm_f32 = precision == 32 && !buffer && gbm_philox_tile(d) ? 1 : 0;
     L.terminal_out = want_paths ? c->term.as<double>() : nullptr;
     L.payoff_out = want_paths ? c->pay.as<double>() : nullptr;
     L.block = tu.block_size > 0 ? tu.block_size : 256;

@@ -52,22 +52,40 @@ __device__ __forceinline__ mp_u4 mp_philox_wave(uint32_t c0, uint32_t c1, uint32
     uint32_t x3 = (uint32_t)p0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
-    // round 2: M0 * x0 is uniform
+    // round 2: M0 * x0 is uniform.  A vector instruction reads at most one
+    // scalar register, so a pair of uniform XOR operands that changes with
+    // c1 is combined on the scalar unit first (readfirstlane pins the value
+    // there; it is uniform, so the lane read is exact) and the vector side is
+    // one XOR.  Terms that do not depend on c1 stay plain, so a caller's loop
+    // over c1 can hoist them (readfirstlane is not moved out of loops).
     {
         const uint32_t q0_hi = __umulhi(0xD2511F53u, x0);
         const uint32_t q0_lo = 0xD2511F53u * x0;
         const uint64_t q1 = (uint64_t)0xCD9E8D57u * x2;
         const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(q1 >> 32), x1, k0, 0x96);
-        const uint32_t n2 = __builtin_amdgcn_bitop3_b32(q0_hi, x3, k1, 0x96);
+        const uint32_t n2 = x3 ^ __builtin_amdgcn_readfirstlane(q0_hi ^ k1);
         x0 = n0;
         x1 = (uint32_t)q1;
         x2 = n2;
-        x3 = q0_lo;
+        x3 = q0_lo;  // uniform
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    // round 3: x3 is still uniform
+    {
+        const uint64_t p0r = (uint64_t)0xD2511F53u * x0;
+        const uint64_t p1r = (uint64_t)0xCD9E8D57u * x2;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1r >> 32), x1, k0, 0x96);
+        const uint32_t n2 = (uint32_t)(p0r >> 32) ^ __builtin_amdgcn_readfirstlane(x3 ^ k1);
+        x0 = n0;
+        x1 = (uint32_t)p1r;
+        x2 = n2;
+        x3 = (uint32_t)p0r;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
 #pragma unroll
-    for (int round = 2; round < 10; ++round) {
+    for (int round = 3; round < 10; ++round) {
         const uint64_t p0r = (uint64_t)0xD2511F53u * x0;
         const uint64_t p1r = (uint64_t)0xCD9E8D57u * x2;
         const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1r >> 32), x1, k0, 0x96);

@@ -134,6 +134,8 @@ struct tile_launch {
     int ppl;                    // paths interleaved per lane
     int mlmc;                   // every request is a multilevel level (own kernel)
     int claim;                  // queue positions per claim, 0 = one tile per wave (launch_tiles sets it)
+    int gbm_f32;                // every request is FP32 GBM-family with Philox draws and the standard
+                                // technique: kernels holding that body only (fewer registers)
 };
 
 hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s);

@@ -523,14 +523,30 @@ __device__ __forceinline__ void gbm_pair_f32(uint32_t a, uint32_t b, float vscal
     s = __builtin_amdgcn_sinf(u2);
 }
 
+// x plus the sum of a Box-Muller pair's two normals in units of
+// vscale sqrt2: r (cos 2 pi u2 + sin 2 pi u2) = sqrt2 r sin(2 pi (u2 + 1/8)),
+// r = sqrt(-2 ln u1) = sqrt(2 ln 2) sqrt(-log2 u1), in one sine.  u2 + 1/8 is
+// formed in the same subtraction that makes u2 from the float 1 + k 2^-23
+// (the constant 7/8 - 2^-24 is exact); it is exact below 1 and rounded to
+// 2^-23 above.
+__device__ __forceinline__ float gbm_pair_sum_f32(uint32_t a, uint32_t b, float x) {
+    const float u1 = mp_u01f(a);
+    const float w = __uint_as_float(__builtin_amdgcn_alignbit(0x7fu, b, 9)) - 0.87499994039535522f;
+    return __builtin_fmaf(__builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1)), __builtin_amdgcn_sinf(w), x);
+}
+
 // FP32 GBM tile with Philox draws and the standard technique (the batching
 // and serving workload).  The same draws as tile_body; the arithmetic is
 // arranged for the vector unit:
 // * European: only S_T matters, so the log-return is the sum of the
-//   volatility terms plus n_steps * drift_dt, added once at the end;
-// * Asian: the path is carried in log2 units, so S/S0 is one v_exp_f32 per
-//   step; each 4-step block's S/S0 values are added pairwise and the block
-//   sum enters a Kahan-compensated running sum;
+//   volatility terms plus n_steps * drift_dt, added once at the end; full
+//   blocks add each Box-Muller pair's two terms as one sine, without the
+//   volatility factor (applied once at the end);
+// * Asian: the path is carried in log2 units without its drift, y_k =
+//   log2(S_k/S0) - k d, so S_k/S0 = 2^(y_k) 2^(k d) is one v_exp_f32 per step
+//   times a factor: within block b the steps' factors 2^(e d), e = 1..4,
+//   weight the block sum, and the block's 2^(4 b d) (a running FP64
+//   product) multiplies it into an FP64 running sum;
 // * down-and-out / lookback: natural-log path, barrier test / running max;
 // * full 4-step blocks run without per-step predicates; the tail block of
 //   n_steps % 4 steps is predicated.
@@ -549,6 +565,7 @@ __device__ __forceinline__ void gbm_philox_f32_tile(const mp::tile_launch& L, ui
 #pragma unroll
     for (int j = 0; j < MP_NS_MAX; ++j) kk[j] = float(q.strikes[j]);
     const float vscale = float(q.vol_dt * (kLog2 ? 1.1774100225154747 * 1.4426950408889634 : 1.1774100225154747));
+    const float vscale2 = float(q.vol_dt * (1.1774100225154747 * 1.4142135623730951));
     const float drift = kLog2 ? float(q.drift_dt * 1.4426950408889634) : m.drift_dt;
     const float xb = m.xb;
     const uint64_t pbeg = (uint64_t)(tile - q.tile0) * q.tile_paths;
@@ -558,29 +575,43 @@ __device__ __forceinline__ void gbm_philox_f32_tile(const mp::tile_launch& L, ui
     const uint32_t seed_hi = __builtin_amdgcn_readfirstlane(q.seed_hi);
     const uint32_t key0 = __builtin_amdgcn_readfirstlane(q.key0);
     const uint32_t key1 = __builtin_amdgcn_readfirstlane(q.key1);
+    // Asian step factors 2^(e d), e = 1..4, and the block factor 2^(4 d)
+    float cf[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) cf[e] = kLog2 ? __builtin_amdgcn_exp2f(float(e + 1) * drift) : 1.0f;
+    const double cb = kLog2 ? (double)__builtin_amdgcn_exp2f(4.0f * drift) : 1.0;
 
     double sum[MP_NS_MAX], sq[MP_NS_MAX];
 #pragma unroll
     for (int j = 0; j < MP_NS_MAX; ++j) sum[j] = sq[j] = 0.0;
 
     for (uint64_t base = pbeg; base < pend; base += (uint64_t)MP_WAVE * PPL) {
-        float x[PPL], xmax[PPL];
-        mp_kahan<float> acc[PPL];
+        float x[PPL], xs[PPL], xmax[PPL];
+        double acc[PPL];
         bool knocked[PPL];
         uint32_t pid32[PPL];
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             pid32[j] = (uint32_t)(base + lane + (uint64_t)MP_WAVE * j);
             x[j] = 0.0f;
-            acc[j].init();
+            xs[j] = 0.0f;
+            acc[j] = 0.0;
             xmax[j] = 0.0f;
             knocked[j] = q.knocked_at_start != 0;
         }
+        double gb = 1.0;  // Asian: 2^(4 b d) of the current block b
         // one 4-step block; nsub < 4 only for the tail block
         auto block = [&](uint32_t blk, int nsub) {
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
                 const mp_u4 r = mp_philox_wave(pid32[j], blk, 0u, seed_hi, key0, key1);
+                if constexpr (PAY == MP_PAY_EUROPEAN) {
+                    if (nsub == 4) {
+                        xs[j] = gbm_pair_sum_f32(r.x, r.y, xs[j]);
+                        xs[j] = gbm_pair_sum_f32(r.z, r.w, xs[j]);
+                        continue;
+                    }
+                }
                 // step e's volatility term is vr[e / 2] * tr[e]
                 float vr[2], tr[4];
                 gbm_pair_f32(r.x, r.y, vscale, vr[0], tr[0], tr[1]);
@@ -593,10 +624,12 @@ __device__ __forceinline__ void gbm_philox_f32_tile(const mp::tile_launch& L, ui
                     float g[4];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        if (e < nsub) x[j] = __builtin_fmaf(vr[e >> 1], tr[e], x[j] + drift);
+                        if (e < nsub) x[j] = __builtin_fmaf(vr[e >> 1], tr[e], x[j]);
                         g[e] = e < nsub ? __builtin_amdgcn_exp2f(x[j]) : 0.0f;
                     }
-                    acc[j].add((g[0] + g[1]) + (g[2] + g[3]));
+                    const float bs = __builtin_fmaf(
+                        cf[3], g[3], __builtin_fmaf(cf[2], g[2], __builtin_fmaf(cf[1], g[1], cf[0] * g[0])));
+                    acc[j] = __builtin_fma(gb, (double)bs, acc[j]);
                 } else if constexpr (PAY == MP_PAY_DOWN_AND_OUT) {
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
@@ -616,17 +649,22 @@ __device__ __forceinline__ void gbm_philox_f32_tile(const mp::tile_launch& L, ui
                 }
             }
         };
-        for (int b = 0; b < nfull; ++b) block((uint32_t)b, 4);
+        for (int b = 0; b < nfull; ++b) {
+            block((uint32_t)b, 4);
+            if constexpr (kLog2) gb *= cb;
+        }
         if (ntail) block((uint32_t)nfull, ntail);
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             const uint64_t p = base + lane + (uint64_t)MP_WAVE * j;
             if (p >= pend) continue;
             path_state<float> st, sa;
-            st.x = PAY == MP_PAY_EUROPEAN ? x[j] + float(n_steps) * drift
-                                          : (kLog2 ? x[j] * 0.69314718055994531f : x[j]);
+            st.x = PAY == MP_PAY_EUROPEAN ? __builtin_fmaf(vscale2, xs[j], x[j]) + float(n_steps) * drift
+                   : kLog2                ? (x[j] + float(n_steps) * drift) * 0.69314718055994531f
+                                          : x[j];
             st.v = 0.0f;
-            st.acc = acc[j];
+            st.acc.s = (float)acc[j];
+            st.acc.c = 0.0f;
             st.xmax = xmax[j];
             st.knocked = knocked[j];
             sa = st;
@@ -749,6 +787,49 @@ __global__ __launch_bounds__(256) void tile_static_kernel(mp::tile_launch L) {
     for (unsigned int qpos = blockIdx.x * (blockDim.x / MP_WAVE) + (threadIdx.x / MP_WAVE);
          qpos < L.n_tiles; qpos += waves)
         run_tile<R, SRC, 1, ML>(L, qpos, lane);
+}
+
+// The FP32 GBM-family Philox tiles only (the batching and serving
+// workload): the same body run_tile selects for them, in kernels that do not
+// carry the other models' code, so they need fewer registers and hold more
+// waves per SIMD.  Same bits as the general kernels.
+template <int PPL>
+__device__ __forceinline__ void run_tile_gbm(const mp::tile_launch& L, unsigned int qpos, int lane) {
+    const uint32_t tile = L.order ? L.order[qpos] : qpos;
+    const uint32_t ri = L.tile_req ? L.tile_req[tile] : 0u;
+    const mp_dreq& q = L.reqs[ri];
+    const int pay = q.payoff;
+    if (pay == MP_PAY_EUROPEAN)
+        gbm_philox_f32_tile<PPL, MP_PAY_EUROPEAN>(L, tile, q, lane);
+    else if (pay == MP_PAY_ASIAN)
+        gbm_philox_f32_tile<PPL, MP_PAY_ASIAN>(L, tile, q, lane);
+    else if (pay == MP_PAY_DOWN_AND_OUT)
+        gbm_philox_f32_tile<PPL, MP_PAY_DOWN_AND_OUT>(L, tile, q, lane);
+    else
+        gbm_philox_f32_tile<PPL, MP_PAY_LOOKBACK>(L, tile, q, lane);
+}
+
+template <int PPL>
+__global__ __launch_bounds__(256) void tile_gbm_kernel(mp::tile_launch L) {
+    __shared__ unsigned int queue_slot[256 / MP_WAVE];
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    volatile unsigned int* slot = &queue_slot[threadIdx.x >> 6];
+    const unsigned int k = (unsigned int)L.claim;
+    unsigned int claims = 0;
+    for (unsigned int q0 = next_queue_position(L.counter, slot, lane, k);
+         q0 < L.n_tiles && claims <= L.n_tiles;
+         q0 = next_queue_position(L.counter, slot, lane, k), ++claims) {
+        const unsigned int qend = min(q0 + k, L.n_tiles);
+        for (unsigned int qpos = q0; qpos < qend; ++qpos) run_tile_gbm<PPL>(L, qpos, lane);
+    }
+}
+
+__global__ __launch_bounds__(256) void tile_gbm_static_kernel(mp::tile_launch L) {
+    const int lane = threadIdx.x & (MP_WAVE - 1);
+    const unsigned int waves = gridDim.x * (blockDim.x / MP_WAVE);
+    for (unsigned int qpos = blockIdx.x * (blockDim.x / MP_WAVE) + (threadIdx.x / MP_WAVE);
+         qpos < L.n_tiles; qpos += waves)
+        run_tile_gbm<1>(L, qpos, lane);
 }
 
 __global__ void reset_kernel(unsigned int* counter, int n) {
@@ -2355,6 +2436,17 @@ hipError_t launch_setup(const mp_dreq& d, mp_dreq* dst, unsigned int* counter, i
 
 template <class R, int SRC>
 static hipError_t tiles_ppl(const tile_launch& L, hipStream_t s) {
+    if (sizeof(R) == 4 && SRC == 0 && L.gbm_f32 && !L.mlmc) {
+        if (L.claim == 0)
+            hipLaunchKernelGGL(tile_gbm_static_kernel, dim3(L.grid), dim3(L.block), 0, s, L);
+        else if (L.ppl == 2)
+            hipLaunchKernelGGL((tile_gbm_kernel<2>), dim3(L.grid), dim3(L.block), 0, s, L);
+        else if (L.ppl == 4)
+            hipLaunchKernelGGL((tile_gbm_kernel<4>), dim3(L.grid), dim3(L.block), 0, s, L);
+        else
+            hipLaunchKernelGGL((tile_gbm_kernel<1>), dim3(L.grid), dim3(L.block), 0, s, L);
+        return hipGetLastError();
+    }
     if (L.claim == 0) {
         if (L.mlmc)
             hipLaunchKernelGGL((tile_static_kernel<R, SRC, true>), dim3(L.grid), dim3(L.block), 0, s, L);

@@ -37,14 +37,18 @@ def test_philox_fp64_normals_match_oracle(engine, oracle):
     assert float(np.max(np.abs(got - ref))) <= 1e-12
 
 
-def test_philox_fp32_hardware_normals_close_to_fp64(engine, oracle):
-    # v_log_f32 / v_sin_f32 / v_cos_f32 path against the f64 restatement
+@pytest.mark.parametrize("stream,path0,n,m", [(2, 5, 1024, 16), (0, 0, 65536, 64)])
+def test_philox_fp32_hardware_normals_close_to_fp64(engine, oracle, stream, path0, n, m):
+    # v_log_f32 / v_sqrt_f32 / v_sin_f32 / v_cos_f32 path against the f64
+    # restatement on the same uniforms (measured: max 7.1e-7 absolute over
+    # 4.2e6 normals, max 7.7e-7 relative where |z| > 1e-3)
     req = make_request("european_call", seed=99, contract_id=0)
-    got = engine.draws(req, 4, 2, 5, 1024, 16)
-    ref = oracle.philox_normals(99, 0, 2, 5, 1024, 16)
+    got = engine.draws(req, 4, stream, path0, n, m)
+    ref = oracle.philox_normals(99, 0, stream, path0, n, m)
     err = np.abs(got - ref)
-    assert float(np.max(err)) <= 2e-4
-    assert float(np.median(err)) <= 2e-6
+    assert float(np.max(err / np.maximum(np.abs(ref), 1.0))) <= 2e-6
+    assert float(np.max(err[np.abs(ref) > 1e-3] / np.abs(ref[np.abs(ref) > 1e-3]))) <= 3e-6
+    assert float(np.median(err)) <= 1e-7
 
 
 def _sobol_raw_ref(oracle, seed, point, dims):
