@@ -504,13 +504,19 @@ int mp_hip_server_stats(mp_server* srv, mp_serve_stats* stats);
 
 /* ---- tuning --------------------------------------------------------------
  * Kernel launch parameters: block size, paths interleaved per lane, paths
- * per work-queue tile.  mp_hip_autotune times the candidates on a sample of
- * the request and keeps the fastest per (model, payoff, steps bucket). */
+ * per work-queue tile (fixed by the request, reported only) and the step
+ * unroll: 4-step draw blocks per iteration of the step loop (1, 2 or 4; 0
+ * in mp_hip_set_tuning means 1), which selects compiled variants of the
+ * FP32 GBM-family Philox body (the batching workload) and is ignored by the
+ * other bodies.  mp_hip_autotune times the candidates on a sample of the
+ * request and keeps the fastest per (model, payoff, steps bucket,
+ * precision).  No choice changes a result: tiles and the order of every
+ * sum are fixed by the request. */
 typedef struct mp_tuning {
     int32_t block_size;
     int32_t paths_per_lane;
     int32_t tile_paths;
-    int32_t reserved;
+    int32_t step_unroll;
 } mp_tuning;
 
 int mp_hip_autotune(mp_ctx* ctx, const mp_request* req, mp_tuning* best, double* best_ms);

@@ -185,7 +185,7 @@ class ServeStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("paths_per_lane", C.c_int32),
-                ("tile_paths", C.c_int32), ("reserved", C.c_int32)]
+                ("tile_paths", C.c_int32), ("step_unroll", C.c_int32)]
 
 
 class Timing(C.Structure):

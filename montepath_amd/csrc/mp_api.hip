@@ -757,6 +757,7 @@ mp_tuning tuning_for(const mp_ctx* c, const mp_dreq& d, int precision) {
     t.block_size = 256;
     t.paths_per_lane = 1;
     t.tile_paths = (int32_t)d.tile_paths;
+    t.step_unroll = 1;
     return t;
 }
 
@@ -773,7 +774,7 @@ struct mp_batch {
     std::vector<prepared> reqs;     // tile-kernel requests of this batch
     std::vector<size_t> req_index;  // position in the caller's array
     uint32_t n_tiles = 0;
-    int grid = 1, block = 256, ppl = 1;
+    int grid = 1, block = 256, ppl = 1, unroll = 1;
     bool mlmc = false;
     bool gbm_f32 = false;  // every request takes the FP32 GBM-family Philox body
     dbuf d_reqs, d_tile_req, d_order, d_partial, d_counter, d_sampler;
@@ -856,6 +857,7 @@ int batch_build(mp_batch* b) {
     mp_tuning tu = tuning_for(b->ctx, dreqs.empty() ? mp_dreq{} : dreqs[0], b->precision);
     b->block = tu.block_size > 0 ? tu.block_size : 256;
     b->ppl = tu.paths_per_lane > 0 ? tu.paths_per_lane : 1;
+    b->unroll = tu.step_unroll > 0 ? tu.step_unroll : 1;
     size_t n_mlmc = 0;
     for (const mp_dreq& d : dreqs) n_mlmc += d.payoff == MP_PAY_ASIAN_MLMC;
     if (n_mlmc != 0 && n_mlmc != dreqs.size())
@@ -887,6 +889,7 @@ int batch_enqueue(mp_batch* b) {
     L.grid = b->grid;
     L.block = b->block;
     L.ppl = b->ppl;
+    L.unroll = b->unroll;
     HIPCHK(mp::launch_reset(L.counter, 1, b->stream));
     HIPCHK(hipEventRecord(b->e0, b->stream));
     HIPCHK(mp::launch_tiles(L, b->precision, b->stream));
@@ -914,6 +917,7 @@ int batch_capture(mp_batch* b) {
     L.grid = b->grid;
     L.block = b->block;
     L.ppl = b->ppl;
+    L.unroll = b->unroll;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = mp::launch_reset(L.counter, 1, b->stream);
     hipError_t e2 = mp::launch_tiles(L, b->precision, b->stream);
@@ -1446,6 +1450,7 @@ int run_tiles_single(mp_ctx* c, prepared& P, int precision, mp_result* out, cons
     L.payoff_out = want_paths ? c->pay.as<double>() : nullptr;
     L.block = tu.block_size > 0 ? tu.block_size : 256;
     L.ppl = tu.paths_per_lane > 0 && d.payoff != MP_PAY_ASIAN_MLMC ? tu.paths_per_lane : 1;
+    L.unroll = tu.step_unroll > 0 ? tu.step_unroll : 1;
     L.grid = grid_for_tiles(c, d.n_tiles, L.block);
     HIPCHK(mp::launch_setup(d, d_reqs, L.counter, 1, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -2645,11 +2650,14 @@ MP_EXPORT int mp_hip_set_tuning(mp_ctx* c, const mp_request* req, const mp_tunin
         return fail(MP_E_INVALID_CONFIG, "block_size must be 64, 128 or 256");
     if (t->paths_per_lane != 1 && t->paths_per_lane != 2 && t->paths_per_lane != 4)
         return fail(MP_E_INVALID_CONFIG, "paths_per_lane must be 1, 2 or 4");
+    if (t->step_unroll != 0 && t->step_unroll != 1 && t->step_unroll != 2 && t->step_unroll != 4)
+        return fail(MP_E_INVALID_CONFIG, "step_unroll must be 1, 2 or 4 (0 = 1)");
     prepared P;
     int rc = prepare_request(*req, P);
     if (rc != MP_OK) return rc;
     mp_tuning v = *t;
     v.tile_paths = (int32_t)P.d.tile_paths;
+    if (v.step_unroll == 0) v.step_unroll = 1;
     c->tuning[tuning_key(P.d, P.precision)] = v;
     return MP_OK;
 }
@@ -2671,24 +2679,30 @@ MP_EXPORT int mp_hip_autotune(mp_ctx* c, const mp_request* req, mp_tuning* best,
     const uint64_t key = tuning_key(P.d, P.precision);
     const int blocks[] = {128, 256};
     const int ppls[] = {1, 2, 4};
+    // the step unroll selects compiled variants of the FP32 GBM-family
+    // Philox body only; the other bodies have one step loop
+    const bool unrolls = P.precision == 32 && gbm_philox_tile(P.d);
+    const int n_unroll = unrolls ? 3 : 1;
+    const int unroll_of[] = {1, 2, 4};
     double best_t = 1e300;
     mp_tuning bt{};
     for (int bs : blocks)
-        for (int ppl : ppls) {
-            mp_tuning t{bs, ppl, (int32_t)P.d.tile_paths, 0};
-            c->tuning[key] = t;
-            mp_result r;
-            double tmin = 1e300;
-            for (int rep = 0; rep < 3; ++rep) {
-                rc = price_one(c, sample, P.precision, &r);
-                if (rc != MP_OK) return rc;
-                tmin = std::min(tmin, c->last.main_kernel_ms);
+        for (int ppl : ppls)
+            for (int ui = 0; ui < n_unroll; ++ui) {
+                mp_tuning t{bs, ppl, (int32_t)P.d.tile_paths, unroll_of[ui]};
+                c->tuning[key] = t;
+                mp_result r;
+                double tmin = 1e300;
+                for (int rep = 0; rep < 3; ++rep) {
+                    rc = price_one(c, sample, P.precision, &r);
+                    if (rc != MP_OK) return rc;
+                    tmin = std::min(tmin, c->last.main_kernel_ms);
+                }
+                if (tmin < best_t) {
+                    best_t = tmin;
+                    bt = t;
+                }
             }
-            if (tmin < best_t) {
-                best_t = tmin;
-                bt = t;
-            }
-        }
     c->tuning[key] = bt;
     *best = bt;
     if (best_ms) *best_ms = best_t;

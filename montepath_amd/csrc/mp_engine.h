@@ -136,6 +136,7 @@ struct tile_launch {
     int claim;                  // queue positions per claim, 0 = one tile per wave (launch_tiles sets it)
     int gbm_f32;                // every request is FP32 GBM-family with Philox draws and the standard
                                 // technique: kernels holding that body only (fewer registers)
+    int unroll;                 // FP32 GBM-family body: 4-step draw blocks per step-loop iteration (1, 2, 4)
 };
 
 hipError_t launch_reset(unsigned int* counter, int n, hipStream_t s);

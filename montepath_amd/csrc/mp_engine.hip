@@ -551,7 +551,10 @@ __device__ __forceinline__ float gbm_pair_sum_f32(uint32_t a, uint32_t b, float 
 // * full 4-step blocks run without per-step predicates; the tail block of
 //   n_steps % 4 steps is predicated.
 // Payoffs, discounting and sums then follow tile_body.
-template <int PPL, int PAY>
+// U: 4-step blocks per iteration of the step loop (the step unroll the
+// autotuner picks); the blocks run in the same order whatever U is, so every
+// U gives the same bits.
+template <int PPL, int PAY, int U>
 __device__ __forceinline__ void gbm_philox_f32_tile(const mp::tile_launch& L, uint32_t tile,
                                                     const mp_dreq& q, int lane) {
     model_params<float> m = load_params<float>(q);
@@ -649,9 +652,19 @@ __device__ __forceinline__ void gbm_philox_f32_tile(const mp::tile_launch& L, ui
                 }
             }
         };
-        for (int b = 0; b < nfull; ++b) {
-            block((uint32_t)b, 4);
-            if constexpr (kLog2) gb *= cb;
+        int b = 0;
+        for (; b + U <= nfull; b += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                block((uint32_t)(b + u), 4);
+                if constexpr (kLog2) gb *= cb;
+            }
+        }
+        if constexpr (U > 1) {
+            for (; b < nfull; ++b) {
+                block((uint32_t)b, 4);
+                if constexpr (kLog2) gb *= cb;
+            }
         }
         if (ntail) block((uint32_t)nfull, ntail);
 #pragma unroll
@@ -719,13 +732,13 @@ __device__ __forceinline__ void run_tile(const mp::tile_launch& L, unsigned int 
                 tile_body<R, SRC, PPL, -1, -1, -1>(L, tile, q, lane);
             } else if (mdl == MP_MODEL_GBM && sizeof(R) == 4) {
                 if (pay == MP_PAY_EUROPEAN)
-                    gbm_philox_f32_tile<PPL, MP_PAY_EUROPEAN>(L, tile, q, lane);
+                    gbm_philox_f32_tile<PPL, MP_PAY_EUROPEAN, 1>(L, tile, q, lane);
                 else if (pay == MP_PAY_ASIAN)
-                    gbm_philox_f32_tile<PPL, MP_PAY_ASIAN>(L, tile, q, lane);
+                    gbm_philox_f32_tile<PPL, MP_PAY_ASIAN, 1>(L, tile, q, lane);
                 else if (pay == MP_PAY_DOWN_AND_OUT)
-                    gbm_philox_f32_tile<PPL, MP_PAY_DOWN_AND_OUT>(L, tile, q, lane);
+                    gbm_philox_f32_tile<PPL, MP_PAY_DOWN_AND_OUT, 1>(L, tile, q, lane);
                 else
-                    gbm_philox_f32_tile<PPL, MP_PAY_LOOKBACK>(L, tile, q, lane);
+                    gbm_philox_f32_tile<PPL, MP_PAY_LOOKBACK, 1>(L, tile, q, lane);
             } else if (mdl == MP_MODEL_GBM) {
                 if (pay == MP_PAY_EUROPEAN)
                     tile_body<R, SRC, PPL, MP_MODEL_GBM, MP_PAY_EUROPEAN, -1>(L, tile, q, lane);
@@ -793,23 +806,23 @@ __global__ __launch_bounds__(256) void tile_static_kernel(mp::tile_launch L) {
 // workload): the same body run_tile selects for them, in kernels that do not
 // carry the other models' code, so they need fewer registers and hold more
 // waves per SIMD.  Same bits as the general kernels.
-template <int PPL>
+template <int PPL, int U>
 __device__ __forceinline__ void run_tile_gbm(const mp::tile_launch& L, unsigned int qpos, int lane) {
     const uint32_t tile = L.order ? L.order[qpos] : qpos;
     const uint32_t ri = L.tile_req ? L.tile_req[tile] : 0u;
     const mp_dreq& q = L.reqs[ri];
     const int pay = q.payoff;
     if (pay == MP_PAY_EUROPEAN)
-        gbm_philox_f32_tile<PPL, MP_PAY_EUROPEAN>(L, tile, q, lane);
+        gbm_philox_f32_tile<PPL, MP_PAY_EUROPEAN, U>(L, tile, q, lane);
     else if (pay == MP_PAY_ASIAN)
-        gbm_philox_f32_tile<PPL, MP_PAY_ASIAN>(L, tile, q, lane);
+        gbm_philox_f32_tile<PPL, MP_PAY_ASIAN, U>(L, tile, q, lane);
     else if (pay == MP_PAY_DOWN_AND_OUT)
-        gbm_philox_f32_tile<PPL, MP_PAY_DOWN_AND_OUT>(L, tile, q, lane);
+        gbm_philox_f32_tile<PPL, MP_PAY_DOWN_AND_OUT, U>(L, tile, q, lane);
     else
-        gbm_philox_f32_tile<PPL, MP_PAY_LOOKBACK>(L, tile, q, lane);
+        gbm_philox_f32_tile<PPL, MP_PAY_LOOKBACK, U>(L, tile, q, lane);
 }
 
-template <int PPL>
+template <int PPL, int U>
 __global__ __launch_bounds__(256) void tile_gbm_kernel(mp::tile_launch L) {
     __shared__ unsigned int queue_slot[256 / MP_WAVE];
     const int lane = threadIdx.x & (MP_WAVE - 1);
@@ -820,16 +833,17 @@ __global__ __launch_bounds__(256) void tile_gbm_kernel(mp::tile_launch L) {
          q0 < L.n_tiles && claims <= L.n_tiles;
          q0 = next_queue_position(L.counter, slot, lane, k), ++claims) {
         const unsigned int qend = min(q0 + k, L.n_tiles);
-        for (unsigned int qpos = q0; qpos < qend; ++qpos) run_tile_gbm<PPL>(L, qpos, lane);
+        for (unsigned int qpos = q0; qpos < qend; ++qpos) run_tile_gbm<PPL, U>(L, qpos, lane);
     }
 }
 
+template <int U>
 __global__ __launch_bounds__(256) void tile_gbm_static_kernel(mp::tile_launch L) {
     const int lane = threadIdx.x & (MP_WAVE - 1);
     const unsigned int waves = gridDim.x * (blockDim.x / MP_WAVE);
     for (unsigned int qpos = blockIdx.x * (blockDim.x / MP_WAVE) + (threadIdx.x / MP_WAVE);
          qpos < L.n_tiles; qpos += waves)
-        run_tile_gbm<1>(L, qpos, lane);
+        run_tile_gbm<1, U>(L, qpos, lane);
 }
 
 __global__ void reset_kernel(unsigned int* counter, int n) {
@@ -2434,18 +2448,25 @@ hipError_t launch_setup(const mp_dreq& d, mp_dreq* dst, unsigned int* counter, i
     return hipGetLastError();
 }
 
+template <int U>
+static hipError_t tiles_gbm(const tile_launch& L, hipStream_t s) {
+    if (L.claim == 0)
+        hipLaunchKernelGGL((tile_gbm_static_kernel<U>), dim3(L.grid), dim3(L.block), 0, s, L);
+    else if (L.ppl == 2)
+        hipLaunchKernelGGL((tile_gbm_kernel<2, U>), dim3(L.grid), dim3(L.block), 0, s, L);
+    else if (L.ppl == 4)
+        hipLaunchKernelGGL((tile_gbm_kernel<4, U>), dim3(L.grid), dim3(L.block), 0, s, L);
+    else
+        hipLaunchKernelGGL((tile_gbm_kernel<1, U>), dim3(L.grid), dim3(L.block), 0, s, L);
+    return hipGetLastError();
+}
+
 template <class R, int SRC>
 static hipError_t tiles_ppl(const tile_launch& L, hipStream_t s) {
     if (sizeof(R) == 4 && SRC == 0 && L.gbm_f32 && !L.mlmc) {
-        if (L.claim == 0)
-            hipLaunchKernelGGL(tile_gbm_static_kernel, dim3(L.grid), dim3(L.block), 0, s, L);
-        else if (L.ppl == 2)
-            hipLaunchKernelGGL((tile_gbm_kernel<2>), dim3(L.grid), dim3(L.block), 0, s, L);
-        else if (L.ppl == 4)
-            hipLaunchKernelGGL((tile_gbm_kernel<4>), dim3(L.grid), dim3(L.block), 0, s, L);
-        else
-            hipLaunchKernelGGL((tile_gbm_kernel<1>), dim3(L.grid), dim3(L.block), 0, s, L);
-        return hipGetLastError();
+        if (L.unroll == 4) return tiles_gbm<4>(L, s);
+        if (L.unroll == 2) return tiles_gbm<2>(L, s);
+        return tiles_gbm<1>(L, s);
     }
     if (L.claim == 0) {
         if (L.mlmc)

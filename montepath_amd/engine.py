@@ -430,10 +430,17 @@ class Engine:
         ms = C.c_double()
         A.check(self.lib.mp_hip_autotune(self._ctx, C.byref(req), C.byref(best), C.byref(ms)))
         return {"block_size": best.block_size, "paths_per_lane": best.paths_per_lane,
-                "tile_paths": best.tile_paths}, ms.value
+                "tile_paths": best.tile_paths, "step_unroll": best.step_unroll}, ms.value
 
-    def set_tuning(self, req: A.Request, block_size: int, paths_per_lane: int) -> None:
-        t = A.Tuning(block_size, paths_per_lane, 0, 0)
+    def get_tuning(self, req: A.Request) -> dict:
+        t = A.Tuning()
+        A.check(self.lib.mp_hip_get_tuning(self._ctx, C.byref(req), C.byref(t)))
+        return {"block_size": t.block_size, "paths_per_lane": t.paths_per_lane,
+                "tile_paths": t.tile_paths, "step_unroll": t.step_unroll}
+
+    def set_tuning(self, req: A.Request, block_size: int, paths_per_lane: int,
+                   step_unroll: int = 1) -> None:
+        t = A.Tuning(block_size, paths_per_lane, 0, step_unroll)
         A.check(self.lib.mp_hip_set_tuning(self._ctx, C.byref(req), C.byref(t)))
 
     # ------------------------------------------------------------- serving

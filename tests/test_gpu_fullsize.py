@@ -71,6 +71,23 @@ def test_4096_contract_batch_full_size(engine):
         pb.free()
     assert [(a.price, a.stderr) for a in first] == [(b.price, b.stderr) for b in second]
 
+    # the same batch under another launch tuning (two paths per lane, four
+    # 4-step draw blocks per step-loop iteration): the same bits
+    default = engine.get_tuning(reqs[0])
+    engine.set_tuning(reqs[0], 128, 2, 4)
+    try:
+        pb = engine.prepare_batch(reqs, use_graph=False)
+        try:
+            pb.launch()
+            pb.sync()
+            retuned = pb.results()
+        finally:
+            pb.free()
+    finally:
+        engine.set_tuning(reqs[0], default["block_size"], default["paths_per_lane"],
+                          default["step_unroll"])
+    assert [(a.price, a.stderr) for a in first] == [(b.price, b.stderr) for b in retuned]
+
     # a fixed sample spanning the path and step ranges, both payoffs
     paths = np.array([r.n_paths for r in reqs])
     steps = np.array([r.n_steps for r in reqs])

@@ -44,9 +44,10 @@ def test_fp32_philox_tunings_and_batch_give_identical_bits(engine, kind, extra):
     alone = engine.price(req)
     for block in (64, 128, 256):
         for ppl in (1, 2, 4):
-            engine.set_tuning(req, block, ppl)
-            got = engine.price(req)
-            assert (got.price, got.stderr) == (alone.price, alone.stderr), (block, ppl)
+            for unroll in (1, 2, 4):
+                engine.set_tuning(req, block, ppl, unroll)
+                got = engine.price(req)
+                assert (got.price, got.stderr) == (alone.price, alone.stderr), (block, ppl, unroll)
     engine.set_tuning(req, 256, 1)
     others = [make_request("european_call", n_paths=9000 + i, n_steps=20 + i, seed=3, contract_id=2 + i)
               for i in range(5)]

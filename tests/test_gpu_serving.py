@@ -110,6 +110,8 @@ def test_autotune_keeps_results(engine):
     before = engine.price(req)
     best, ms = engine.autotune(req)
     assert best["block_size"] in (128, 256) and best["paths_per_lane"] in (1, 2, 4)
+    assert best["step_unroll"] in (1, 2, 4)
+    assert engine.get_tuning(req) == best
     assert ms > 0
     after = engine.price(req)
     assert after.price == before.price and after.stderr == before.stderr
