@@ -118,22 +118,25 @@ def bench_batch(eng, reps: int = 5) -> dict:
     batched = info["path_steps"] / dt
     res = pb.results()
     pb.free()
-    # plain FP64, one launch per request, on a sample of the same contracts
-    sample = list(range(0, len(reqs), 16))
-    ps, t_plain = 0, 0.0
-    for i in sample:
-        r = reqs[i]
+    # plain FP64 pricing, one launch per request, of every contract: the
+    # GPU time of each request's kernels from HIP events (launch and host
+    # gaps left out, which favours the plain path), and host wall time
+    ps, t_gpu, t_wall = 0, 0.0, 0.0
+    for i, r in enumerate(reqs):
         r64 = make_request("european_call" if r.workload == 1 else "arithmetic_asian_call",
                            n_paths=r.n_paths, n_steps=r.n_steps, k=r.k, sigma=r.sigma, t=r.t,
                            r=r.r, seed=7, contract_id=i, precision=64)
         t0 = time.perf_counter()
-        eng.price_plain_fp64(r64)
-        t_plain += time.perf_counter() - t0
+        pr = eng.price_plain_fp64(r64)
+        t_wall += time.perf_counter() - t0
+        t_gpu += pr.runtime_ms * 1e-3
         ps += r.n_paths * r.n_steps
-    plain = ps / t_plain
+    plain = ps / t_gpu
     return {"contracts": len(reqs), "path_steps": info["path_steps"], "tiles": info["n_tiles"],
             "ms_per_launch": dt * 1e3, "path_steps_per_s": batched,
-            "plain_fp64_path_steps_per_s": plain, "speedup_vs_plain_fp64": batched / plain,
+            "plain_fp64_path_steps_per_s": plain, "plain_fp64_gpu_s": t_gpu,
+            "plain_fp64_wall_s": t_wall, "plain_fp64_wall_path_steps_per_s": ps / t_wall,
+            "speedup_vs_plain_fp64": batched / plain,
             "mean_price": float(np.mean([x.price for x in res]))}
 
 
@@ -225,7 +228,25 @@ def bench_plain_c1(eng, threads: int, reps: int = 20) -> dict:
     for _ in range(3):
         cpu_price, _ = O.gbm_price(cfg)
     cpu_ms = (time.perf_counter() - t0) / 3 * 1e3
+    # the reference's CUDA kernel contract on the reference's own normal
+    # stream (seed 42, path-major, made on the host and copied in)
+    import math
+
+    z = O.rng_normals(42, n * m).reshape(n, m)
+    dt = 1.0 / m
+    sc = dict(log_s0=math.log(100.0), strike=100.0, drift_dt=(0.03 - 0.02) * dt,
+              vol_dt=0.2 * math.sqrt(dt), discount=math.exp(-0.03))
+    eng.european_call_stepwise_v1(z, **sc)
+    k_ms, w_ms = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        pay = eng.european_call_stepwise_v1(z, **sc)
+        w_ms.append((time.perf_counter() - t0) * 1e3)
+        k_ms.append(eng.last_timing()["main_kernel_ms"])
+    contract = {"kernel_ms": float(np.median(k_ms)), "with_copies_ms": float(np.median(w_ms)),
+                "normals_bytes": n * m * 8, "price": float(pay.mean())}
     return {"n_paths": n, "n_steps": m, "precision": "fp64", "gpu_ms": gpu_ms,
+            "cuda_contract_v1": contract,
             "gpu_path_steps_per_s": n * m / (gpu_ms * 1e-3), "plain_fp64_ms": plain_ms,
             "cpu_ms": cpu_ms, "cpu_threads": threads, "cpu_path_steps_per_s": n * m / (cpu_ms * 1e-3),
             "gpu_price": res.price, "gpu_stderr": res.stderr, "cpu_price": cpu_price,

@@ -347,6 +347,23 @@ void mp_hip_batch_free(mp_batch* batch);
  * GBM European/Asian requests only. */
 int mp_hip_price_plain_fp64(mp_ctx* ctx, const mp_request* req, mp_result* out);
 
+/* The reference's CUDA kernel contract for the stepwise European call,
+ * binding for binding (crates/mc-core/src/backend/cuda.rs:247-310,
+ * first_cuda_kernel_contract; kernel body
+ * crates/mc-core/src/backend/kernels/european_call_stepwise_v1.cu:1-28):
+ * normals (binding 0, FP64, n_paths * n_steps values, path-major: path p's
+ * step s at p * n_steps + s), payoffs (binding 1, FP64, n_paths values: the
+ * discounted payoff (S_T - K) * discount if S_T > K, else 0, with
+ * log S_T = log_s0 + sum over steps of (drift_dt + vol_dt * z) in step
+ * order, each product rounded), and the scalars n_paths, n_steps, log_s0,
+ * strike, drift_dt, vol_dt, discount; one logical thread per path, 256 per
+ * workgroup.  Both buffers are host memory: normals are copied to the
+ * device, payoffs back.  At most 2^31 normals.  mp_hip_last_timing reports
+ * the kernel's time (main_kernel_ms). */
+int mp_hip_european_call_stepwise_v1(mp_ctx* ctx, const double* normals, double* payoffs,
+                                     int32_t n_paths, int32_t n_steps, double log_s0, double strike,
+                                     double drift_dt, double vol_dt, double discount);
+
 /* Parity hook: run the request's path arithmetic on caller-supplied normal
  * draws instead of the engine's generator ("identical normal buffers").
  * normals are step-major: normals[(step * F + f) * n_paths + path] with F

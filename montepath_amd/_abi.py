@@ -217,6 +217,8 @@ SIGNATURES = {
     "mp_hip_batch_info": (C.c_int, [_P, _U64, _U64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mp_hip_batch_free": (None, [_P]),
     "mp_hip_price_plain_fp64": (C.c_int, [_P, C.POINTER(Request), C.POINTER(Result)]),
+    "mp_hip_european_call_stepwise_v1": (C.c_int, [_P, _D, _D, C.c_int32, C.c_int32, C.c_double,
+                                                   C.c_double, C.c_double, C.c_double, C.c_double]),
     "mp_hip_paths_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, _D, _D, _D, _D]),
     "mp_hip_price_from_normals": (C.c_int, [_P, C.POINTER(Request), _D, _D, _D, C.POINTER(Result)]),
     "mp_hip_greeks": (C.c_int, [_P, C.POINTER(Request), C.c_size_t, C.c_int32, C.POINTER(Result),

@@ -1874,6 +1874,35 @@ MP_EXPORT int mp_hip_price_plain_fp64(mp_ctx* c, const mp_request* req, mp_resul
     return MP_OK;
 }
 
+MP_EXPORT int mp_hip_european_call_stepwise_v1(mp_ctx* c, const double* normals, double* payoffs,
+                                               int32_t n_paths, int32_t n_steps, double log_s0,
+                                               double strike, double drift_dt, double vol_dt,
+                                               double discount) {
+    if (!c || !normals || !payoffs) return fail(MP_E_INVALID_CONFIG, "null argument");
+    if (n_paths < 1 || n_steps < 1) return fail(MP_E_INVALID_CONFIG, "n_paths and n_steps must be at least 1");
+    if ((uint64_t)n_paths * (uint64_t)n_steps > (1ull << 31))
+        return fail(MP_E_INVALID_CONFIG, "the contract's normals buffer holds at most 2^31 values");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t nn = (size_t)n_paths * (size_t)n_steps;
+    ENSURE(c->normals, nn * 8);
+    ENSURE(c->plain_pay, (size_t)n_paths * 8);
+    HIPCHK(hipMemcpyAsync(c->normals.p, normals, nn * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(mp::launch_european_call_stepwise_v1(c->normals.as<double>(), c->plain_pay.as<double>(), n_paths,
+                                                n_steps, log_s0, strike, drift_dt, vol_dt, discount,
+                                                c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(hipMemcpyAsync(payoffs, c->plain_pay.p, (size_t)n_paths * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->last.main_kernel_ms = ms;
+    c->last.finalize_ms = 0.0;
+    c->last.total_ms = ms;
+    c->last.main_launches = 1;
+    return MP_OK;
+}
+
 MP_EXPORT int mp_hip_paths_from_normals(mp_ctx* c, const mp_request* req, const double* normals,
                                         const double* uniforms, const double* jump_normals,
                                         double* terminal_out, double* payoff_out) {

@@ -199,6 +199,11 @@ hipError_t launch_lsm_final(const mp_lsm_params* p, const double* cash, const ui
                             double* out, int grid, int block, hipStream_t s);
 hipError_t launch_plain_fp64(const mp_dreq* req, double* payoffs, int grid, int block,
                              hipStream_t s);
+// The reference's stepwise European contract: normals [path][step] in, the
+// discounted payoff of each path out; one thread per path, 256 per block.
+hipError_t launch_european_call_stepwise_v1(const double* normals, double* payoffs, int n_paths,
+                                           int n_steps, double log_s0, double strike, double drift_dt,
+                                           double vol_dt, double discount, hipStream_t s);
 hipError_t launch_plain_reduce(const double* payoffs, uint64_t n, double* partial,
                                int n_blocks, hipStream_t s);
 hipError_t launch_draws(const mp_dreq* req, const mp_qmc_params& qp, const uint32_t* nib,

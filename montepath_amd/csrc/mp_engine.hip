@@ -2367,6 +2367,50 @@ __global__ __launch_bounds__(256) void plain_fp64_kernel(const mp_dreq* __restri
     payoffs[p] = q.discount * payoff_of(st, m, q, q.n_steps, q.s0, q.strikes[0]);
 }
 
+// ---------------------------------------------------------------------------
+// The reference's CUDA kernel contract for the stepwise European call
+// (crates/mc-core/src/backend/cuda.rs:247-310: buffer 0 normals, path-major
+// [path][step], FP64; buffer 1 payoffs [path], FP64; scalars n_paths,
+// n_steps, log_s0, strike, drift_dt, vol_dt, discount; one logical thread
+// per path, 256 per group; body kernels/european_call_stepwise_v1.cu:1-28).
+// One lane per path as the contract fixes, but the block's 256 rows are
+// read as tiles of 256 paths x 16 steps: consecutive lanes load consecutive
+// addresses of a row (128-byte runs), the tile goes through LDS (rows padded
+// by one word against bank conflicts), and each lane then walks its own row.
+// Per path: log S += drift_dt + vol_dt * z in step order, then
+// (S_T - K) * discount if S_T > K, else 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void european_call_stepwise_v1_kernel(
+    const double* __restrict__ normals, double* __restrict__ payoffs, int n_paths, int n_steps,
+    double log_s0, double strike, double drift_dt, double vol_dt, double discount) {
+    constexpr int C = 16;
+    __shared__ double tile[256][C + 1];
+    const int t = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.x * 256;
+    const int64_t path = row0 + t;
+    double log_s_t = log_s0;
+    for (int s0 = 0; s0 < n_steps; s0 += C) {
+        const int cols = min(C, n_steps - s0);
+        // all C loads of a thread in flight before the first LDS store
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            const int e = t + 256 * i;  // element (r, c) of the tile, c fastest
+            const int r = e / C, c = e % C;
+            const int64_t pr = row0 + r;
+            if (c < cols && pr < n_paths) tile[r][c] = normals[pr * n_steps + s0 + c];
+        }
+        __syncthreads();
+        {
+#pragma clang fp contract(off)  // each product rounded, as the reference's CPU path computes it
+            for (int c = 0; c < cols; ++c) log_s_t += drift_dt + vol_dt * tile[t][c];
+        }
+        __syncthreads();
+    }
+    if (path >= n_paths) return;
+    const double s_t = exp(log_s_t);
+    payoffs[path] = s_t > strike ? (s_t - strike) * discount : 0.0;
+}
+
 __global__ __launch_bounds__(256) void plain_reduce_kernel(const double* payoffs, uint64_t n,
                                                           double* partial) {
     double v[2] = {0.0, 0.0};
@@ -2835,6 +2879,15 @@ hipError_t launch_lsm_final(const mp_lsm_params* p, const double* cash, const ui
 hipError_t launch_plain_fp64(const mp_dreq* req, double* payoffs, int grid, int block,
                              hipStream_t s) {
     hipLaunchKernelGGL(plain_fp64_kernel, dim3(grid), dim3(block), 0, s, req, payoffs);
+    return hipGetLastError();
+}
+
+hipError_t launch_european_call_stepwise_v1(const double* normals, double* payoffs, int n_paths,
+                                           int n_steps, double log_s0, double strike, double drift_dt,
+                                           double vol_dt, double discount, hipStream_t s) {
+    const int grid = (n_paths + 255) / 256;
+    hipLaunchKernelGGL(european_call_stepwise_v1_kernel, dim3(grid), dim3(256), 0, s, normals, payoffs,
+                       n_paths, n_steps, log_s0, strike, drift_dt, vol_dt, discount);
     return hipGetLastError();
 }
 

@@ -297,6 +297,20 @@ class Engine:
         A.check(self.lib.mp_hip_price_plain_fp64(self._ctx, C.byref(req), C.byref(res)))
         return PriceResult.of(res)
 
+    def european_call_stepwise_v1(self, normals: np.ndarray, log_s0: float, strike: float,
+                                  drift_dt: float, vol_dt: float, discount: float) -> np.ndarray:
+        """The reference's CUDA kernel contract: normals [n_paths, n_steps]
+        (path-major, FP64) in, each path's discounted payoff out."""
+        z = np.ascontiguousarray(normals, dtype=np.float64)
+        if z.ndim != 2:
+            raise ValueError("normals must be a [n_paths, n_steps] array")
+        n_paths, n_steps = z.shape
+        out = np.empty(n_paths)
+        A.check(self.lib.mp_hip_european_call_stepwise_v1(
+            self._ctx, _pd(z), _pd(out), n_paths, n_steps, log_s0, strike, drift_dt, vol_dt,
+            discount))
+        return out
+
     def prepare_batch(self, reqs: list, use_graph: bool = True) -> "PreparedBatch":
         return PreparedBatch(self, reqs, use_graph)
 

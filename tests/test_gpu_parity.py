@@ -253,6 +253,54 @@ def test_plain_fp64_path_equals_engine_fp64(engine):
     assert abs(a.stderr - b.stderr) <= 1e-9 * a.stderr
 
 
+def _contract_scalars(cfg):
+    dt = cfg.t / cfg.n_steps
+    return dict(log_s0=math.log(cfg.s0), strike=cfg.k,
+                drift_dt=(cfg.r - 0.5 * cfg.sigma * cfg.sigma) * dt,
+                vol_dt=cfg.sigma * math.sqrt(dt), discount=math.exp(-cfg.r * cfg.t))
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (4099, 37), (1000, 16), (257, 200)])
+def test_cuda_contract_kernel_matches_oracle_path_by_path(engine, oracle, n, m):
+    # the reference's stepwise European contract on the reference's own
+    # normal stream, path-major: log S_T is formed in the reference's order
+    # with every product rounded, so a payoff differs from the oracle's by
+    # the last bits of exp only
+    cfg = oracle.Gbm(n_paths=n, n_steps=m, seed=42)
+    z = oracle.rng_normals(42, n * m).reshape(n, m)
+    _, ref = oracle.gbm_paths_from_normals(cfg, z)
+    got = engine.european_call_stepwise_v1(z, **_contract_scalars(cfg))
+    assert engine.last_timing()["main_launches"] == 1
+    err = np.abs(got - ref)
+    assert float(err.max()) <= 4 * np.finfo(float).eps * 400.0, float(err.max())
+    assert np.array_equal(got > 0, ref > 0)
+
+
+def test_cuda_contract_kernel_prices_the_cpu_runnable_case(engine, oracle):
+    # 100,000 paths x 64 steps, seed 42, one thread: the contract's payoffs
+    # summarised as the reference does equal the oracle's price; the price is
+    # within 3 standard errors of Black-Scholes
+    n, m = 100_000, 64
+    cfg = oracle.Gbm(n_paths=n, n_steps=m, seed=42, n_threads=1)
+    z = oracle.rng_normals(42, n * m).reshape(n, m)
+    pay = engine.european_call_stepwise_v1(z, **_contract_scalars(cfg))
+    price, stderr = oracle.gbm_price(cfg)
+    mean = pay.mean()
+    se = math.sqrt(max((pay * pay).mean() - mean * mean, 0.0)) / math.sqrt(n)
+    assert abs(mean - price) <= 1e-12 * price, (mean, price)
+    assert abs(se - stderr) <= 1e-9 * stderr
+    assert abs(mean - 9.413403383853016) <= 3 * se
+
+
+def test_cuda_contract_kernel_rejects_invalid_shapes(engine):
+    from montepath_amd import _abi as A
+
+    with pytest.raises(A.MontepathHipError):
+        engine.european_call_stepwise_v1(np.zeros((0, 4)), 0.0, 1.0, 0.0, 0.1, 1.0)
+    with pytest.raises(A.MontepathHipError):
+        engine.european_call_stepwise_v1(np.zeros((4, 0)), 0.0, 1.0, 0.0, 0.1, 1.0)
+
+
 def test_heston_full_truncation_matches_characteristic_function(engine):
     from oracle import closed_forms as cf
 
