@@ -1274,7 +1274,57 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                         for (int f = 0; f < F; ++f) cur_d[f] = a[f];
                     }
                 // the 16-step subtree: midpoints 8, 4, 2, 1, 3, 6, 5, 7, 12, ...
-                // (halves and quarters as loops, one copy of the quarter code)
+                // When its 15 splits stay inside the current 256-dimension
+                // offset (all but about one subtree in eight), they run as
+                // straight-line code: split K (preorder) reads the dimensions
+                // at walk position + K F + f by immediate offsets, and the
+                // walk moves once, after the subtree.  Otherwise halves and
+                // quarters run as loops (one copy of the quarter code) that
+                // check for the offset change before every split.  Both forms
+                // consume the same dimensions with the same arithmetic.
+                if (dims + 15 * F <= 256) {
+                    auto fsplit = [&](int K, const R (&d)[F], R (&l)[F], R (&r)[F]) {
+                        const mp_bbop& op = plan[k + K];
+                        R rw, sd;
+                        if constexpr (sizeof(R) == 4) {
+                            rw = op.rwf;
+                            sd = op.sdf;
+                        } else {
+                            rw = op.rw;
+                            sd = op.sd;
+                        }
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            const R z = cur.normal_of(cur.xrev_rel(K * F + f), (uint32_t)(dims + K * F + f));
+                            l[f] = rw * d[f] + sd * z;
+                            r[f] = d[f] - l[f];
+                        }
+                    };
+                    auto fpair2 = [&](int K, const R (&d)[F]) {
+                        R a[F], b[F];
+                        fsplit(K, d, a, b);
+                        leaf(a);
+                        leaf(b);
+                    };
+                    auto fquad4 = [&](int K, const R (&d)[F]) {
+                        R a[F], b[F];
+                        fsplit(K, d, a, b);
+                        fpair2(K + 1, a);
+                        fpair2(K + 2, b);
+                    };
+                    R a8[F], b8[F], a4[F], b4[F];
+                    fsplit(0, cur_d, a8, b8);
+                    fsplit(1, a8, a4, b4);
+                    fquad4(2, a4);
+                    fquad4(5, b4);
+                    fsplit(8, b8, a4, b4);
+                    fquad4(9, a4);
+                    fquad4(12, b4);
+                    cur.advance(15 * F);
+                    dims += 15 * F;
+                    k += 15;
+                    continue;
+                }
                 R a8[F], b8[F];
                 split(cur_d, a8, b8);
 #pragma unroll 1
