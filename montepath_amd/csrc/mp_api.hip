@@ -1280,8 +1280,12 @@ int run_lsm_once(mp_ctx* c, const mp_request& r, prepared& P, int precision, con
     // matched launches use the one-launch block size (same per-thread paths
     // and block sums); plain ones 256-thread blocks
     const int sblock = matched ? fused_bs : block;
+    // forward paths with a grid that fills the device (grid-stride, so any
+    // grid gives the same paths)
+    const int fwd_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + block - 1) / block,
+                                                                      (uint64_t)c->n_cu * 16));
     HIPCHK(mp::launch_lsm_paths(dp, normals_host ? c->normals.as<double>() : nullptr, c->lsm_paths.p,
-                                c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), precision, grid,
+                                c->lsm_cash.as<double>(), c->lsm_ex.as<uint16_t>(), precision, fwd_grid,
                                 block, c->stream));
     // each pass also applies the exercise rule of the date regressed before it
     int prev = 0;

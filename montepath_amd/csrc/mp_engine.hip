@@ -1698,6 +1698,22 @@ __device__ __forceinline__ void laguerre(double x, int degree, bool weighted, do
     b[3] = degree >= 3 ? (1.0 - 3.0 * x + 1.5 * x2 - x3_6) * w : 0.0;
 }
 
+// The continuation value of the unweighted basis, beta . L(x), as the cubic
+// c0 + c1 x + c2 x^2 + c3 x^3 whose coefficients are formed once per date
+// (L0 = 1, L1 = 1 - x, L2 = 1 - 2x + x^2/2, L3 = 1 - 3x + 3x^2/2 - x^3/6;
+// beta_i = 0 beyond the regression's degree): three FMAs per path instead
+// of the basis and the dot product.  The weighted basis keeps laguerre().
+__device__ __forceinline__ void lsm_cont_poly(const double (&beta)[4], double (&c)[4]) {
+    c[0] = ((beta[0] + beta[1]) + beta[2]) + beta[3];
+    c[1] = (-beta[1] - 2.0 * beta[2]) - 3.0 * beta[3];
+    c[2] = 0.5 * beta[2] + 1.5 * beta[3];
+    c[3] = beta[3] * -0.16666666666666666;
+}
+
+__device__ __forceinline__ double lsm_cont_eval(const double (&c)[4], double x) {
+    return __builtin_fma(__builtin_fma(__builtin_fma(c[3], x, c[2]), x, c[1]), x, c[0]);
+}
+
 // Gauss-Jordan with partial pivoting, failing when the best pivot <= 1e-12
 // (solve_lsm_normal_equations cpu.rs:7684-7736), with the pivot row scaled
 // by the pivot's reciprocal instead of divided element by element (one
@@ -1987,6 +2003,8 @@ __global__ __launch_bounds__(1024) void lsm_step_kernel(const mp_lsm_params* __r
 #pragma unroll
         for (int i = 0; i < 4; ++i) beta[i] = coeff_all[step_update * 4 + i];
     }
+    double cpoly[4];
+    lsm_cont_poly(beta, cpoly);
     const double dpow_u = step_update >= 1 ? disc_pow[step_update] : 0.0;
     const double grow_g = step_gram >= 1 ? growth[step_gram] : 0.0;
     // 10 unique Gram entries + 4 RHS + count
@@ -2012,12 +2030,17 @@ __global__ __launch_bounds__(1024) void lsm_step_kernel(const mp_lsm_params* __r
             const uint64_t p = base + u * T;
             const double imm_u = k - su[u] > 0.0 ? k - su[u] : 0.0;
             if (imm_u > 0.0) {  // false when not updating or p >= n (su = k)
-                double bb[4];
-                laguerre(su[u] * inv_k, deg, wtd, bb);
-                // beta[i] = 0 for i >= nc, so the full sum equals the sum over nc
-                double cont = 0.0;
+                double cont;
+                if (wtd) {
+                    double bb[4];
+                    laguerre(su[u] * inv_k, deg, wtd, bb);
+                    // beta[i] = 0 for i >= nc, so the full sum equals the sum over nc
+                    cont = 0.0;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) cont += beta[i] * bb[i];
+                    for (int i = 0; i < 4; ++i) cont += beta[i] * bb[i];
+                } else {
+                    cont = lsm_cont_eval(cpoly, su[u] * inv_k);
+                }
                 if (imm_u > (cont > 0.0 ? cont : 0.0)) {
                     c[u] = imm_u * dpow_u;
                     cash[p] = c[u];
@@ -2233,7 +2256,7 @@ __global__ __launch_bounds__(BS) void lsm_fused_kernel(const mp_lsm_params* __re
     }
 
     __shared__ double tot[15], gsum[15];
-    __shared__ double beta_s[4];
+    __shared__ double beta_s[4], cpoly_s[4];
     __shared__ int valid_s;
     // Lanes past the last path take no part in the regression.  (A padded
     // price of R(k) is not enough: in FP32, k - float(k) can be positive.)
@@ -2277,9 +2300,18 @@ __global__ __launch_bounds__(BS) void lsm_fused_kernel(const mp_lsm_params* __re
         if (n_reg > 1) load_date(1, sn);
     }
     const int NG = (G + 31) >> 5;
+    // a date's index and discount factors are loaded one date ahead, so the
+    // dependent global loads stay off the per-date critical path
+    int date_n = n_reg > 0 ? reg_dates[0] : 0;
+    double grow_n = n_reg > 0 ? growth[date_n] : 0.0, dpow_n = n_reg > 0 ? disc_pow[date_n] : 0.0;
     for (int i = 0; i < n_reg; ++i) {
-        const int date = reg_dates[i];
-        const double grow = growth[date];
+        const int date = date_n;
+        const double grow = grow_n, dpow = dpow_n;
+        if (i + 1 < n_reg) {
+            date_n = reg_dates[i + 1];
+            grow_n = growth[date_n];
+            dpow_n = disc_pow[date_n];
+        }
         double acc[15];
 #pragma unroll
         for (int q = 0; q < 10; ++q) acc[q] = gacc[q];
@@ -2317,7 +2349,12 @@ __global__ __launch_bounds__(BS) void lsm_fused_kernel(const mp_lsm_params* __re
         if (threadIdx.x == 0) {
             double c[4];
             const int ok = lsm_coefficients(tot, nc, c);
-            for (int a = 0; a < 4; ++a) beta_s[a] = c[a];
+            double cp[4];
+            lsm_cont_poly(c, cp);
+            for (int a = 0; a < 4; ++a) {
+                beta_s[a] = c[a];
+                cpoly_s[a] = cp[a];
+            }
             valid_s = ok;
             if (blockIdx.x == 0) {
                 // host outputs and the regression count (read after the final ticket)
@@ -2327,20 +2364,27 @@ __global__ __launch_bounds__(BS) void lsm_fused_kernel(const mp_lsm_params* __re
         }
         __syncthreads();
         if (valid_s) {
-            double beta[4];
+            double beta[4], cp[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) beta[a] = beta_s[a];
-            const double dpow = disc_pow[date];
+            for (int a = 0; a < 4; ++a) {
+                beta[a] = beta_s[a];
+                cp[a] = cpoly_s[a];
+            }
 #pragma unroll
             for (int j = 0; j < J; ++j) {
                 const double s = (double)sd[j];
                 const double imm = k - s > 0.0 ? k - s : 0.0;
                 if (live[j] && imm > 0.0) {
-                    double bx[4];
-                    laguerre(s * inv_k, deg, wtd, bx);
-                    double cont = 0.0;
+                    double cont;
+                    if (wtd) {
+                        double bx[4];
+                        laguerre(s * inv_k, deg, wtd, bx);
+                        cont = 0.0;
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) cont += beta[a] * bx[a];
+                        for (int a = 0; a < 4; ++a) cont += beta[a] * bx[a];
+                    } else {
+                        cont = lsm_cont_eval(cp, s * inv_k);
+                    }
                     if (imm > (cont > 0.0 ? cont : 0.0)) {
                         cash[j] = imm * dpow;
                         exd[j] = date;
