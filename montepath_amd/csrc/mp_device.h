@@ -1,5 +1,6 @@
 // mp_device.h -- device helpers of the montepath HIP engine (gfx950, wave64).
 #pragma once
+#include "mp_inv_normal_f32_coeffs.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -293,15 +294,22 @@ __device__ __forceinline__ R mp_acklam(R p) {
 }
 
 // ---------------------------------------------------------------------------
-// FP32 inverse normal for the FP32 paths: Phi^-1(p) = sqrt(2) erfinv(2p - 1)
-// with Giles' single-precision erfinv ("Approximating the erfinv function",
-// GPU Computing Gems, 2011; max relative error ~1.2e-7).  Acklam's rational
-// form, which the FP64 paths keep because it is the reference's, loses up to
-// 1e-4 relative in FP32 to cancellation between its alternating coefficients,
-// and its tail branch (4.85% of draws) diverges in almost every wave; this
-// tail (w >= 5, 0.34% of draws) is rare.  p lies on the 2^-23 grid in
-// [0,1) and arrives as the bits of the float 1 + p; 4p(1-p) is formed
-// exactly up to one rounding.  p == 0 returns the FP64 reference value at
+// FP32 inverse normal for the FP32 paths: Phi^-1(p) = x R(w), x = 2p - 1,
+// w = -ln(1 - x^2).  Central branch (w < 7, all but 0.046% of the
+// uniforms): R is a degree-9 polynomial in w - 3.5 fitted for this build
+// (gen_inv_normal_f32.py: reweighted Chebyshev least squares on the
+// relative error of sqrt(2) erfinv(x) / x; 3.8e-7 relative evaluated in
+// FP32 over the whole uniform grid).  Tail (w >= 7): the w >= 5 branch of
+// Giles' single-precision erfinv ("Approximating the erfinv function", GPU
+// Computing Gems, 2011), coefficients times sqrt(2).  The wide central
+// branch keeps the divergent tail out of most waves: with w < 5 as in
+// Giles' own split, some lane of a 64-lane wave took the tail on about one
+// draw in five.  Acklam's rational form, which the FP64 paths keep because
+// it is the reference's, loses up to 1e-4 relative in FP32 to cancellation
+// between its alternating coefficients, and its tail branch (4.85% of
+// draws) diverges in almost every wave.  p lies on the 2^-23 grid in [0,1)
+// and arrives as the bits of the float 1 + p; 4p(1-p) is formed exactly up
+// to one rounding.  p == 0 returns the FP64 reference value at
 // f64::MIN_POSITIVE; it lies in the tail (log2 0 = -inf), so the test for
 // it costs nothing on the central branch.
 // ---------------------------------------------------------------------------
@@ -310,21 +318,21 @@ __device__ __forceinline__ float mp_inv_normal_f32(uint32_t fbits) {
     const float f = __uint_as_float(fbits);
     const float x = __builtin_fmaf(f, 2.0f, -3.0f);  // 2p - 1, exact
     // 1 - x^2 = 4p(1-p) exactly; the FMA rounds it once.  l = log2 of it,
-    // w = -ln(4p(1-p)) = -ln2 * l.  The coefficients below are Giles'
-    // multiplied by sqrt(2), so that r * x is the normal itself.
+    // w = -ln(4p(1-p)) = -ln2 * l.
     const float l = __builtin_amdgcn_logf(__builtin_fmaf(-x, x, 1.0f));
     float r;
-    if (l > -7.2134752f) {  // w < 5
-        const float w = __builtin_fmaf(-0.69314718f, l, -2.5f);
-        r = 3.97426021e-08f;
-        r = 4.85462635e-07f + r * w;
-        r = -4.9828227e-06f + r * w;
-        r = -6.21052823e-06f + r * w;
-        r = 0.00030912002f + r * w;
-        r = -0.00177303492f + r * w;
-        r = -0.00590813393f + r * w;
-        r = 0.348802656f + r * w;
-        r = 2.12331367f + r * w;
+    if (l > MP_INVN_L_CENTRAL) {  // w < 7
+        const float t = __builtin_fmaf(-0.69314718f, l, -MP_INVN_CENTER);
+        r = MP_INVN_C9;
+        r = MP_INVN_C8 + r * t;
+        r = MP_INVN_C7 + r * t;
+        r = MP_INVN_C6 + r * t;
+        r = MP_INVN_C5 + r * t;
+        r = MP_INVN_C4 + r * t;
+        r = MP_INVN_C3 + r * t;
+        r = MP_INVN_C2 + r * t;
+        r = MP_INVN_C1 + r * t;
+        r = MP_INVN_C0 + r * t;
     } else {
         const float w = __builtin_amdgcn_sqrtf(-0.69314718f * l) - 3.0f;
         r = -0.000283145724f;
