@@ -296,10 +296,10 @@ __device__ __forceinline__ R mp_acklam(R p) {
 // ---------------------------------------------------------------------------
 // FP32 inverse normal for the FP32 paths: Phi^-1(p) = x R(w), x = 2p - 1,
 // w = -ln(1 - x^2).  Central branch (w < 7, all but 0.046% of the
-// uniforms): R is a degree-9 polynomial in w - 3.5 fitted for this build
-// (gen_inv_normal_f32.py: reweighted Chebyshev least squares on the
-// relative error of sqrt(2) erfinv(x) / x; 3.8e-7 relative evaluated in
-// FP32 over the whole uniform grid).  Tail (w >= 7): the w >= 5 branch of
+// uniforms): R is a degree-9 polynomial in l = log2(1 - x^2), the log's
+// own output, fitted for this build (gen_inv_normal_f32.py: reweighted
+// Chebyshev least squares on the relative error of sqrt(2) erfinv(x) / x;
+// 3.3e-7 relative evaluated in FP32 over the whole uniform grid).  Tail (w >= 7): the w >= 5 branch of
 // Giles' single-precision erfinv ("Approximating the erfinv function", GPU
 // Computing Gems, 2011), coefficients times sqrt(2).  The wide central
 // branch keeps the divergent tail out of most waves: with w < 5 as in
@@ -318,21 +318,20 @@ __device__ __forceinline__ float mp_inv_normal_f32(uint32_t fbits) {
     const float f = __uint_as_float(fbits);
     const float x = __builtin_fmaf(f, 2.0f, -3.0f);  // 2p - 1, exact
     // 1 - x^2 = 4p(1-p) exactly; the FMA rounds it once.  l = log2 of it,
-    // w = -ln(4p(1-p)) = -ln2 * l.
+    // w = -ln(4p(1-p)) = -ln2 * l (the tail's variable).
     const float l = __builtin_amdgcn_logf(__builtin_fmaf(-x, x, 1.0f));
     float r;
-    if (l > MP_INVN_L_CENTRAL) {  // w < 7
-        const float t = __builtin_fmaf(-0.69314718f, l, -MP_INVN_CENTER);
+    if (l > MP_INVN_L_CENTRAL) {  // w < 7: a polynomial in l itself
         r = MP_INVN_C9;
-        r = MP_INVN_C8 + r * t;
-        r = MP_INVN_C7 + r * t;
-        r = MP_INVN_C6 + r * t;
-        r = MP_INVN_C5 + r * t;
-        r = MP_INVN_C4 + r * t;
-        r = MP_INVN_C3 + r * t;
-        r = MP_INVN_C2 + r * t;
-        r = MP_INVN_C1 + r * t;
-        r = MP_INVN_C0 + r * t;
+        r = MP_INVN_C8 + r * l;
+        r = MP_INVN_C7 + r * l;
+        r = MP_INVN_C6 + r * l;
+        r = MP_INVN_C5 + r * l;
+        r = MP_INVN_C4 + r * l;
+        r = MP_INVN_C3 + r * l;
+        r = MP_INVN_C2 + r * l;
+        r = MP_INVN_C1 + r * l;
+        r = MP_INVN_C0 + r * l;
     } else {
         const float w = __builtin_amdgcn_sqrtf(-0.69314718f * l) - 3.0f;
         r = -0.000283145724f;
