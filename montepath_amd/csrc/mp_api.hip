@@ -320,9 +320,10 @@ int bridge_plan(uint64_t n_steps, double maturity, std::vector<mp_bbop>& ops, in
     struct item {
         uint64_t l, r;
         int slot;
+        int depth;
     };
     std::vector<item> st;
-    st.push_back({0, n_steps, 0});
+    st.push_back({0, n_steps, 0, 0});
     while (!st.empty()) {
         item it = st.back();
         st.pop_back();
@@ -335,14 +336,16 @@ int bridge_plan(uint64_t n_steps, double maturity, std::vector<mp_bbop>& ops, in
         op.sd = std::sqrt((tm - tl) * (tr - tm) / denom);
         op.rwf = (float)op.rw;
         op.sdf = (float)op.sd;
+        op.sdf_e = (float)std::ldexp(op.sd, it.depth + 1);
+        op.pad_ = 0.f;
         op.slot = it.slot;
         const bool left_leaf = mid <= it.l + 1;
         const bool right_leaf = it.r <= mid + 1;
         op.flags = (left_leaf ? 1 : 0) | (right_leaf ? 2 : 0);
         ops.push_back(op);
         n_slots = std::max(n_slots, it.slot + (left_leaf ? 1 : 2));
-        st.push_back({mid, it.r, it.slot});
-        st.push_back({it.l, mid, it.slot + 1});
+        st.push_back({mid, it.r, it.slot, it.depth + 1});
+        st.push_back({it.l, mid, it.slot + 1, it.depth + 1});
     }
     return MP_OK;
 }

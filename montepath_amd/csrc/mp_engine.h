@@ -82,6 +82,8 @@ struct mp_dreq {
 // in stack[slot]).  rwf/sdf are rw/sd rounded to FP32 for the FP32 kernel.
 struct mp_bbop {
     float rwf, sdf;
+    float sdf_e;    // FP32 register bridge: 2^(depth + 1) sd, depth of the split interval (root 0)
+    float pad_;
     double rw, sd;
     int32_t slot;
     int32_t flags;

@@ -143,6 +143,21 @@ __device__ __forceinline__ void heston_step_dw(path_state<R>& st, const model_pa
     observe(st, m);
 }
 
+// heston_step_dw for European payoffs: x_T is linear in the per-step rate
+// term, so the caller adds n rate_dt once after the last step and each step
+// is two FMAs on x; the variance update is heston_step_dw's.
+template <class R>
+__device__ __forceinline__ void heston_step_dw_eu(path_state<R>& st, const model_params<R>& m, R dw_var,
+                                                  R dw_ind) {
+    const R vp = mp_relu(st.v);
+    const R sv = mp_sqrt(vp);
+    const R dw_price = m.rho * dw_var + m.rho_perp * dw_ind;
+    st.x = sv * dw_price + (st.x - m.half_dt * vp);
+    R v = st.v + m.kappa_dt * (m.theta - vp) + m.xi * sv * dw_var;
+    if (m.scheme == 0) v = mp_relu(v);
+    st.v = v;
+}
+
 // Poisson count by inversion of u against the request's cumulative
 // probabilities (reference sample_poisson cpu.rs:932-953; the same recursion
 // in FP64, so both precisions count identical jumps).
@@ -1162,17 +1177,27 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
         constexpr bool kInc = sizeof(R) == 4;
         const R to_step = kInc ? R(1) : inv_sqrt_dt;  // bridge increment -> step input
         const R z_to_step = kInc ? m.sqrt_dt : R(1);  // normal -> step input
+        // the step's parameters (the FP32 register bridge scales the ones
+        // that multiply the increments)
+        model_params<R> ms = m;
+        // FP32 European register-bridge kernels leave the per-step drift /
+        // rate term out of the steps and add it once after the last
+        constexpr bool kDriftOnce = kInc && TREE && PAY == MP_PAY_EUROPEAN;
         auto step1 = [&](path_state<R>& x, R i0, R i1) {
             if constexpr (F == 1) {
-                if constexpr (kInc)
-                    gbm_step_dw(x, m, i0);
+                if constexpr (kDriftOnce)
+                    x.x = ms.sigma * i0 + x.x;
+                else if constexpr (kInc)
+                    gbm_step_dw(x, ms, i0);
                 else
-                    gbm_step(x, m, i0);
+                    gbm_step(x, ms, i0);
             } else {
-                if constexpr (kInc)
-                    heston_step_dw(x, m, i0, i1);
+                if constexpr (kDriftOnce)
+                    heston_step_dw_eu(x, ms, i0, i1);
+                else if constexpr (kInc)
+                    heston_step_dw(x, ms, i0, i1);
                 else
-                    heston_step(x, m, i0, i1);
+                    heston_step(x, ms, i0, i1);
             }
         };
         auto emit = [&](R i0, R i1) {
@@ -1204,10 +1229,23 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
             // J - ctz(s) (level 0 for s = 0) down to level J, keeping each
             // split's right half in the register pending[level]; a subtree's
             // 15 splits are static code.  Splits are consumed in the plan's
-            // preorder (op k uses dimensions (k+1)F + f, the plan's weights),
-            // with the stack kernel's arithmetic, so results are identical.
+            // preorder (op k uses dimensions (k+1)F + f, the plan's weights).
+            // FP64 uses the stack kernel's arithmetic, so results are
+            // identical.  FP32 carries each interval's increment D scaled by
+            // 2^depth (E = 2^k D; exact): the midpoint split D -> (D/2 + sd z,
+            // D/2 - sd z) becomes E -> E +- se z, two FMAs, with
+            // se = 2^(k+1) sd from the plan, and the leaves' 2^-(4+J) is
+            // folded into the step's parameters.  Each half is rounded once
+            // (the stack kernel forms the right half as D - left).
             const int J = qp.tree_levels;
             int k = 0, dims = 0, o = 0;
+            if constexpr (sizeof(R) == 4) {
+                const R leaf_scale = R(1) / R(16 << J);  // a power of two
+                ms.sigma *= leaf_scale;
+                ms.rho *= leaf_scale;
+                ms.rho_perp *= leaf_scale;
+                ms.xi *= leaf_scale;
+            }
             auto split = [&](const R (&d)[F], R (&l)[F], R (&r)[F]) {
                 if (dims == 256) {  // next 256-dimension offset
                     ++o;
@@ -1215,19 +1253,16 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                     dims = 0;
                 }
                 const mp_bbop& op = plan[k];
-                R rw, sd;
-                if constexpr (sizeof(R) == 4) {
-                    rw = op.rwf;
-                    sd = op.sdf;
-                } else {
-                    rw = op.rw;
-                    sd = op.sd;
-                }
 #pragma unroll
                 for (int f = 0; f < F; ++f) {
                     const R z = cur.normal_rel((uint32_t)(dims + f), f);
-                    l[f] = rw * d[f] + sd * z;
-                    r[f] = d[f] - l[f];
+                    if constexpr (sizeof(R) == 4) {
+                        l[f] = __builtin_fmaf(op.sdf_e, z, d[f]);
+                        r[f] = __builtin_fmaf(-op.sdf_e, z, d[f]);
+                    } else {
+                        l[f] = op.rw * d[f] + op.sd * z;
+                        r[f] = d[f] - l[f];
+                    }
                 }
                 cur.advance(F);
                 dims += F;
@@ -1285,19 +1320,16 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                 if (dims + 15 * F <= 256) {
                     auto fsplit = [&](int K, const R (&d)[F], R (&l)[F], R (&r)[F]) {
                         const mp_bbop& op = plan[k + K];
-                        R rw, sd;
-                        if constexpr (sizeof(R) == 4) {
-                            rw = op.rwf;
-                            sd = op.sdf;
-                        } else {
-                            rw = op.rw;
-                            sd = op.sd;
-                        }
 #pragma unroll
                         for (int f = 0; f < F; ++f) {
                             const R z = cur.normal_of(cur.xrev_rel(K * F + f), (uint32_t)(dims + K * F + f));
-                            l[f] = rw * d[f] + sd * z;
-                            r[f] = d[f] - l[f];
+                            if constexpr (sizeof(R) == 4) {
+                                l[f] = __builtin_fmaf(op.sdf_e, z, d[f]);
+                                r[f] = __builtin_fmaf(-op.sdf_e, z, d[f]);
+                            } else {
+                                l[f] = op.rw * d[f] + op.sd * z;
+                                r[f] = d[f] - l[f];
+                            }
                         }
                     };
                     auto fpair2 = [&](int K, const R (&d)[F]) {
@@ -1407,6 +1439,7 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                 }
             }
         }
+        if constexpr (kDriftOnce) st.x += R(n_steps) * (F == 1 ? m.drift_dt : m.rate_dt);
         double sum[MP_NS_MAX], sq[MP_NS_MAX];
 #pragma unroll
         for (int j = 0; j < MP_NS_MAX; ++j) sum[j] = sq[j] = 0.0;

@@ -1,10 +1,16 @@
 """Sobol + Brownian-bridge pricing (FP32 and FP64) with the bridge in
 registers (step counts 16 * 2^J) against the LDS-stack kernel.
 
-Both consume the bridge plan in the same preorder with the same arithmetic,
-so the bar is bit-identical per-path terminal values and prices.  The stack
-kernel is pinned to the oracle by the parity tests (FP64 within 1e-12, FP32
-within 1e-5).
+Both consume the bridge plan in the same preorder.  FP64 uses the same
+arithmetic, so the bar is bit-identical per-path terminal values and prices.
+The FP32 register bridge carries increments scaled by 2^depth and rounds
+each half of a split once (the stack kernel forms the right half as the
+interval minus the left), and its European kernels add the per-step drift
+once per path; each FP32 form is within about 1.5e-6 of FP64 on the same
+draws (the parity tests hold both to 1e-5), so the bar between them is
+5e-6 relative per path (measured 2.2e-6) and 1e-4 standard errors on
+prices (measured 4e-5).  The stack kernel is pinned to the oracle by the
+parity tests (FP64 within 1e-12, FP32 within 1e-5).
 """
 import os
 
@@ -52,7 +58,17 @@ def test_register_bridge_equals_stack_bridge(engine, stack_engine, kind, m, extr
                        sampling="scrambled_sobol_brownian_bridge", **extra)
     a, term_a, pay_a = engine.price_with_paths(req)
     b, term_b, pay_b = stack_engine.price_with_paths(req)
-    assert np.array_equal(term_a, term_b)
-    assert np.array_equal(pay_a, pay_b)
-    assert a.strike_prices == b.strike_prices
-    assert a.strike_stderrs == b.strike_stderrs
+    if precision == 64:
+        assert np.array_equal(term_a, term_b)
+        assert np.array_equal(pay_a, pay_b)
+        assert a.strike_prices == b.strike_prices
+        assert a.strike_stderrs == b.strike_stderrs
+        return
+    used = term_b > 0  # antithetic requests leave the entries past their pairs at 0
+    assert np.array_equal(term_a > 0, used)
+    rel = np.abs(term_a[used] - term_b[used]) / term_b[used]
+    assert float(rel.max()) <= 5e-6, float(rel.max())
+    assert float(np.max(np.abs(pay_a - pay_b) / (np.abs(pay_b) + 100.0))) <= 5e-6
+    for pa, pb, sa, sb in zip(a.strike_prices, b.strike_prices, a.strike_stderrs, b.strike_stderrs):
+        assert abs(pa - pb) <= 1e-4 * sb + 1e-12, (pa, pb, sb)
+        assert abs(sa - sb) <= 1e-4 * sb + 1e-12, (sa, sb)
