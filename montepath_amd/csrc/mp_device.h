@@ -401,6 +401,23 @@ __device__ __forceinline__ uint32_t mp_mul24_su(uint32_t a_uniform, uint32_t b) 
     return r;
 }
 
+__device__ __forceinline__ uint32_t mp_mul24_vv(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// mp_lk_low24 with the hashed seed hs and its multiplier hm = (hs >> 16) | 1
+// in vector registers (read from a table).
+__device__ __forceinline__ uint32_t mp_lk_low24_vv(uint32_t n, uint32_t hs, uint32_t hm) {
+    n ^= mp_mul24_su(0x3d20adeau, n);
+    n += hs;
+    n = mp_mul24_vv(hm, n);
+    n ^= mp_mul24_su(0x05526c56u, n);
+    n ^= mp_mul24_su(0x53a22864u, n);
+    return n;
+}
+
 // mp_lk_low24 for a wave-uniform hashed seed.
 __device__ __forceinline__ uint32_t mp_lk_low24_su(uint32_t n, uint32_t hs_uniform) {
     n ^= mp_mul24_su(0x3d20adeau, n);

@@ -293,7 +293,7 @@ __device__ __noinline__ double structured_normal(const mp_sampler_dim* tab, cons
     return mp_acklam<double>(u);
 }
 
-template <class R, bool U>
+template <class R, bool U, bool HT = false>
 struct sobol_cursor;
 
 // Normal of a structured source at (point, dimension): randomized Halton and
@@ -1022,11 +1022,18 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* dst) {
 // (paths of one tile never straddle a 2^16 block), so the per-dimension
 // hash runs on the scalar unit and the permutation uses 24-bit multiplies
 // by a scalar operand.
-template <class R, bool U>
+// HT (the register-bridge kernels): the hashed per-dimension seeds of the
+// tile's 2^16-point block come from a table in LDS (hst[g] = {hash, its
+// multiplier (hash >> 16) | 1} for global dimension g = 256 offset + dd),
+// filled by the block once per tile, instead of the scalar unit's hash per
+// draw, whose dependent chain of about ten scalar instructions per draw
+// held back the vector unit.
+template <class R, bool U, bool HT>
 struct sobol_cursor {
     const uint32_t* nib;
     const uint16_t *t0, *t1, *t2;  // dimension 0 of the current offset
     const uint16_t *c0, *c1, *c2;  // the walk position (seek / advance)
+    const uint2* hst;              // HT: the hashed-seed table
     uint32_t bseed, seed_o, local;
     int off;
     __device__ __forceinline__ void init(const uint32_t* tables, uint32_t block_seed, uint32_t local_index) {
@@ -1069,12 +1076,17 @@ struct sobol_cursor {
     // current offset whose reversed unscrambled point is x, from the
     // 24-bit-exact permutation.
     __device__ __forceinline__ uint32_t scrambled_rev(uint32_t x, uint32_t dd) const {
-        const uint32_t hs = mp_hash32(dd ^ seed_o);
         uint32_t y;
-        if constexpr (U)
-            y = mp_lk_low24_su(x, hs);
-        else
-            y = mp_lk_low24(x, hs);
+        if constexpr (HT) {
+            const uint2 h = hst[(uint32_t)off * 256u + dd];
+            y = mp_lk_low24_vv(x, h.x, h.y);
+        } else {
+            const uint32_t hs = mp_hash32(dd ^ seed_o);
+            if constexpr (U)
+                y = mp_lk_low24_su(x, hs);
+            else
+                y = mp_lk_low24(x, hs);
+        }
         return mp_rev32(y);
     }
     // Standard normal of dimension dd of the current offset: inverse normal
@@ -1164,8 +1176,21 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
         const uint64_t tile_base = (uint64_t)tile * bd;
         const uint64_t p = tile_base + tid;
         const bool active = p < q.n_paths;
-        sobol_cursor<R, true> cur;
+        sobol_cursor<R, true, TREE> cur;
         cur.init(nib, qp.base_seed + (uint32_t)(tile_base >> 16) * 0x9E3779B9u, (uint32_t)(p & 0xFFFFu));
+        if constexpr (TREE) {
+            // the hashed seeds of every dimension the tile draws (F per step),
+            // after the stack area's place in LDS; the previous tile's reads
+            // ended at its block_sum_store's barriers
+            uint2* hst = reinterpret_cast<uint2*>(smem + kNibWords);
+            const int ndims = F * n_steps;
+            for (int g = tid; g < ndims; g += bd) {
+                const uint32_t hs = mp_hash32((uint32_t)(g & 255) ^ (cur.bseed + (uint32_t)(g >> 8)));
+                hst[g] = make_uint2(hs, (hs >> 16) | 1u);
+            }
+            __syncthreads();
+            cur.hst = hst;
+        }
 
         // antithetic requests: the unit is a pair, its second path (sa) takes
         // the negated draws -- for the bridge, the negated increments
@@ -2724,8 +2749,11 @@ hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, i
                       const uint32_t* nib, const mp_bbop* plan, double* partial,
                       double* terminal_out, double* payoff_out, int precision, int grid, int block,
                       bool nib_lds, hipStream_t s) {
-    const size_t lds =
-        qmc_lds_bytes(precision, qp.factors, qp.n_slots > 0 ? qp.n_slots : 1, block, nib_lds);
+    size_t lds = qmc_lds_bytes(precision, qp.factors, qp.n_slots > 0 ? qp.n_slots : 1, block, nib_lds);
+    // register-bridge kernels: the hashed-seed table (8 bytes per dimension)
+    // after the Sobol tables
+    if (nib_lds && qp.bridge && qp.tree_levels >= 0)
+        lds = std::max(lds, (size_t)kNibWords * 4 + (size_t)qp.factors * (16u << qp.tree_levels) * 8);
     const dim3 g(grid), b(block);
     const bool euro = payoff == MP_PAY_EUROPEAN;
     if (precision == 64) {
