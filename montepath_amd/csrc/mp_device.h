@@ -313,39 +313,51 @@ __device__ __forceinline__ R mp_acklam(R p) {
 // f64::MIN_POSITIVE; it lies in the tail (log2 0 = -inf), so the test for
 // it costs nothing on the central branch.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float mp_inv_normal_f32(uint32_t fbits) {
+// The parts of mp_inv_normal_f32, for callers that form the central values
+// of several draws before one shared check for the rare tail:
+// mp_invn_central(fbits, x, l) returns R on the central branch and sets
+// x = 2p - 1 and l = log2(1 - x^2); the draw lies in the tail when
+// !(l > MP_INVN_L_CENTRAL), where mp_invn_tail(fbits, l) gives R instead;
+// the normal is R * x.
+__device__ __forceinline__ float mp_invn_central(uint32_t fbits, float& x, float& l) {
     // fbits = 0x3f800000 | t23 is the float 1 + p, p = t23 * 2^-23
     const float f = __uint_as_float(fbits);
-    const float x = __builtin_fmaf(f, 2.0f, -3.0f);  // 2p - 1, exact
+    x = __builtin_fmaf(f, 2.0f, -3.0f);  // 2p - 1, exact
     // 1 - x^2 = 4p(1-p) exactly; the FMA rounds it once.  l = log2 of it,
     // w = -ln(4p(1-p)) = -ln2 * l (the tail's variable).
-    const float l = __builtin_amdgcn_logf(__builtin_fmaf(-x, x, 1.0f));
-    float r;
-    if (l > MP_INVN_L_CENTRAL) {  // w < 7: a polynomial in l itself
-        r = MP_INVN_C9;
-        r = MP_INVN_C8 + r * l;
-        r = MP_INVN_C7 + r * l;
-        r = MP_INVN_C6 + r * l;
-        r = MP_INVN_C5 + r * l;
-        r = MP_INVN_C4 + r * l;
-        r = MP_INVN_C3 + r * l;
-        r = MP_INVN_C2 + r * l;
-        r = MP_INVN_C1 + r * l;
-        r = MP_INVN_C0 + r * l;
-    } else {
-        const float w = __builtin_amdgcn_sqrtf(-0.69314718f * l) - 3.0f;
-        r = -0.000283145724f;
-        r = 0.000142765653f + r * w;
-        r = 0.00190825947f + r * w;
-        r = -0.00519501232f + r * w;
-        r = 0.00811688974f + r * w;
-        r = -0.0107797878f + r * w;
-        r = 0.0133485785f + r * w;
-        r = 1.41658103f + r * w;
-        r = 4.00643444f + r * w;
-        // p == 0: x = -1, so r * x below is exactly the tail value
-        r = fbits == 0x3f800000u ? 37.51937936814848f : r;
-    }
+    l = __builtin_amdgcn_logf(__builtin_fmaf(-x, x, 1.0f));
+    float r = MP_INVN_C9;  // w < 7: a polynomial in l itself
+    r = MP_INVN_C8 + r * l;
+    r = MP_INVN_C7 + r * l;
+    r = MP_INVN_C6 + r * l;
+    r = MP_INVN_C5 + r * l;
+    r = MP_INVN_C4 + r * l;
+    r = MP_INVN_C3 + r * l;
+    r = MP_INVN_C2 + r * l;
+    r = MP_INVN_C1 + r * l;
+    r = MP_INVN_C0 + r * l;
+    return r;
+}
+
+__device__ __forceinline__ float mp_invn_tail(uint32_t fbits, float l) {
+    const float w = __builtin_amdgcn_sqrtf(-0.69314718f * l) - 3.0f;
+    float r = -0.000283145724f;
+    r = 0.000142765653f + r * w;
+    r = 0.00190825947f + r * w;
+    r = -0.00519501232f + r * w;
+    r = 0.00811688974f + r * w;
+    r = -0.0107797878f + r * w;
+    r = 0.0133485785f + r * w;
+    r = 1.41658103f + r * w;
+    r = 4.00643444f + r * w;
+    // p == 0: x = -1, so r * x is exactly the tail value
+    return fbits == 0x3f800000u ? 37.51937936814848f : r;
+}
+
+__device__ __forceinline__ float mp_inv_normal_f32(uint32_t fbits) {
+    float x, l;
+    float r = mp_invn_central(fbits, x, l);
+    if (!(l > MP_INVN_L_CENTRAL)) r = mp_invn_tail(fbits, l);
     return r * x;
 }
 

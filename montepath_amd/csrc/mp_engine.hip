@@ -1099,6 +1099,10 @@ struct sobol_cursor {
             return mp_acklam<R>(R(scrambled_rev(x, dd) >> 9) * R(0x1p-23));
         }
     }
+    // The bits of the float 1 + p, p the 23-bit uniform of dimension dd.
+    __device__ __forceinline__ uint32_t uniform_bits(uint32_t x, uint32_t dd) const {
+        return __builtin_amdgcn_alignbit(0x7fu, scrambled_rev(x, dd), 9);
+    }
     __device__ __forceinline__ R normal_at(uint32_t dd) const { return normal_of(xrev(dd), dd); }
     // Dimension dd, which is the walk position + j.
     __device__ __forceinline__ R normal_rel(uint32_t dd, int j) const { return normal_of(xrev_rel(j), dd); }
@@ -1345,14 +1349,37 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                 if (dims + 15 * F <= 256) {
                     auto fsplit = [&](int K, const R (&d)[F], R (&l)[F], R (&r)[F]) {
                         const mp_bbop& op = plan[k + K];
+                        R z[F];
+                        if constexpr (sizeof(R) == 4) {
+                            // the F draws' central values, then one check
+                            // for the rare tail (one branch per split: the
+                            // straight-line code between branches is what
+                            // lets draws and steps interleave)
+                            uint32_t fb[F];
+                            float xs[F], ls[F];
+                            bool tail = false;
 #pragma unroll
-                        for (int f = 0; f < F; ++f) {
-                            const R z = cur.normal_of(cur.xrev_rel(K * F + f), (uint32_t)(dims + K * F + f));
-                            if constexpr (sizeof(R) == 4) {
-                                l[f] = __builtin_fmaf(op.sdf_e, z, d[f]);
-                                r[f] = __builtin_fmaf(-op.sdf_e, z, d[f]);
-                            } else {
-                                l[f] = op.rw * d[f] + op.sd * z;
+                            for (int f = 0; f < F; ++f) {
+                                fb[f] = cur.uniform_bits(cur.xrev_rel(K * F + f), (uint32_t)(dims + K * F + f));
+                                z[f] = mp_invn_central(fb[f], xs[f], ls[f]);
+                                tail = tail || !(ls[f] > MP_INVN_L_CENTRAL);
+                            }
+                            if (tail) {
+#pragma unroll
+                                for (int f = 0; f < F; ++f)
+                                    if (!(ls[f] > MP_INVN_L_CENTRAL)) z[f] = mp_invn_tail(fb[f], ls[f]);
+                            }
+#pragma unroll
+                            for (int f = 0; f < F; ++f) {
+                                z[f] = z[f] * xs[f];
+                                l[f] = __builtin_fmaf(op.sdf_e, z[f], d[f]);
+                                r[f] = __builtin_fmaf(-op.sdf_e, z[f], d[f]);
+                            }
+                        } else {
+#pragma unroll
+                            for (int f = 0; f < F; ++f) {
+                                z[f] = cur.normal_of(cur.xrev_rel(K * F + f), (uint32_t)(dims + K * F + f));
+                                l[f] = op.rw * d[f] + op.sd * z[f];
                                 r[f] = d[f] - l[f];
                             }
                         }
