@@ -296,8 +296,10 @@ def measured_traffic() -> dict | None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    # the shader clock ramps up over the first launches (~2.5 ms for the
+    # first, ~1.9 ms from about the tenth on)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip batch and serving")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
