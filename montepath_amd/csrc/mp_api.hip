@@ -313,7 +313,10 @@ bool sobol_index_tables(const uint32_t dirs[256][32], std::vector<uint32_t>& wor
 
 // Streaming Brownian-bridge plan (preorder of the reference's
 // BrownianBridgePlan::new, cpu.rs:5923-5961) with stack slots.
-int bridge_plan(uint64_t n_steps, double maturity, std::vector<mp_bbop>& ops, int& n_slots) {
+// c0, c1: the factor scales of the FP32 European Heston register bridge
+// (xi and rho_perp); K = log2(n_steps) for power-of-two grids (0 otherwise)
+int bridge_plan(uint64_t n_steps, double maturity, std::vector<mp_bbop>& ops, int& n_slots,
+                double c0 = 1.0, double c1 = 1.0, int K = 0) {
     ops.clear();
     n_slots = 1;
     const double dt = maturity / (double)n_steps;
@@ -337,6 +340,8 @@ int bridge_plan(uint64_t n_steps, double maturity, std::vector<mp_bbop>& ops, in
         op.rwf = (float)op.rw;
         op.sdf = (float)op.sd;
         op.sdf_e = (float)std::ldexp(op.sd, it.depth + 1);
+        op.sdf_c0 = (float)(c0 * std::ldexp(op.sd, it.depth + 1 - K));
+        op.sdf_c1 = (float)(c1 * std::ldexp(op.sd, it.depth + 1 - K));
         op.pad_ = 0.f;
         op.slot = it.slot;
         const bool left_leaf = mid <= it.l + 1;
@@ -985,7 +990,15 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
     qp.bridge = P.bridge;
     qp.sqrt_t = std::sqrt(r.t);
     qp.inv_sqrt_dt = 1.0 / std::sqrt(r.t / (double)d.n_steps);
-    if (P.bridge) bridge_plan((uint64_t)d.n_steps, r.t, ops, n_slots);
+    int log2_steps = 0;
+    while ((1ull << (log2_steps + 1)) <= (uint64_t)d.n_steps) ++log2_steps;
+    const int K = (1ull << log2_steps) == (uint64_t)d.n_steps ? log2_steps : 0;
+    qp.root_c0 = r.xi * std::ldexp(std::sqrt(r.t), -K);
+    qp.root_c1 = std::sqrt(std::max(0.0, 1.0 - r.rho * r.rho)) * std::ldexp(std::sqrt(r.t), -K);
+    qp.rho_over_xi = r.xi != 0.0 ? r.rho / r.xi : 0.0;
+    if (P.bridge)
+        bridge_plan((uint64_t)d.n_steps, r.t, ops, n_slots, r.xi,
+                    std::sqrt(std::max(0.0, 1.0 - r.rho * r.rho)), K);
     qp.n_ops = (int)ops.size();
     qp.n_slots = n_slots;
     // bridges on 16 * 2^J steps run in registers when the Sobol tables sit
@@ -1032,7 +1045,10 @@ int run_qmc(mp_ctx* c, const mp_request& r, prepared& P, int precision, mp_resul
         RETCHK(upload(c, c->plan.p, ops.data(), ops.size() * sizeof(mp_bbop)));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     // techniques run in the generic kernel (payoff -1 selects it)
-    const int spec_payoff = d.technique == MP_TECH_STANDARD ? d.payoff : -1;
+    // (a Heston request with xi = 0 too: the European register-bridge
+    // kernels scale the variance factor by xi)
+    const int spec_payoff =
+        d.technique == MP_TECH_STANDARD && !(P.factors == 2 && r.xi == 0.0) ? d.payoff : -1;
     HIPCHK(mp::launch_qmc(c->reqs.as<mp_dreq>(), qp, spec_payoff, d.scheme, c->d_nib, c->plan.as<mp_bbop>(),
                           c->partial.as<double>(), term_host ? c->term.as<double>() : nullptr,
                           pay_host ? c->pay.as<double>() : nullptr, precision, grid, block, nib_lds,

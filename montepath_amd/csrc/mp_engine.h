@@ -83,6 +83,8 @@ struct mp_dreq {
 struct mp_bbop {
     float rwf, sdf;
     float sdf_e;    // FP32 register bridge: 2^(depth + 1) sd, depth of the split interval (root 0)
+    float sdf_c0;   // FP32 European Heston register bridge: factor f carries c_f 2^(depth - K) D,
+    float sdf_c1;   // so a split adds +- c_f 2^(depth + 1 - K) sd z (c_0 = xi, c_1 = rho_perp, 2^K steps)
     float pad_;
     double rw, sd;
     int32_t slot;
@@ -99,6 +101,10 @@ struct mp_qmc_params {
     int32_t pad;
     double sqrt_t;
     double inv_sqrt_dt;
+    // FP32 European Heston register bridge: the root increments' scales
+    // c_f 2^-K sqrt(T) and rho / xi (xi != 0; the host sends xi == 0 to the
+    // generic kernel)
+    double root_c0, root_c1, rho_over_xi;
 };
 
 // Followed in device memory by disc_pow[n_steps+1] (disc_step^s) and

@@ -143,17 +143,19 @@ __device__ __forceinline__ void heston_step_dw(path_state<R>& st, const model_pa
     observe(st, m);
 }
 
-// heston_step_dw for European payoffs: x_T is linear in the per-step rate
-// term, so the caller adds n rate_dt once after the last step and each step
-// is two FMAs on x; the variance update is heston_step_dw's.
-template <class R>
-__device__ __forceinline__ void heston_step_dw_eu(path_state<R>& st, const model_params<R>& m, R dw_var,
-                                                  R dw_ind) {
-    const R vp = mp_relu(st.v);
-    const R sv = mp_sqrt(vp);
-    const R dw_price = m.rho * dw_var + m.rho_perp * dw_ind;
+// The European Heston step of the FP32 register bridge, whose leaves carry
+// the variance factor's increment times xi (u_var = xi dW_var) and the
+// independent factor's times rho_perp (u_ind = rho_perp dW_ind), so the
+// price increment is one FMA and the variance noise needs no multiply by
+// xi.  x_T is linear in the per-step rate term, so the caller adds
+// n rate_dt once after the last step.
+__device__ __forceinline__ void heston_step_u(path_state<float>& st, const model_params<float>& m,
+                                              float rho_over_xi, float u_var, float u_ind) {
+    const float vp = mp_relu(st.v);
+    const float sv = mp_sqrt(vp);
+    const float dw_price = rho_over_xi * u_var + u_ind;  // rho dW_var + rho_perp dW_ind
     st.x = sv * dw_price + (st.x - m.half_dt * vp);
-    R v = st.v + m.kappa_dt * (m.theta - vp) + m.xi * sv * dw_var;
+    float v = (st.v + m.kappa_dt * (m.theta - vp)) + sv * u_var;
     if (m.scheme == 0) v = mp_relu(v);
     st.v = v;
 }
@@ -1212,6 +1214,10 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
         // FP32 European register-bridge kernels leave the per-step drift /
         // rate term out of the steps and add it once after the last
         constexpr bool kDriftOnce = kInc && TREE && PAY == MP_PAY_EUROPEAN;
+        // FP32 European Heston register bridge: factor 0 carries xi dW and
+        // factor 1 rho_perp dW (scaled in the plan), see heston_step_u
+        constexpr bool kFactorScaled = kDriftOnce && F == 2;
+        const R rho_over_xi = kFactorScaled ? mp_param<R>(qp.rho_over_xi) : R(0);
         auto step1 = [&](path_state<R>& x, R i0, R i1) {
             if constexpr (F == 1) {
                 if constexpr (kDriftOnce)
@@ -1221,8 +1227,8 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                 else
                     gbm_step(x, ms, i0);
             } else {
-                if constexpr (kDriftOnce)
-                    heston_step_dw_eu(x, ms, i0, i1);
+                if constexpr (kFactorScaled)
+                    heston_step_u(x, ms, rho_over_xi, i0, i1);
                 else if constexpr (kInc)
                     heston_step_dw(x, ms, i0, i1);
                 else
@@ -1268,13 +1274,20 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
             // (the stack kernel forms the right half as D - left).
             const int J = qp.tree_levels;
             int k = 0, dims = 0, o = 0;
-            if constexpr (sizeof(R) == 4) {
+            if constexpr (sizeof(R) == 4 && !kFactorScaled) {
                 const R leaf_scale = R(1) / R(16 << J);  // a power of two
                 ms.sigma *= leaf_scale;
                 ms.rho *= leaf_scale;
                 ms.rho_perp *= leaf_scale;
                 ms.xi *= leaf_scale;
             }
+            // a split's scale per factor
+            auto split_scale = [&](const mp_bbop& op, int f) -> float {
+                if constexpr (kFactorScaled)
+                    return f == 0 ? op.sdf_c0 : op.sdf_c1;
+                else
+                    return op.sdf_e;
+            };
             auto split = [&](const R (&d)[F], R (&l)[F], R (&r)[F]) {
                 if (dims == 256) {  // next 256-dimension offset
                     ++o;
@@ -1286,8 +1299,9 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                 for (int f = 0; f < F; ++f) {
                     const R z = cur.normal_rel((uint32_t)(dims + f), f);
                     if constexpr (sizeof(R) == 4) {
-                        l[f] = __builtin_fmaf(op.sdf_e, z, d[f]);
-                        r[f] = __builtin_fmaf(-op.sdf_e, z, d[f]);
+                        const float se = split_scale(op, f);
+                        l[f] = __builtin_fmaf(se, z, d[f]);
+                        r[f] = __builtin_fmaf(-se, z, d[f]);
                     } else {
                         l[f] = op.rw * d[f] + op.sd * z;
                         r[f] = d[f] - l[f];
@@ -1317,8 +1331,16 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
                 pair2(b);
             };
             R cur_d[F], pending[MP_TREE_MAX_LEVELS][F];  // pending[l - 1]: level l
+            R root[F];
 #pragma unroll
-            for (int f = 0; f < F; ++f) cur_d[f] = sqrt_t * cur.normal_rel((uint32_t)f, f);
+            for (int f = 0; f < F; ++f) {
+                if constexpr (kFactorScaled)
+                    root[f] = mp_param<R>(f == 0 ? qp.root_c0 : qp.root_c1);
+                else
+                    root[f] = sqrt_t;
+            }
+#pragma unroll
+            for (int f = 0; f < F; ++f) cur_d[f] = root[f] * cur.normal_rel((uint32_t)f, f);
             cur.advance(F);
             dims = F;
             for (int sub = 0; sub < (1 << J); ++sub) {
@@ -1372,8 +1394,9 @@ __device__ __forceinline__ void qmc_body(const mp_dreq* __restrict__ reqp, const
 #pragma unroll
                             for (int f = 0; f < F; ++f) {
                                 z[f] = z[f] * xs[f];
-                                l[f] = __builtin_fmaf(op.sdf_e, z[f], d[f]);
-                                r[f] = __builtin_fmaf(-op.sdf_e, z[f], d[f]);
+                                const float se = split_scale(op, f);
+                                l[f] = __builtin_fmaf(se, z[f], d[f]);
+                                r[f] = __builtin_fmaf(-se, z[f], d[f]);
                             }
                         } else {
 #pragma unroll
