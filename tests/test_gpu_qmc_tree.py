@@ -72,3 +72,22 @@ def test_register_bridge_equals_stack_bridge(engine, stack_engine, kind, m, extr
     for pa, pb, sa, sb in zip(a.strike_prices, b.strike_prices, a.strike_stderrs, b.strike_stderrs):
         assert abs(pa - pb) <= 1e-4 * sb + 1e-12, (pa, pb, sb)
         assert abs(sa - sb) <= 1e-4 * sb + 1e-12, (sa, sb)
+
+
+def test_heston_xi_zero_on_the_register_bridge(engine):
+    # The European register-bridge kernels scale the variance factor by xi,
+    # so xi = 0 is priced by the generic kernel: constant variance v0 = theta
+    # makes the model GBM with sigma = sqrt(v0)
+    import math
+
+    kw = dict(n_paths=1 << 16, n_steps=256, seed=3, kappa=2.0, theta=0.04, v0=0.04, xi=0.0, rho=-0.7,
+              heston_scheme="full_truncation", sampling="scrambled_sobol_brownian_bridge")
+    a, t32, _ = engine.price_with_paths(make_request("heston_european_call", precision=32, **kw))
+    b, t64, _ = engine.price_with_paths(make_request("heston_european_call", precision=64, **kw))
+    assert float(np.max(np.abs(t32 - t64) / t64)) <= 1e-5
+    sd = 0.2
+    d1 = (math.log(100.0 / 100.0) + (0.03 + 0.5 * sd * sd)) / sd
+    ncdf = lambda x: 0.5 * math.erfc(-x / math.sqrt(2.0))  # noqa: E731
+    bs = 100.0 * ncdf(d1) - 100.0 * math.exp(-0.03) * ncdf(d1 - sd)
+    assert abs(b.price - bs) <= 4 * b.stderr + 1e-3, (b.price, bs, b.stderr)
+    assert abs(a.price - b.price) <= 1e-4 * b.stderr + 1e-9
