@@ -1602,9 +1602,14 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
         const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
         const bool active = p < q.n_paths;
         const uint64_t pr = active ? p : q.n_paths - 1;
-        R x[NA], xa[NA];
+        // FP32: log S_T of each asset is linear in the draws, so the steps
+        // only add each asset's normals (zs) and the Cholesky factor, the
+        // volatility and the drift apply once after the last step; FP64
+        // keeps the reference's per-step order
+        constexpr bool kLinear = sizeof(R) == 4;
+        R x[NA], xa[NA], zs[NA];
 #pragma unroll
-        for (int a = 0; a < NA; ++a) x[a] = xa[a] = R(0);
+        for (int a = 0; a < NA; ++a) x[a] = xa[a] = zs[a] = R(0);
         for (int s0i = 0; s0i < n_steps; s0i += 4) {
             const int nsub = min(4, n_steps - s0i);
             R z[NA][4];
@@ -1627,16 +1632,32 @@ __global__ __launch_bounds__(256) void basket_kernel(const mp_dreq* __restrict__
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 if (e < nsub) {
+                    if constexpr (kLinear) {
 #pragma unroll
-                    for (int a = 0; a < NA; ++a) {
-                        R c = R(0);
+                        for (int a = 0; a < NA; ++a) zs[a] += z[a][e];
+                    } else {
 #pragma unroll
-                        for (int b = 0; b <= a; ++b) c += L[a][b] * z[b][e];
-                        x[a] += dr[a] + vo[a] * c;
-                        // the correlated shock of -z is exactly -c
-                        if constexpr (ANTI) xa[a] += dr[a] + vo[a] * -c;
+                        for (int a = 0; a < NA; ++a) {
+                            R c = R(0);
+#pragma unroll
+                            for (int b = 0; b <= a; ++b) c += L[a][b] * z[b][e];
+                            x[a] += dr[a] + vo[a] * c;
+                            // the correlated shock of -z is exactly -c
+                            if constexpr (ANTI) xa[a] += dr[a] + vo[a] * -c;
+                        }
                     }
                 }
+            }
+        }
+        if constexpr (kLinear) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                R c = R(0);
+#pragma unroll
+                for (int b = 0; b <= a; ++b) c += L[a][b] * zs[b];
+                const R drift_total = R(n_steps) * dr[a];
+                x[a] = drift_total + vo[a] * c;
+                if constexpr (ANTI) xa[a] = drift_total + vo[a] * -c;
             }
         }
         R basket = R(0), basket_a = R(0);
