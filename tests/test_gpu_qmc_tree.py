@@ -91,3 +91,38 @@ def test_heston_xi_zero_on_the_register_bridge(engine):
     bs = 100.0 * ncdf(d1) - 100.0 * math.exp(-0.03) * ncdf(d1 - sd)
     assert abs(b.price - bs) <= 4 * b.stderr + 1e-3, (b.price, bs, b.stderr)
     assert abs(a.price - b.price) <= 1e-4 * b.stderr + 1e-9
+
+
+@pytest.mark.parametrize("rho,xi", [(-1.0, 0.5), (1.0, 0.3), (0.2, 0.01)])
+def test_heston_factor_scaled_bridge_edges(engine, rho, xi):
+    # The European register-bridge kernels carry xi dW on the variance
+    # factor and sqrt(1 - rho^2) dW on the independent one: rho = +-1 (the
+    # independent factor's scales are zero) and a small xi (rho / xi
+    # large).  FP32 per path against FP64 on the same draws (measured worst
+    # 4.0e-6, 1.2e-6, 7.8e-7).
+    kw = dict(n_paths=1 << 16, n_steps=256, seed=11, kappa=2.0, theta=0.04, v0=0.04, xi=xi, rho=rho,
+              heston_scheme="full_truncation", sampling="scrambled_sobol_brownian_bridge")
+    print(f"[qmc-tree] heston rho={rho} xi={xi}", flush=True)
+    a, t32, _ = engine.price_with_paths(make_request("heston_european_call", precision=32, **kw))
+    b, t64, _ = engine.price_with_paths(make_request("heston_european_call", precision=64, **kw))
+    err = np.abs(t32 - t64) / t64
+    print(f"[qmc-tree] max {float(err.max()):.3g}", flush=True)
+    assert float(err.max()) <= 1e-5, float(err.max())
+    assert abs(a.price - b.price) <= 1e-4 * b.stderr + 1e-9, (a.price, b.price, b.stderr)
+
+
+def test_heston_ill_conditioned_regime_agrees_on_prices(engine):
+    # xi = 1 (2 kappa theta / xi^2 = 0.16): paths spend steps near v = 0,
+    # where the full-truncation update's derivative xi sqrt(dt) Z / (2 sqrt(v))
+    # is unbounded, so the scheme amplifies any perturbation of the draws;
+    # FP64 arithmetic fed the same increments rounded to FP32 diverges as
+    # much (tests/test_heston_conditioning_cpu.py).  No per-path bar can hold
+    # between the FP32 and FP64 runs here (measured: 18% of paths beyond 1e-5,
+    # worst 0.26); the prices agree (measured 0.04 standard errors apart).
+    kw = dict(n_paths=1 << 16, n_steps=256, seed=11, kappa=2.0, theta=0.04, v0=0.04, xi=1.0, rho=-0.7,
+              heston_scheme="full_truncation", sampling="scrambled_sobol_brownian_bridge")
+    print("[qmc-tree] heston xi=1.0 prices", flush=True)
+    a = engine.price(make_request("heston_european_call", precision=32, **kw))
+    b = engine.price(make_request("heston_european_call", precision=64, **kw))
+    assert abs(a.price - b.price) <= 0.2 * b.stderr, (a.price, b.price, b.stderr)
+    assert abs(a.stderr - b.stderr) <= 0.02 * b.stderr, (a.stderr, b.stderr)
