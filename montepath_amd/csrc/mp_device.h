@@ -271,6 +271,28 @@ __device__ __forceinline__ R mp_acklam(R p) {
             d2 = R(2.445134137142996), d3 = R(3.754408661907416);
     const R p_low = R(0.02425);
     const R p_high = R(1) - p_low;
+    if constexpr (sizeof(R) == 8) {
+        // Straight-line FP64: the tail branch (4.85% of uniforms) is taken
+        // by some lane in almost every wave, so every lane forms the tail's
+        // q and both rationals share one Horner evaluation with per-lane
+        // coefficients.  The central denominator has five steps and the
+        // tail's four; a leading zero coefficient gives the tail five (the
+        // first FMA returns d0 exactly), so each lane performs its branch's
+        // operations in its branch's order: the same bits as the branches.
+        const bool central = p >= p_low && p <= p_high;
+        const bool lower = p < p_low;
+        const R tail = lower ? p : R(1) - p;
+        const R lnp = mp_log(tail > R(2.2250738585072014e-308) ? tail : R(2.2250738585072014e-308));
+        const R qt = mp_sqrt(R(-2) * lnp);
+        const R qc = p - R(0.5);
+        const R t = central ? qc * qc : qt;
+        const R num = (((((central ? a0 : c0) * t + (central ? a1 : c1)) * t + (central ? a2 : c2)) * t +
+                        (central ? a3 : c3)) * t + (central ? a4 : c4)) * t + (central ? a5 : c5);
+        const R den = (((((central ? b0 : R(0)) * t + (central ? b1 : d0)) * t + (central ? b2 : d1)) * t +
+                        (central ? b3 : d2)) * t + (central ? b4 : d3)) * t + R(1);
+        const R z = (central ? num * qc : num) * mp_rcp(den);
+        return central || lower ? z : -z;
+    }
     if (p >= p_low && p <= p_high) {
         const R q = p - R(0.5);
         const R r = q * q;

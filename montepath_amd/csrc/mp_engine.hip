@@ -2828,8 +2828,16 @@ hipError_t launch_qmc(const mp_dreq* req, const mp_qmc_params& qp, int payoff, i
     const dim3 g(grid), b(block);
     const bool euro = payoff == MP_PAY_EUROPEAN;
     if (precision == 64) {
-        if (qp.factors == 2)
+        // payoff and scheme compiled in as for FP32; FP64 arithmetic is the
+        // same in every specialisation (the same bits as the generic body)
+        if (qp.factors == 2 && euro && scheme == 1)
+            qmc_launch_nl<double, 2, MP_PAY_EUROPEAN, 1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+        else if (qp.factors == 2 && euro && scheme == 0)
+            qmc_launch_nl<double, 2, MP_PAY_EUROPEAN, 0>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+        else if (qp.factors == 2)
             qmc_launch_nl<double, 2, -1, -1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
+        else if (euro)
+            qmc_launch_nl<double, 1, MP_PAY_EUROPEAN, -1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
         else
             qmc_launch_nl<double, 1, -1, -1>(nib_lds, g, b, lds, s, req, qp, nib, plan, partial, terminal_out, payoff_out);
     } else if (qp.factors == 2) {
