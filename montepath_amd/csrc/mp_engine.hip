@@ -2598,36 +2598,53 @@ __global__ __launch_bounds__(256) void plain_fp64_kernel(const mp_dreq* __restri
 // per path, 256 per group; body kernels/european_call_stepwise_v1.cu:1-28).
 // One lane per path as the contract fixes, but the block's 256 rows are
 // read as tiles of 256 paths x 16 steps: consecutive lanes load consecutive
-// addresses of a row (128-byte runs), the tile goes through LDS (rows padded
-// by one word against bank conflicts), and each lane then walks its own row.
-// Per path: log S += drift_dt + vol_dt * z in step order, then
-// (S_T - K) * discount if S_T > K, else 0.
+// addresses of a row (128-byte runs), four tiles (64 steps) are loaded into
+// registers at once so that one HBM round trip covers them, and each tile
+// then goes through LDS (rows padded by one word against bank conflicts)
+// for each lane to walk its own row.  Per path: log S += drift_dt +
+// vol_dt * z in step order, then (S_T - K) * discount if S_T > K, else 0.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void european_call_stepwise_v1_kernel(
     const double* __restrict__ normals, double* __restrict__ payoffs, int n_paths, int n_steps,
     double log_s0, double strike, double drift_dt, double vol_dt, double discount) {
-    constexpr int C = 16;
+    constexpr int C = 16;  // steps per tile
+    constexpr int P = 4;   // tiles in flight
     __shared__ double tile[256][C + 1];
     const int t = threadIdx.x;
     const int64_t row0 = (int64_t)blockIdx.x * 256;
     const int64_t path = row0 + t;
     double log_s_t = log_s0;
-    for (int s0 = 0; s0 < n_steps; s0 += C) {
-        const int cols = min(C, n_steps - s0);
-        // all C loads of a thread in flight before the first LDS store
+    for (int s0 = 0; s0 < n_steps; s0 += C * P) {
+        // element (r, c) of tile p is normals[(row0 + r) * n_steps + s0 + p C + c],
+        // thread t holding e = t + 256 i (r = e / C, c = e % C)
+        double v[P][C];
 #pragma unroll
-        for (int i = 0; i < C; ++i) {
-            const int e = t + 256 * i;  // element (r, c) of the tile, c fastest
-            const int r = e / C, c = e % C;
-            const int64_t pr = row0 + r;
-            if (c < cols && pr < n_paths) tile[r][c] = normals[pr * n_steps + s0 + c];
-        }
-        __syncthreads();
-        {
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                const int e = t + 256 * i;
+                const int r = e / C, c = e % C;
+                const int64_t pr = row0 + r;
+                const int step = s0 + p * C + c;
+                v[p][i] = step < n_steps && pr < n_paths ? normals[pr * n_steps + step] : 0.0;
+            }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int base = s0 + p * C;
+            if (base >= n_steps) break;
+            const int cols = min(C, n_steps - base);
+#pragma unroll
+            for (int i = 0; i < C; ++i) {
+                const int e = t + 256 * i;
+                tile[e / C][e % C] = v[p][i];
+            }
+            __syncthreads();
+            {
 #pragma clang fp contract(off)  // each product rounded, as the reference's CPU path computes it
-            for (int c = 0; c < cols; ++c) log_s_t += drift_dt + vol_dt * tile[t][c];
+                for (int c = 0; c < cols; ++c) log_s_t += drift_dt + vol_dt * tile[t][c];
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (path >= n_paths) return;
     const double s_t = exp(log_s_t);
