@@ -1,0 +1,114 @@
+"""Seeded random market parameters and shapes, FP64 on the GPU against the
+oracle on identical draws (per-path bar 1e-12 relative, as the parity tests).
+
+The parity tests fix the market (S0 = K = 100, r = 0.03, sigma = 0.2, T = 1);
+here every case draws its own rate (negative included), volatility, maturity,
+strike, barrier, path count and step count, so that a constant folded in the
+wrong place or a parameter read from the wrong field fails.  Heston cases keep
+xi <= 0.6: far outside the Feller condition the full-truncation scheme is ill
+conditioned in its draws, and the GPU's fused multiply-adds (the oracle rounds
+every product) would be amplified past 1e-12 there
+(tests/test_heston_conditioning_cpu.py).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from montepath_amd.engine import make_request
+
+pytestmark = pytest.mark.gpu
+
+GBM_PAYOFFS = ["european_call", "arithmetic_asian_call", "down_and_out_call", "lookback_call"]
+
+
+def relmax(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def _gbm_cases(k=24, seed=2024):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        out.append(dict(payoff=GBM_PAYOFFS[i % 4], n=int(rng.integers(1, 3000)), m=int(rng.integers(1, 130)),
+                        r=float(rng.uniform(-0.02, 0.12)), sigma=float(rng.uniform(0.01, 0.9)),
+                        t=float(rng.uniform(0.05, 3.0)), k=float(rng.uniform(60.0, 150.0)),
+                        barrier=float(rng.uniform(50.0, 99.0)), seed=1000 + i))
+    return out
+
+
+@pytest.mark.parametrize("case", _gbm_cases(), ids=lambda c: f"{c['payoff']}-n{c['n']}-m{c['m']}")
+def test_gbm_random_parameters(engine, oracle, case):
+    c = case
+    print(f"[random] {c}", flush=True)
+    n, m = c["n"], c["m"]
+    z = oracle.rng_normals(c["seed"], n * m).reshape(n, m)
+    cfg = oracle.Gbm(n_paths=n, n_steps=m, seed=c["seed"], r=c["r"], sigma=c["sigma"], t=c["t"], k=c["k"],
+                     barrier=c["barrier"])
+    term_ref, pay_ref = oracle.gbm_paths_from_normals(cfg, z, c["payoff"])
+    req = make_request(c["payoff"], n_paths=n, n_steps=m, precision=64, r=c["r"], sigma=c["sigma"], t=c["t"],
+                       k=c["k"], barrier=c["barrier"])
+    term, pay = engine.paths_from_normals(req, z.T)
+    assert relmax(term, term_ref) <= 1e-12
+    disc = math.exp(-c["r"] * c["t"])
+    assert np.all(np.abs(pay * disc - pay_ref) <= 1e-12 * (np.abs(pay_ref) + 100.0))
+
+
+def _heston_cases(k=12, seed=77):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        out.append(dict(n=int(rng.integers(1, 2000)), m=int(rng.integers(1, 100)), r=float(rng.uniform(-0.01, 0.1)),
+                        t=float(rng.uniform(0.1, 2.0)), k=float(rng.uniform(70.0, 130.0)),
+                        v0=float(rng.uniform(0.01, 0.2)), kappa=float(rng.uniform(0.3, 5.0)),
+                        theta=float(rng.uniform(0.01, 0.15)), xi=float(rng.uniform(0.05, 0.6)),
+                        rho=float(rng.uniform(-0.95, 0.5)), scheme=["reference_floor", "full_truncation"][i % 2],
+                        seed=2000 + i))
+    return out
+
+
+@pytest.mark.parametrize("case", _heston_cases(), ids=lambda c: f"{c['scheme']}-n{c['n']}-m{c['m']}")
+def test_heston_random_parameters(engine, oracle, case):
+    c = case
+    print(f"[random] heston {c}", flush=True)
+    n, m = c["n"], c["m"]
+    z = oracle.rng_normals(c["seed"], n * m * 2).reshape(n, m, 2)
+    cfg = oracle.Heston(n_paths=n, n_steps=m, r=c["r"], t=c["t"], k=c["k"], v0=c["v0"], kappa=c["kappa"],
+                        theta=c["theta"], xi=c["xi"], rho=c["rho"])
+    term_ref, _ = oracle.heston_paths_from_normals(cfg, z, 0 if c["scheme"] == "reference_floor" else 1)
+    req = make_request("heston_european_call", n_paths=n, n_steps=m, precision=64, r=c["r"], t=c["t"], k=c["k"],
+                       v0=c["v0"], kappa=c["kappa"], theta=c["theta"], xi=c["xi"], rho=c["rho"],
+                       heston_scheme=c["scheme"])
+    term, _ = engine.paths_from_normals(req, np.ascontiguousarray(z.transpose(1, 2, 0)))
+    assert relmax(term, term_ref) <= 1e-12
+
+
+def _merton_cases(k=8, seed=5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        out.append(dict(n=int(rng.integers(1, 3000)), m=[1, 4, 16, 33][i % 4], r=float(rng.uniform(-0.01, 0.1)),
+                        sigma=float(rng.uniform(0.05, 0.5)), t=float(rng.uniform(0.1, 2.0)),
+                        lam=float(rng.uniform(0.0, 8.0)), mu_j=float(rng.uniform(-0.3, 0.2)),
+                        sigma_j=float(rng.uniform(0.0, 0.4)), seed=3000 + i))
+    return out
+
+
+@pytest.mark.parametrize("case", _merton_cases(), ids=lambda c: f"n{c['n']}-m{c['m']}")
+def test_merton_random_parameters(engine, oracle, case):
+    c = case
+    print(f"[random] merton {c}", flush=True)
+    n, m = c["n"], c["m"]
+    rng = np.random.default_rng(c["seed"])
+    z = rng.standard_normal((n, m))
+    u = (rng.integers(0, 1 << 23, size=(n, m)) + 0.5) / float(1 << 23)
+    zj = rng.standard_normal((n, m))
+    cfg = oracle.Merton(n_paths=n, r=c["r"], sigma=c["sigma"], t=c["t"], lam=c["lam"], mu_j=c["mu_j"],
+                        sigma_j=c["sigma_j"])
+    term_ref, _ = oracle.merton_paths_from_draws(cfg, m, z, u, zj)
+    req = make_request("merton_jump_diffusion_call", n_paths=n, n_steps=m, precision=64, r=c["r"],
+                       sigma=c["sigma"], t=c["t"], jump_intensity=c["lam"], jump_mean=c["mu_j"],
+                       jump_volatility=c["sigma_j"])
+    term, _ = engine.paths_from_normals(req, z.T.copy(), u.T.copy(), zj.T.copy())
+    assert relmax(term, term_ref) <= 1e-12
