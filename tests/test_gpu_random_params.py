@@ -112,3 +112,64 @@ def test_merton_random_parameters(engine, oracle, case):
                        jump_volatility=c["sigma_j"])
     term, _ = engine.paths_from_normals(req, z.T.copy(), u.T.copy(), zj.T.copy())
     assert relmax(term, term_ref) <= 1e-12
+
+
+def _qmc_cases(k=16, seed=99):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        m = int(rng.choice([1, 7, 16, 64, 100, 256, 300]))
+        n = int(min(rng.integers(1, 70_000), 3_000_000 // m))
+        out.append(dict(payoff=GBM_PAYOFFS[i % 4], bridge=bool(i % 2), n=n, m=m,
+                        seed=int(rng.integers(0, 1 << 32)), r=float(rng.uniform(-0.02, 0.12)),
+                        sigma=float(rng.uniform(0.05, 0.8)), t=float(rng.uniform(0.1, 2.5)),
+                        k=float(rng.uniform(70.0, 140.0)), barrier=float(rng.uniform(50.0, 99.0))))
+    return out
+
+
+@pytest.mark.parametrize("case", _qmc_cases(),
+                         ids=lambda c: f"{c['payoff']}-bb{int(c['bridge'])}-n{c['n']}-m{c['m']}")
+def test_gbm_sobol_random_parameters(engine, oracle, case):
+    # scrambled Sobol (with and without the Brownian bridge) across the
+    # 2^16-point block and the 256-dimension wrap, random seeds and markets
+    c = case
+    print(f"[random] sobol {c}", flush=True)
+    cfg = oracle.Gbm(n_paths=c["n"], n_steps=c["m"], seed=c["seed"], r=c["r"], sigma=c["sigma"], t=c["t"],
+                     k=c["k"], barrier=c["barrier"])
+    p_ref, _, term_ref = oracle.gbm_qmc_price(cfg, c["payoff"], bridge=c["bridge"])
+    sampling = "scrambled_sobol_brownian_bridge" if c["bridge"] else "scrambled_sobol"
+    req = make_request(c["payoff"], n_paths=c["n"], n_steps=c["m"], seed=c["seed"], precision=64, r=c["r"],
+                       sigma=c["sigma"], t=c["t"], k=c["k"], barrier=c["barrier"], sampling=sampling)
+    res, term, _ = engine.price_with_paths(req)
+    assert relmax(term, term_ref) <= 1e-12
+    assert abs(res.price - p_ref) <= 1e-12 * (abs(p_ref) + 1.0)
+
+
+def _heston_qmc_cases(k=8, seed=123):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        m = int(rng.choice([16, 32, 64, 100, 128, 256]))
+        out.append(dict(n=int(rng.integers(1, 4000)), m=m, seed=int(rng.integers(0, 1 << 32)),
+                        r=float(rng.uniform(-0.01, 0.1)), t=float(rng.uniform(0.1, 2.0)),
+                        k=float(rng.uniform(70.0, 130.0)), v0=float(rng.uniform(0.01, 0.2)),
+                        kappa=float(rng.uniform(0.3, 5.0)), theta=float(rng.uniform(0.01, 0.15)),
+                        xi=float(rng.uniform(0.05, 0.6)), rho=float(rng.uniform(-0.95, 0.5))))
+    return out
+
+
+@pytest.mark.parametrize("case", _heston_qmc_cases(), ids=lambda c: f"n{c['n']}-m{c['m']}")
+def test_heston_sobol_bridge_random_parameters(engine, oracle, case):
+    # two-factor Sobol + bridge (register bridge for 16 * 2^J steps, LDS
+    # stack otherwise), full truncation, random seeds and markets
+    c = case
+    print(f"[random] heston sobol {c}", flush=True)
+    cfg = oracle.Heston(n_paths=c["n"], n_steps=c["m"], seed=c["seed"], r=c["r"], t=c["t"], k=c["k"],
+                        v0=c["v0"], kappa=c["kappa"], theta=c["theta"], xi=c["xi"], rho=c["rho"])
+    term_ref = oracle.heston_qmc_paths(cfg, 1, path0=0)
+    req = make_request("heston_european_call", n_paths=c["n"], n_steps=c["m"], seed=c["seed"], precision=64,
+                       r=c["r"], t=c["t"], k=c["k"], v0=c["v0"], kappa=c["kappa"], theta=c["theta"],
+                       xi=c["xi"], rho=c["rho"], heston_scheme="full_truncation",
+                       sampling="scrambled_sobol_brownian_bridge")
+    _, term, _ = engine.price_with_paths(req)
+    assert relmax(term, term_ref) <= 1e-12
