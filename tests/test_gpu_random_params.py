@@ -173,3 +173,46 @@ def test_heston_sobol_bridge_random_parameters(engine, oracle, case):
                        sampling="scrambled_sobol_brownian_bridge")
     _, term, _ = engine.price_with_paths(req)
     assert relmax(term, term_ref) <= 1e-12
+
+
+def _lsm_cases(k=10, seed=31):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        m = int(rng.integers(2, 41))
+        sched = None
+        if i % 3 == 2:
+            pick = rng.choice(np.arange(1, m), size=min(m - 1, int(rng.integers(1, 8))), replace=False)
+            sched = sorted(int(s) for s in pick) + [m]
+        out.append(dict(n=int(rng.integers(500, 20_000)), m=m, seed=int(rng.integers(1, 1 << 31)),
+                        r=float(rng.uniform(0.0, 0.1)), sigma=float(rng.uniform(0.1, 0.5)),
+                        t=float(rng.uniform(0.25, 2.0)), k=float(rng.uniform(80.0, 120.0)),
+                        degree=int(rng.integers(1, 4)), weighted=bool(i % 4 == 3), sched=sched))
+    return out
+
+
+@pytest.mark.parametrize("case", _lsm_cases(),
+                         ids=lambda c: f"n{c['n']}-m{c['m']}-d{c['degree']}-{'berm' if c['sched'] else 'amer'}")
+def test_lsm_random_parameters(engine, oracle, case):
+    # FP64 Longstaff-Schwartz on identical normals: random markets, basis
+    # degree 1-3, both bases, American and Bermudan schedules.  The Gram
+    # solves round differently from the oracle's (pivot reciprocal, FMA), so
+    # a path whose exercise test is a tie at that rounding may take another
+    # date; each such path moves the price by at most K / n.
+    c = case
+    print(f"[random] lsm {c}", flush=True)
+    n, m = c["n"], c["m"]
+    cfg = oracle.Lsm(n_paths=n, n_steps=m, seed=c["seed"], r=c["r"], sigma=c["sigma"], t=c["t"], k=c["k"],
+                     basis_degree=c["degree"], basis_weighted=c["weighted"])
+    z = oracle.rng_normals(c["seed"], n * m).reshape(n, m)
+    paths = oracle.lsm_paths_from_normals(cfg, z)
+    ref = oracle.lsm_from_paths(cfg, paths, schedule=c["sched"], want_detail=True)
+    kind = "bermudan_put" if c["sched"] else "american_put"
+    extra = {"exercise_steps": c["sched"]} if c["sched"] else {}
+    req = make_request(kind, n_paths=n, n_steps=m, precision=64, r=c["r"], sigma=c["sigma"], t=c["t"], k=c["k"],
+                       basis_degree=c["degree"], basis_weighted=c["weighted"], **extra)
+    res, _, ex = engine.lsm_from_normals(req, z.T.copy())
+    assert res.regression_steps == ref["regression_steps"]
+    flips = int(np.sum(ex != ref["exercise_steps"]))
+    assert flips <= max(2, int(1e-4 * n)), flips
+    assert abs(res.price - ref["price"]) <= 1e-10 * ref["price"] + flips * c["k"] / n
