@@ -216,3 +216,70 @@ def test_lsm_random_parameters(engine, oracle, case):
     flips = int(np.sum(ex != ref["exercise_steps"]))
     assert flips <= max(2, int(1e-4 * n)), flips
     assert abs(res.price - ref["price"]) <= 1e-10 * ref["price"] + flips * c["k"] / n
+
+
+def _basket_cases(k=10, seed=17):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        na = int(rng.integers(1, 9))
+        f = rng.standard_normal((na, na + 2))
+        c = f @ f.T
+        d = np.sqrt(np.diag(c))
+        corr = c / np.outer(d, d)
+        np.fill_diagonal(corr, 1.0)
+        out.append(dict(na=na, n=int(rng.integers(1, 3000)), m=int(rng.integers(1, 41)),
+                        s0=[float(x) for x in rng.uniform(60.0, 140.0, na)],
+                        sigma=[float(x) for x in rng.uniform(0.05, 0.6, na)],
+                        w=[float(x) for x in rng.uniform(0.05, 1.0, na)], corr=corr,
+                        r=float(rng.uniform(-0.01, 0.1)), t=float(rng.uniform(0.1, 2.5)),
+                        k=float(rng.uniform(70.0, 130.0)), seed=4000 + i))
+    return out
+
+
+@pytest.mark.parametrize("case", _basket_cases(), ids=lambda c: f"a{c['na']}-n{c['n']}-m{c['m']}")
+def test_basket_random_parameters(engine, oracle, case):
+    # random assets (1-8), weights, correlation matrices (random factor
+    # loadings) and markets; FP64 basket values per path against the oracle
+    c = case
+    print(f"[random] basket a={c['na']} n={c['n']} m={c['m']}", flush=True)
+    n, m, na = c["n"], c["m"], c["na"]
+    z = oracle.rng_normals(c["seed"], n * m * na).reshape(n, m, na)
+    ref = oracle.basket_paths_from_normals(c["s0"], c["sigma"], c["w"], c["corr"], c["r"], c["t"], m, n, z)
+    req = make_request("basket_call", n_paths=n, n_steps=m, precision=64, r=c["r"], t=c["t"], k=c["k"],
+                       asset_s0=c["s0"], asset_sigma=c["sigma"], asset_weight=c["w"], correlation=c["corr"])
+    term, _ = engine.paths_from_normals(req, np.ascontiguousarray(z.transpose(1, 2, 0)))
+    assert relmax(term, ref) <= 1e-12
+
+
+SAMPLINGS = ["randomized_halton", "latin_hypercube", "scrambled_sobol", "scrambled_sobol_brownian_bridge"]
+TECHNIQUES = ["standard", "antithetic", "control_variate"]
+
+
+def _structured_cases(k=16, seed=8):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        out.append(dict(payoff=GBM_PAYOFFS[i % 4], sampling=SAMPLINGS[(i // 4) % 4], technique=TECHNIQUES[i % 3],
+                        n=int(rng.integers(2, 6000)), m=int(rng.integers(1, 60)), seed=int(rng.integers(1, 1 << 31)),
+                        r=float(rng.uniform(-0.01, 0.1)), sigma=float(rng.uniform(0.05, 0.7)),
+                        t=float(rng.uniform(0.1, 2.5)), k=float(rng.uniform(75.0, 130.0)),
+                        barrier=float(rng.uniform(55.0, 95.0))))
+    return out
+
+
+@pytest.mark.parametrize("case", _structured_cases(),
+                         ids=lambda c: f"{c['payoff']}-{c['sampling']}-{c['technique']}-n{c['n']}-m{c['m']}")
+def test_structured_samplers_random_parameters(engine, oracle, case):
+    # every structured sampler with every technique at random markets and
+    # shapes: FP64 price and standard error against the oracle's pricers
+    c = case
+    print(f"[random] structured {c}", flush=True)
+    cfg = oracle.Gbm(n_paths=c["n"], n_steps=c["m"], seed=c["seed"], r=c["r"], sigma=c["sigma"], t=c["t"],
+                     k=c["k"], barrier=c["barrier"])
+    p_ref, se_ref, _ = oracle.gbm_structured_price(cfg, c["payoff"], c["sampling"], c["technique"])
+    res = engine.price(make_request(c["payoff"], n_paths=c["n"], n_steps=c["m"], seed=c["seed"], precision=64,
+                                    r=c["r"], sigma=c["sigma"], t=c["t"], k=c["k"], barrier=c["barrier"],
+                                    sampling=c["sampling"], technique=c["technique"]))
+    assert abs(res.price - p_ref) <= 1e-12 * (abs(p_ref) + 1e-3), (res.price, p_ref)
+    assert abs(res.stderr - se_ref) <= 1e-10 * (abs(se_ref) + 1e-6), (res.stderr, se_ref)
