@@ -283,3 +283,32 @@ def test_structured_samplers_random_parameters(engine, oracle, case):
                                     sampling=c["sampling"], technique=c["technique"]))
     assert abs(res.price - p_ref) <= 1e-12 * (abs(p_ref) + 1e-3), (res.price, p_ref)
     assert abs(res.stderr - se_ref) <= 1e-10 * (abs(se_ref) + 1e-6), (res.stderr, se_ref)
+
+
+def _greek_cases(k=8, seed=41):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        out.append(dict(estimator=["pathwise", "likelihood_ratio"][i % 2], n=int(rng.integers(1000, 60_000)),
+                        s0=float(rng.uniform(70.0, 130.0)), k=float(rng.uniform(70.0, 130.0)),
+                        sigma=float(rng.uniform(0.05, 0.6)), t=float(rng.uniform(0.1, 3.0)),
+                        r=float(rng.uniform(-0.01, 0.1)), seed=5000 + i))
+    return out
+
+
+@pytest.mark.parametrize("case", _greek_cases(), ids=lambda c: f"{c['estimator']}-n{c['n']}")
+def test_single_pass_greeks_random_parameters(engine, oracle, case):
+    # pathwise and likelihood-ratio Greeks of the terminal European call at
+    # random markets, FP64 on identical normals against the oracle
+    c = case
+    print(f"[random] greeks {c}", flush=True)
+    z = oracle.rng_normals(c["seed"], c["n"])
+    mk = dict(s0=c["s0"], k=c["k"], sigma=c["sigma"], t=c["t"], r=c["r"])
+    want = oracle.terminal_greeks(oracle.Gbm(n_paths=c["n"], n_steps=1, **mk), c["estimator"], z)
+    req = make_request("european_call", n_paths=c["n"], n_steps=64, precision=64, **mk)
+    _, est = engine.greeks_from_normals(req, c["estimator"], z)
+    assert [e["greek"] for e in est] == list(want)
+    for e in est:
+        v, se = want[e["greek"]]
+        assert abs(e["value"] - v) <= 1e-10 * abs(v) + 1e-12, (e["greek"], e["value"], v)
+        assert abs(e["stderr"] - se) <= 1e-9 * se + 1e-15, (e["greek"], e["stderr"], se)
