@@ -66,7 +66,28 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, out)
     for obj in objs:
         os.remove(obj)
+    build_host(verbose)
     return out
+
+
+HOST = os.path.join(PKG, "host")
+HOST_SELFTEST = os.path.join(LIB_DIR, "backend_selftest")
+
+
+def build_host(verbose: bool = False) -> str:
+    """Compile the C++ host layer (montepath::RocmHipBackend over the C-ABI)
+    into backend_selftest, linked against the library built beside it."""
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "lib")
+    tmp = HOST_SELFTEST + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", HOST,
+           os.path.join(HOST, "rocm_backend.cpp"), os.path.join(HOST, "backend_selftest.cpp"),
+           "-L", LIB_DIR, "-lmontepath_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath-link,{rocm_lib}",
+           "-lpthread", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, HOST_SELFTEST)
+    return HOST_SELFTEST
 
 
 if __name__ == "__main__":
