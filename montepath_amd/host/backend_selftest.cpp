@@ -139,11 +139,30 @@ int gpu_checks() {
     DownAndOutCallConfig d;
     d.n_paths = 100000;
     d.n_steps = 64;
+    HestonEuropeanCallConfig h;
+    h.n_paths = 100000;
+    h.n_steps = 64;
+    MertonJumpDiffusionCallConfig m;
+    m.n_paths = 100000;
+    LookbackCallConfig l;
+    l.n_paths = 100000;
+    l.n_steps = 64;
+    BasketCallConfig bk;
+    bk.n_paths = 100000;
+    AmericanPutConfig am;
+    am.n_paths = 100000;
+    BermudanPutConfig be;
+    be.n_paths = 100000;
     const std::pair<const char*, BackendExecutionInput> inputs[] = {
-        {"european_call", e}, {"arithmetic_asian_call", a}, {"down_and_out_call", d}};
+        {"european_call", e},   {"arithmetic_asian_call", a}, {"down_and_out_call", d},
+        {"heston_european_call", h}, {"merton_jump_diffusion_call", m}, {"lookback_call", l},
+        {"basket_call", bk},    {"american_put", am},         {"bermudan_put", be}};
     for (const auto& kv : inputs) {
+        // an artifact of the input's own shape (step-less inputs match any)
+        ExecutionPlan p2 = plan;
+        p2.n_steps = 64;
         for (const RocmHipBackend* b : {&b32, &b64}) {
-            const CompiledArtifact art2 = b->compile(plan, dev);
+            const CompiledArtifact art2 = b->compile(p2, dev);
             const RunOutput o = b->execute(art2, kv.second);
             const RunOutput o2 = b->execute(art2, kv.second);
             std::printf("{\"check\":\"execute\",\"workload\":%s,\"precision\":%d,\"price\":%.17g,\"std_error\":%.17g,"

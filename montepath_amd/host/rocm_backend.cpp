@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "montepath_hip.h"
 
@@ -123,18 +124,111 @@ mp_request gbm_request(const Cfg& c, int32_t workload, int precision) {
     return r;
 }
 
-mp_request to_request(const BackendExecutionInput& input, int precision) {
+// The request of one execution input.  schedule receives a Bermudan
+// schedule, which the request points into.
+mp_request to_request(const BackendExecutionInput& input, int precision, std::vector<uint32_t>& schedule) {
     if (const auto* e = std::get_if<EuropeanCallConfig>(&input)) return gbm_request(*e, MP_EUROPEAN_CALL, precision);
     if (const auto* a = std::get_if<ArithmeticAsianCallConfig>(&input))
         return gbm_request(*a, MP_ARITHMETIC_ASIAN_CALL, precision);
-    const auto& d = std::get<DownAndOutCallConfig>(input);
-    mp_request r = gbm_request(d, MP_DOWN_AND_OUT_CALL, precision);
-    r.barrier = d.barrier;
+    if (const auto* d = std::get_if<DownAndOutCallConfig>(&input)) {
+        mp_request r = gbm_request(*d, MP_DOWN_AND_OUT_CALL, precision);
+        r.barrier = d->barrier;
+        return r;
+    }
+    if (const auto* l = std::get_if<LookbackCallConfig>(&input)) return gbm_request(*l, MP_LOOKBACK_CALL, precision);
+    mp_request r;
+    std::memset(&r, 0, sizeof(r));
+    r.precision = precision;
+    r.sampling = MP_PSEUDORANDOM;
+    r.technique = MP_TECHNIQUE_STANDARD;
+    r.method = MP_METHOD_STEPWISE;
+    if (const auto* h = std::get_if<HestonEuropeanCallConfig>(&input)) {
+        r.workload = MP_HESTON_EUROPEAN_CALL;
+        r.heston_scheme = MP_HESTON_REFERENCE_FLOOR;
+        r.technique = technique_code(h->technique);
+        r.n_paths = h->n_paths;
+        r.n_steps = h->n_steps;
+        r.seed = h->seed;
+        r.s0 = h->s0;
+        r.k = h->k;
+        r.r = h->r;
+        r.t = h->t;
+        r.v0 = h->v0;
+        r.kappa = h->kappa;
+        r.theta = h->theta;
+        r.xi = h->vol_of_vol;
+        r.rho = h->rho;
+    } else if (const auto* m = std::get_if<MertonJumpDiffusionCallConfig>(&input)) {
+        r.workload = MP_MERTON_JUMP_DIFFUSION_CALL;
+        r.n_paths = m->n_paths;
+        r.n_steps = 0;  // one step to maturity, as the reference's config
+        r.seed = m->seed;
+        r.s0 = m->s0;
+        r.k = m->k;
+        r.r = m->r;
+        r.sigma = m->sigma;
+        r.t = m->t;
+        r.jump_intensity = m->jump_intensity;
+        r.jump_mean = m->jump_mean;
+        r.jump_volatility = m->jump_volatility;
+    } else if (const auto* b = std::get_if<BasketCallConfig>(&input)) {
+        r.workload = MP_BASKET_CALL;
+        r.sampling = sampling_code(b->sampling);
+        r.technique = technique_code(b->technique);
+        r.n_paths = b->n_paths;
+        r.n_steps = 1;  // terminal, as the reference's two-asset basket
+        r.seed = b->seed;
+        r.k = b->k;
+        r.r = b->r;
+        r.t = b->t;
+        r.n_assets = 2;
+        r.asset_s0[0] = b->s01;
+        r.asset_s0[1] = b->s02;
+        r.asset_sigma[0] = b->sigma1;
+        r.asset_sigma[1] = b->sigma2;
+        r.asset_weight[0] = b->weight1;
+        r.asset_weight[1] = b->weight2;
+        r.correlation[0] = 1.0;
+        r.correlation[1] = b->rho;
+        r.correlation[2] = b->rho;
+        r.correlation[3] = 1.0;
+    } else {
+        const bool bermudan = std::holds_alternative<BermudanPutConfig>(input);
+        const AmericanPutConfig* am = std::get_if<AmericanPutConfig>(&input);
+        const BermudanPutConfig* be = std::get_if<BermudanPutConfig>(&input);
+        r.workload = bermudan ? MP_BERMUDAN_PUT : MP_AMERICAN_PUT;
+        r.allow_intrinsic_now = bermudan ? 0 : 1;
+        r.n_paths = bermudan ? be->n_paths : am->n_paths;
+        r.n_steps = bermudan ? be->n_steps : am->n_steps;
+        r.seed = bermudan ? be->seed : am->seed;
+        r.s0 = bermudan ? be->s0 : am->s0;
+        r.k = bermudan ? be->k : am->k;
+        r.r = bermudan ? be->r : am->r;
+        r.sigma = bermudan ? be->sigma : am->sigma;
+        r.t = bermudan ? be->t : am->t;
+        r.basis_degree = static_cast<int32_t>(bermudan ? be->basis_degree : am->basis_degree);
+        if (bermudan) {
+            schedule.clear();
+            for (std::size_t s : be->exercise_steps) schedule.push_back(static_cast<uint32_t>(s));
+            r.n_exercise = static_cast<int32_t>(schedule.size());
+            r.exercise_steps = schedule.data();
+        }
+    }
     return r;
 }
 
-std::pair<std::size_t, std::size_t> shape_of(const BackendExecutionInput& input) {
-    return std::visit([](const auto& c) { return std::make_pair(c.n_paths, c.n_steps); }, input);
+// (n_paths, n_steps) of an input; n_steps is nullopt for the step-less
+// Merton and basket configs
+std::pair<std::size_t, std::optional<std::size_t>> shape_of(const BackendExecutionInput& input) {
+    return std::visit(
+        [](const auto& c) -> std::pair<std::size_t, std::optional<std::size_t>> {
+            using C = std::decay_t<decltype(c)>;
+            if constexpr (std::is_same_v<C, MertonJumpDiffusionCallConfig> || std::is_same_v<C, BasketCallConfig>)
+                return {c.n_paths, std::nullopt};
+            else
+                return {c.n_paths, c.n_steps};
+        },
+        input);
 }
 
 std::uint64_t splitmix64(std::uint64_t x) {
@@ -219,7 +313,8 @@ CostEstimate RocmHipBackend::estimate_cost(const ExecutionPlan& plan, const Devi
     EuropeanCallConfig c;
     c.n_paths = plan.n_paths;
     c.n_steps = plan.n_steps;
-    const mp_request r = to_request(BackendExecutionInput{c}, precision_);
+    std::vector<uint32_t> schedule;
+    const mp_request r = to_request(BackendExecutionInput{c}, precision_, schedule);
     std::uint64_t bytes = 0;
     if (mp_hip_request_footprint(&r, &bytes) == MP_OK) {
         peak_mb = std::ceil(static_cast<double>(bytes) / (1024.0 * 1024.0));
@@ -249,13 +344,15 @@ CompiledArtifact RocmHipBackend::compile(const ExecutionPlan& plan, const Device
 
 RunOutput RocmHipBackend::execute(const CompiledArtifact& artifact, const BackendExecutionInput& input) const {
     if (artifact.backend_id != BackendId::RocmHip) throw BackendError::incompatible_execution_input();
-    // the shape check of the GPU backends' executors (backend/mod.rs:488-490)
+    // the shape check of the GPU backends' executors (backend/mod.rs:488-490);
+    // step-less inputs match on their path count
     const auto shape = shape_of(input);
-    if (shape.first != artifact.n_paths || shape.second != artifact.n_steps)
+    if (shape.first != artifact.n_paths || (shape.second && *shape.second != artifact.n_steps))
         throw BackendError::incompatible_execution_input();
     const int d = device_index(artifact.device_id);
     if (d < 0) throw BackendError::unknown_device(artifact.device_id);
-    const mp_request r = to_request(input, precision_);
+    std::vector<uint32_t> schedule;
+    const mp_request r = to_request(input, precision_, schedule);
     const auto started = std::chrono::steady_clock::now();
     mp_result out;
     std::memset(&out, 0, sizeof(out));

@@ -170,8 +170,71 @@ struct DownAndOutCallConfig {
     SamplingMethod sampling = SamplingMethod::Pseudorandom;
 };
 
-// BackendExecutionInput (backend/mod.rs:179-184)
-using BackendExecutionInput = std::variant<EuropeanCallConfig, ArithmeticAsianCallConfig, DownAndOutCallConfig>;
+// The families the reference prices on the CPU only, as additive execution
+// inputs: HestonEuropeanCallConfig, MertonJumpDiffusionCallConfig,
+// LookbackCallConfig, BasketCallConfig, AmericanPutConfig, BermudanPutConfig
+// (runtime/cpu.rs:1430-1818), reference defaults.  The reference's Heston
+// CPU path floors the variance (the engine's reference_floor scheme); its
+// Merton and basket configs have no step count (one step to maturity).
+struct HestonEuropeanCallConfig {
+    double s0 = 100.0, k = 100.0, r = 0.03, v0 = 0.04, kappa = 1.5, theta = 0.04, vol_of_vol = 0.3, rho = -0.6,
+           t = 1.0;
+    std::size_t n_paths = 100000, n_steps = 252;
+    std::uint64_t seed = 42;
+    std::size_t n_threads = 0;
+    MonteCarloTechnique technique = MonteCarloTechnique::Standard;
+};
+
+struct MertonJumpDiffusionCallConfig {
+    double s0 = 100.0, k = 100.0, r = 0.03, sigma = 0.2, jump_intensity = 0.4, jump_mean = -0.08,
+           jump_volatility = 0.25, t = 1.0;
+    std::size_t n_paths = 100000;
+    std::uint64_t seed = 42;
+    std::size_t n_threads = 0;
+};
+
+struct LookbackCallConfig {
+    double s0 = 100.0, k = 100.0, r = 0.03, sigma = 0.2, t = 1.0;
+    std::size_t n_paths = 100000, n_steps = 252;
+    std::uint64_t seed = 42;
+    std::size_t n_threads = 0;
+    MonteCarloTechnique technique = MonteCarloTechnique::Standard;
+    SamplingMethod sampling = SamplingMethod::Pseudorandom;
+};
+
+struct BasketCallConfig {
+    double s01 = 100.0, s02 = 95.0, k = 100.0, r = 0.03, sigma1 = 0.2, sigma2 = 0.25, rho = 0.35, weight1 = 0.5,
+           weight2 = 0.5, t = 1.0;
+    std::size_t n_paths = 100000;
+    std::uint64_t seed = 42;
+    std::size_t n_threads = 0;
+    MonteCarloTechnique technique = MonteCarloTechnique::Standard;
+    SamplingMethod sampling = SamplingMethod::Pseudorandom;
+};
+
+struct AmericanPutConfig {
+    double s0 = 100.0, k = 100.0, r = 0.03, sigma = 0.2, t = 1.0;
+    std::size_t n_paths = 100000, n_steps = 64;
+    std::uint64_t seed = 42;
+    std::size_t n_threads = 0;
+    std::size_t basis_degree = 2;
+};
+
+struct BermudanPutConfig {
+    double s0 = 100.0, k = 100.0, r = 0.03, sigma = 0.2, t = 1.0;
+    std::size_t n_paths = 100000, n_steps = 64;
+    std::uint64_t seed = 42;
+    std::size_t n_threads = 0;
+    std::size_t basis_degree = 2;
+    std::vector<std::size_t> exercise_steps{16, 32, 48, 64};
+};
+
+// BackendExecutionInput (backend/mod.rs:179-184), the reference's three
+// variants first, then the additive ones
+using BackendExecutionInput =
+    std::variant<EuropeanCallConfig, ArithmeticAsianCallConfig, DownAndOutCallConfig, HestonEuropeanCallConfig,
+                 MertonJumpDiffusionCallConfig, LookbackCallConfig, BasketCallConfig, AmericanPutConfig,
+                 BermudanPutConfig>;
 
 // BackendError (backend/mod.rs:186-194): the three variants and their messages.
 class BackendError : public std::runtime_error {
@@ -201,8 +264,8 @@ class RuntimeBackend {
     virtual ReproSupport reproducibility_capabilities(const DeviceInfo& device) const = 0;
 };
 
-// The rocm_hip backend: devices "rocm:<n>", native execution of the three
-// execution inputs through mp_hip_price (FP32 paths with FP64 sums, or FP64
+// The rocm_hip backend: devices "rocm:<n>", native execution of every
+// execution input through mp_hip_price (FP32 paths with FP64 sums, or FP64
 // throughout), one engine context per thread and device (contexts are
 // single-thread-affine, as the Metal backend's runtime cache is
 // thread-local, backend/metal.rs:52-55).

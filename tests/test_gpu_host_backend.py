@@ -1,6 +1,7 @@
 """The C++ host layer (montepath::RocmHipBackend) on the GPU: discovery,
 support, cost, compile and execution of the reference's three execution
-inputs (backend/mod.rs:179-184) through the C-ABI, in FP32 and FP64.  Each
+inputs (backend/mod.rs:179-184) and of the additive ones for the families
+the reference prices on the CPU only, through the C-ABI, in FP32 and FP64.  Each
 execution must equal the same request priced through the Python engine bit
 for bit, repeat identically, and the European price must sit within 3
 standard errors of Black-Scholes.
@@ -39,12 +40,29 @@ def test_discovery_and_plan(rows):
     assert p["peak_mb"] >= 1 and p["runtime_ms"] > 0
 
 
+# the same requests through the engine: the reference configs' defaults
+# (runtime/cpu.rs:1296-1818) at 100,000 paths and 64 steps
+ENGINE_KW = {
+    "european_call": dict(n_steps=64),
+    "arithmetic_asian_call": dict(n_steps=64),
+    "down_and_out_call": dict(n_steps=64, barrier=80.0),
+    "lookback_call": dict(n_steps=64),
+    "heston_european_call": dict(n_steps=64, v0=0.04, kappa=1.5, theta=0.04, xi=0.3, rho=-0.6,
+                                 heston_scheme="reference_floor"),
+    "merton_jump_diffusion_call": dict(n_steps=0, jump_intensity=0.4, jump_mean=-0.08, jump_volatility=0.25),
+    "basket_call": dict(n_steps=1, asset_s0=[100.0, 95.0], asset_sigma=[0.2, 0.25], asset_weight=[0.5, 0.5],
+                        correlation=[[1.0, 0.35], [0.35, 1.0]]),
+    "american_put": dict(n_steps=64, basis_degree=2),
+    "bermudan_put": dict(n_steps=64, basis_degree=2, exercise_steps=[16, 32, 48, 64]),
+}
+
+
 def test_executions_equal_the_engine(rows, engine):
     ex = [r for r in rows if r["check"] == "execute"]
-    assert len(ex) == 6
+    assert len(ex) == 2 * len(ENGINE_KW)
     for r in ex:
-        res = engine.price(make_request(r["workload"], n_paths=100_000, n_steps=64, seed=42,
-                                        precision=r["precision"]))
+        res = engine.price(make_request(r["workload"], n_paths=100_000, seed=42, precision=r["precision"],
+                                        **ENGINE_KW[r["workload"]]))
         assert (r["price"], r["std_error"]) == (res.price, res.stderr), r
         assert r["repeat_identical"]
     eu = [r for r in ex if r["workload"] == "european_call"]
