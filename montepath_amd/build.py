@@ -82,7 +82,7 @@ def build_host(verbose: bool = False) -> str:
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", HOST,
            os.path.join(HOST, "rocm_backend.cpp"), os.path.join(HOST, "backend_selftest.cpp"),
            "-L", LIB_DIR, "-lmontepath_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath-link,{rocm_lib}",
-           "-lpthread", "-o", tmp]
+           "-pthread", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rocm_backend.hpp"
@@ -170,6 +171,28 @@ int gpu_checks() {
                         q(kv.first).c_str(), b == &b32 ? 32 : 64, o.price, o.std_error, o.runtime_ms,
                         (o.price == o2.price && o.std_error == o2.std_error) ? "true" : "false");
         }
+    }
+    // one context per thread: two threads executing at once each open their
+    // own and give the single-thread result
+    {
+        const RunOutput ref = b32.execute(art, BackendExecutionInput{e});
+        RunOutput got[2] = {};
+        std::string err[2];
+        std::vector<std::thread> th;
+        for (int i = 0; i < 2; ++i)
+            th.emplace_back([&, i] {
+                try {
+                    got[i] = b32.execute(art, BackendExecutionInput{e});
+                } catch (const BackendError& x) {
+                    err[i] = x.what();
+                }
+            });
+        for (auto& t : th) t.join();
+        const bool same = err[0].empty() && err[1].empty() && got[0].price == ref.price &&
+                          got[1].price == ref.price && got[0].std_error == ref.std_error &&
+                          got[1].std_error == ref.std_error;
+        std::printf("{\"check\":\"threads\",\"identical\":%s,\"errors\":%s}\n", same ? "true" : "false",
+                    list({err[0], err[1]}).c_str());
     }
     EuropeanCallConfig other = e;
     other.n_paths = 99999;

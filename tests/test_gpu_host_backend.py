@@ -74,3 +74,8 @@ def test_executions_equal_the_engine(rows, engine):
 def test_execution_errors(rows):
     e = [r for r in rows if r["check"] == "execute_errors"][0]
     assert e["shape_mismatch"] == "IncompatibleExecutionInput"
+
+
+def test_threads_each_open_their_own_context(rows):
+    t = [r for r in rows if r["check"] == "threads"][0]
+    assert t["identical"], t
