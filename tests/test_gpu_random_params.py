@@ -312,3 +312,39 @@ def test_single_pass_greeks_random_parameters(engine, oracle, case):
         v, se = want[e["greek"]]
         assert abs(e["value"] - v) <= 1e-10 * abs(v) + 1e-12, (e["greek"], e["value"], v)
         assert abs(e["stderr"] - se) <= 1e-9 * se + 1e-15, (e["greek"], e["stderr"], se)
+
+
+def _mlmc_cases(k=8, seed=61):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        base, ref, level = int(rng.integers(2, 17)), int(rng.integers(2, 5)), int(rng.integers(0, 4))
+        while base * ref ** level > 400:
+            level -= 1
+        out.append(dict(base=base, ref=ref, level=level, paths=int(rng.integers(1, 3000)),
+                        s0=float(rng.uniform(80.0, 120.0)), k=float(rng.uniform(80.0, 120.0)),
+                        r=float(rng.uniform(-0.01, 0.1)), sigma=float(rng.uniform(0.05, 0.6)),
+                        t=float(rng.uniform(0.2, 2.0)), seed=6000 + i))
+    return out
+
+
+@pytest.mark.parametrize("case", _mlmc_cases(), ids=lambda c: f"b{c['base']}-r{c['ref']}-l{c['level']}-n{c['paths']}")
+def test_mlmc_level_terms_random_parameters(engine, oracle, case):
+    # multilevel Asian level terms (fine path and coupled coarse path) at
+    # random markets, base steps, refinement factors and levels, FP64 on
+    # identical normals; level differences are compared on the scale of the
+    # fine payoff sum, as they cancel
+    from montepath_amd.engine import make_mlmc_config
+
+    c = case
+    print(f"[random] mlmc {c}", flush=True)
+    fine = c["base"] * c["ref"] ** c["level"]
+    z = oracle.rng_normals(c["seed"], c["paths"] * fine).reshape(c["paths"], fine)
+    mk = dict(s0=c["s0"], k=c["k"], r=c["r"], sigma=c["sigma"], t=c["t"])
+    cfg_o = oracle.mlmc_config(base_steps=c["base"], levels=4, refinement_factor=c["ref"], **mk)
+    want = oracle.mlmc_level_moments(cfg_o, c["level"], c["paths"], z)
+    cfg = make_mlmc_config(base_steps=c["base"], levels=4, refinement_factor=c["ref"], precision=64, **mk)
+    got = engine.mlmc_level_from_normals(cfg, c["level"], z)
+    scale = abs(want[0]) if c["level"] == 0 else max(abs(want[0]), math.sqrt(want[1] * c["paths"]))
+    assert abs(got[0] - want[0]) <= 1e-12 * scale + 1e-300, (got, want)
+    assert abs(got[1] - want[1]) <= 1e-11 * abs(want[1]) + 1e-300, (got, want)
