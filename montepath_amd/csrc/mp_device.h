@@ -171,6 +171,69 @@ __device__ __forceinline__ double mp_log(double x) { return log(x); }
 __device__ __forceinline__ float mp_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double mp_rcp(double x) { return 1.0 / x; }
 
+// FP64 arithmetic of the FP64 inverse normal, for positive normal
+// arguments of moderate size only (its callers clamp).  The library's log,
+// sqrt and divide spend about 75, 17 and 12 instructions on double-double
+// compensation, range scaling and special cases that these arguments do
+// not need; the rational approximation that follows (Acklam, 1.15e-9
+// relative) could not use the last half-ulp either.
+//
+// n / d: v_rcp_f64 seed, two Newton steps on the reciprocal, then one
+// residual correction of the quotient (Markstein), which leaves it within
+// one rounding of n / d.
+__device__ __forceinline__ double mp_div_f64(double n, double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(e, r, r);
+    e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(e, r, r);
+    const double q = n * r;
+    return __builtin_fma(__builtin_fma(-d, q, n), r, q);
+}
+
+// sqrt x: v_rsq_f64 seed y, then coupled Newton steps on g ~ sqrt x and
+// h ~ 1 / (2 sqrt x), the last two on the residual x - g^2.
+__device__ __forceinline__ double mp_sqrt_pos_f64(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
+// log x: x = 2^e m with m in [sqrt(1/2), sqrt(2)) (m - 1 is exact);
+// log m = 2 atanh(s), s = (m - 1) / (m + 1), |s| < 0.1716, summed through
+// s^21 with the Taylor coefficients 2 / (2k + 1) (truncation below
+// 2.5e-17 relative); e ln 2 is added in two parts, the first with 42
+// significant bits so that e * ln2_hi is exact.
+__device__ __forceinline__ double mp_log_pos_f64(double x) {
+    double m = __builtin_amdgcn_frexp_mant(x);  // [1/2, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    const bool low = m < 0.70710678118654752;
+    m = low ? m + m : m;
+    e = low ? e - 1 : e;
+    const double f = m - 1.0;
+    const double s = mp_div_f64(f, 2.0 + f);
+    const double z = s * s;
+    double p = 2.0 / 21.0;
+    p = __builtin_fma(p, z, 2.0 / 19.0);
+    p = __builtin_fma(p, z, 2.0 / 17.0);
+    p = __builtin_fma(p, z, 2.0 / 15.0);
+    p = __builtin_fma(p, z, 2.0 / 13.0);
+    p = __builtin_fma(p, z, 2.0 / 11.0);
+    p = __builtin_fma(p, z, 2.0 / 9.0);
+    p = __builtin_fma(p, z, 2.0 / 7.0);
+    p = __builtin_fma(p, z, 2.0 / 5.0);
+    p = __builtin_fma(p, z, 2.0 / 3.0);
+    const double lm = __builtin_fma(s * z, p, s + s);
+    const double ed = static_cast<double>(e);
+    return __builtin_fma(ed, 0.6931471805598903, __builtin_fma(ed, 5.497923018708371e-14, lm));
+}
+
 // Compensated (Kahan) running sum in the path precision.
 template <class R>
 struct mp_kahan {
@@ -273,24 +336,24 @@ __device__ __forceinline__ R mp_acklam(R p) {
     const R p_high = R(1) - p_low;
     if constexpr (sizeof(R) == 8) {
         // Straight-line FP64: the tail branch (4.85% of uniforms) is taken
-        // by some lane in almost every wave, so every lane forms the tail's
-        // q and both rationals share one Horner evaluation with per-lane
-        // coefficients.  The central denominator has five steps and the
-        // tail's four; a leading zero coefficient gives the tail five (the
-        // first FMA returns d0 exactly), so each lane performs its branch's
-        // operations in its branch's order: the same bits as the branches.
+        // by some lane in almost every wave, so every lane evaluates both
+        // rationals, each with its branch's operations in its branch's
+        // order, and divides the pair its branch selects.  Two Horner
+        // chains (19 FMAs) issue fewer instructions than one chain with
+        // per-lane coefficient selects (10 FMAs, 22 selects).
         const bool central = p >= p_low && p <= p_high;
         const bool lower = p < p_low;
         const R tail = lower ? p : R(1) - p;
-        const R lnp = mp_log(tail > R(2.2250738585072014e-308) ? tail : R(2.2250738585072014e-308));
-        const R qt = mp_sqrt(R(-2) * lnp);
+        const R q = mp_sqrt_pos_f64(R(-2) * mp_log_pos_f64(tail > R(2.2250738585072014e-308)
+                                                                ? tail
+                                                                : R(2.2250738585072014e-308)));
         const R qc = p - R(0.5);
-        const R t = central ? qc * qc : qt;
-        const R num = (((((central ? a0 : c0) * t + (central ? a1 : c1)) * t + (central ? a2 : c2)) * t +
-                        (central ? a3 : c3)) * t + (central ? a4 : c4)) * t + (central ? a5 : c5);
-        const R den = (((((central ? b0 : R(0)) * t + (central ? b1 : d0)) * t + (central ? b2 : d1)) * t +
-                        (central ? b3 : d2)) * t + (central ? b4 : d3)) * t + R(1);
-        const R z = (central ? num * qc : num) * mp_rcp(den);
+        const R r = qc * qc;
+        const R nc = (((((a0 * r + a1) * r + a2) * r + a3) * r + a4) * r + a5) * qc;
+        const R dc = ((((b0 * r + b1) * r + b2) * r + b3) * r + b4) * r + R(1);
+        const R nt = ((((c0 * q + c1) * q + c2) * q + c3) * q + c4) * q + c5;
+        const R dt = (((d0 * q + d1) * q + d2) * q + d3) * q + R(1);
+        const R z = mp_div_f64(central ? nc : nt, central ? dc : dt);
         return central || lower ? z : -z;
     }
     if (p >= p_low && p <= p_high) {

@@ -76,6 +76,31 @@ def test_sobol_normals_match_oracle(engine, oracle):
         assert float(np.max(np.abs(got[i] - ref))) <= 1e-12
 
 
+def test_sobol_fp64_normals_tight_on_the_tails(engine, oracle):
+    # The FP64 inverse normal's own log, sqrt and divide against the
+    # oracle's libm log and sqrt and IEEE divide, on 4,096 points x 256
+    # dimensions (about 50,000 tail draws: p < 0.02425 or p > 0.97575,
+    # |z| > 1.97296).  The tail rational barely cancels, so what remains
+    # is a few roundings: the GPU fuses the Horner steps into FMAs and the
+    # oracle does not (3 ulp apart at most, by exact arithmetic on the
+    # CPU), and the log is within 1 ulp of libm's.  The central rational
+    # cancels by a factor of ~200 next to the branch points, where fused
+    # and unfused Horner steps alone differ by up to 3.5e-13 relative, so
+    # the whole range keeps the 1e-12 absolute bar.  Measured: tail 9.1e-16
+    # relative (50,857 draws), whole range 6.7e-13 absolute.
+    req = make_request("european_call", seed=5)
+    got = engine.draws(req, 2, 0, 0, 4096, 256)
+    ref = np.stack([oracle.sobol_point_normals(5, i, 256) for i in range(4096)])
+    tail = np.abs(ref) > 1.973
+    rel_tail = float(np.max(np.abs(got[tail] - ref[tail]) / np.abs(ref[tail])))
+    abs_all = float(np.max(np.abs(got - ref)))
+    print(f"[fp64 normals] tail draws {int(tail.sum())}, max rel {rel_tail:.3g}; all max abs {abs_all:.3g}",
+          flush=True)
+    assert int(tail.sum()) > 40000
+    assert rel_tail <= 4e-15
+    assert abs_all <= 1e-12
+
+
 # ------------------------------------------------- per-path arithmetic (GBM)
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("payoff", GBM_PAYOFFS)
