@@ -82,8 +82,9 @@ def test_sobol_fp64_normals_tight_on_the_tails(engine, oracle):
     # dimensions (about 50,000 tail draws: p < 0.02425 or p > 0.97575,
     # |z| > 1.97296).  The tail rational barely cancels, so what remains
     # is a few roundings: the GPU fuses the Horner steps into FMAs and the
-    # oracle does not (3 ulp apart at most, by exact arithmetic on the
-    # CPU), and the log is within 1 ulp of libm's.  The central rational
+    # oracle does not (at most 1.1e-15 relative apart, by exact arithmetic
+    # on the CPU in test_fp64_inverse_normal_cpu.py), and the log is within
+    # 1 ulp of libm's.  The central rational
     # cancels by a factor of ~200 next to the branch points, where fused
     # and unfused Horner steps alone differ by up to 3.5e-13 relative, so
     # the whole range keeps the 1e-12 absolute bar.  Measured: tail 9.1e-16
